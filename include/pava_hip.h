@@ -1,0 +1,244 @@
+/*
+ * pava_hip.h -- C ABI of the MI355X CVR SpMV path ("cvr_hip" format).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns
+ * PAVA_HIP_OK (0) or a negative PAVA_HIP_E* code.  pava_hip_last_error() gives a
+ * message for the most recent failure of the calling host thread.
+ *
+ * Streams are passed as `void*` holding a hipStream_t (NULL = the null stream),
+ * so this header needs no HIP include.
+ *
+ * Reference interfaces replaced (paths relative to the pava source tree):
+ *   pava_coo_matrix / pava_csr_matrix  <- struct _PAVA_COOMatrix / _PAVA_CSRMatrix
+ *                                         utils/basicMatrix.h:38-60
+ *   pava_read_coo_matrix               <- readCOOMatrix       utils/basicMatrix.cpp:63-253
+ *   pava_convert_coo_to_csr            <- convertCOO2CSR      utils/basicMatrix.cpp:350-415
+ *   pava_delete_{coo,csr}_matrix       <- deleteCOOMatrix / deleteCSRMatrix basicMatrix.h:124-125
+ *   pava_init_vectors                  <- initVectors         utils/basicMatrix.cpp:499-520
+ *   pava_reference_spmv                <- referenceSpMV       utils/basicMatrix.cpp:538-556
+ *   pava_check_results                 <- checkResults        utils/basicMatrix.cpp:559-587
+ *   pava_print_performance             <- printPerformance    utils/basicMatrix.cpp:491-495
+ *   pava_microtime                     <- microtime           utils/microtime.h:9-17
+ *   conduct_cvr_hip                    <- conduct_cvr         cvr/cvr.h:1953-2186
+ *   cvr_hip_create / cvr_hip_create_device
+ *                                      <- pre_processing      cvr/cvr.h:583-1060
+ *   cvr_hip_spmv / cvr_hip_spmv_host   <- spmv_compute_kernel cvr/cvr.h:1062-1884
+ *   pava_bench_file                    <- benchFile + conduct_benchmark (CVR block)
+ *                                         main.cpp:61-114, 203-207, 554-571
+ */
+#ifndef PAVA_HIP_H
+#define PAVA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ errors */
+#define PAVA_HIP_OK 0
+#define PAVA_HIP_EINVAL (-1)   /* bad argument or malformed matrix */
+#define PAVA_HIP_ENOMEM (-2)   /* host or device allocation failed */
+#define PAVA_HIP_EDEVICE (-3)  /* HIP runtime error / no device */
+#define PAVA_HIP_EIO (-4)      /* file could not be opened / read / written */
+#define PAVA_HIP_EFORMAT (-5)  /* Matrix Market banner or body not supported */
+#define PAVA_HIP_ERANGE (-6)   /* index out of range or size beyond int32 */
+#define PAVA_HIP_ESTATE (-7)   /* handle used with the wrong precision / API */
+
+/* Reader return codes follow readCOOMatrix (utils/basicMatrix.cpp:84-132):
+ * -1 bad banner, -2 not a matrix, -3 not sparse (coordinate), -4 bad size line,
+ * -5 allocation failure; this build adds -6 (entry out of range / truncated
+ * body) and -7 (file cannot be opened; the reference calls exit(1) there). */
+#define PAVA_MM_EBANNER (-1)
+#define PAVA_MM_ENOTMATRIX (-2)
+#define PAVA_MM_ENOTSPARSE (-3)
+#define PAVA_MM_ESIZE (-4)
+#define PAVA_MM_ENOMEM (-5)
+#define PAVA_MM_EBODY (-6)
+#define PAVA_MM_EOPEN (-7)
+
+/* --------------------------------------------------------- host matrices */
+/* Same fields, order and meaning as _PAVA_COOMatrix / _PAVA_CSRMatrix
+ * (utils/basicMatrix.h:38-60): 0-based int32 indices, FP64 values. Arrays
+ * returned by this library are 64-byte aligned and released by the matching
+ * pava_delete_* call. */
+typedef struct pava_coo_matrix {
+    int nnz;
+    int numRows;
+    int numCols;
+    double *vals;
+    int *cols;
+    int *rows;
+} pava_coo_matrix;
+
+typedef struct pava_csr_matrix {
+    int nnz;
+    int numRows;
+    int numCols;
+    double *vals;
+    int *cols;
+    int *rowOffsets;
+} pava_csr_matrix;
+
+/* Matrix Market (coordinate; real/integer/pattern/complex-real-part;
+ * general/symmetric/skew-symmetric/hermitian) -> COO with 0-based indices,
+ * symmetric halves mirrored. Unlike the reference no padding entries are
+ * added and indices are decremented to 0-based. */
+int pava_read_coo_matrix(const char *path, pava_coo_matrix *out);
+/* COO -> CSR sorted by (row, col); duplicate entries are summed in file order. */
+int pava_convert_coo_to_csr(const pava_coo_matrix *coo, pava_csr_matrix *out);
+/* Convenience: read + convert. */
+int pava_read_csr_matrix(const char *path, pava_csr_matrix *out);
+/* Write a CSR as "%%MatrixMarket matrix coordinate real general", 1-based,
+ * values printed with 17 significant digits (round-trips FP64 exactly). */
+int pava_write_matrix_market(const char *path, const pava_csr_matrix *A);
+void pava_delete_coo_matrix(pava_coo_matrix *m);
+void pava_delete_csr_matrix(pava_csr_matrix *m);
+/* Validate a CSR (monotone offsets, indices in range). */
+int pava_csr_validate(const pava_csr_matrix *A);
+
+/* initVectors: y = 0, y_ref = 0, x[i] = i % 3 (any pointer may be NULL). */
+void pava_init_vectors(int numRows, int numCols, double *x, double *y, double *y_ref);
+/* x[i] ~ U[lo, hi) from a counter-based generator keyed by (seed, i). */
+void pava_fill_uniform(double *x, long long n, unsigned long long seed, double lo, double hi);
+/* The driver's own verification SpMV (main.cpp:318 calls referenceSpMV):
+ * sequential FP64 sum per row with fused multiply-add, OpenMP over rows. */
+void pava_reference_spmv(const pava_csr_matrix *A, const double *x, double *y_ref);
+/* Number of rows with |y - y_ref| > 0.01 |y_ref|; prints "Checking Pass" or
+ * the warning line exactly like checkResults. */
+int pava_check_results(int size, const double *y, const double *y_ref);
+/* " Performance <matrix> <format> <threads> <convert_s> <exec_s> " */
+int pava_print_performance(const char *matrixName, const char *format, int numThreads,
+                           double convertTime, double executionTime);
+double pava_microtime(void);
+
+/* ------------------------------------------------- synthetic generators */
+/* 2D 5-point Laplacian on nx*ny grid: diagonal 4, neighbours -1. */
+int pava_gen_laplacian2d(int nx, int ny, pava_csr_matrix *out);
+/* 3D 27-point stencil on n^3 grid: diagonal 26, off-diagonal -1 + U(-jitter, jitter). */
+int pava_gen_stencil27(int n, double jitter, unsigned long long seed, pava_csr_matrix *out);
+/* R-MAT: 2^scale square, edge_factor*2^scale edges, quadrant probabilities
+ * a,b,c (d = 1-a-b-c), values U(0,1], duplicates summed. No vertex permutation. */
+int pava_gen_rmat(int scale, int edge_factor, double a, double b, double c,
+                  unsigned long long seed, pava_csr_matrix *out);
+
+/* Row-length distributions and column structures of the synthetic families. */
+#define PAVA_RL_CONSTANT 0
+#define PAVA_RL_POISSON 1
+#define PAVA_RL_LOGNORMAL 2
+#define PAVA_RL_POWERLAW 3
+#define PAVA_COL_BANDED 0
+#define PAVA_COL_BLOCKDIAG 1
+#define PAVA_COL_UNIFORM 2
+#define PAVA_COL_MIXED 3 /* band_frac of the entries within +-half_band, rest uniform */
+typedef struct pava_gen_params {
+    int rows;            /* square matrix */
+    int row_dist;        /* PAVA_RL_* */
+    double len_param;    /* mean length (constant / Poisson / lognormal median) or x_min (power law) */
+    double shape;        /* lognormal sigma, or power-law alpha */
+    int col_struct;      /* PAVA_COL_* */
+    int half_band;       /* banded / mixed: columns within row +- half_band */
+    int block;           /* block-diagonal block size */
+    double band_frac;    /* mixed: fraction of banded entries */
+    int value_dist;      /* 0: N(0,1); 1: U(0,1] */
+    int max_len;         /* cap on a row length (<= rows) */
+    unsigned long long seed;
+} pava_gen_params;
+int pava_gen_synthetic(const pava_gen_params *p, pava_csr_matrix *out);
+
+/* ------------------------------------------------------------- CVR-HIP */
+#define CVR_HIP_LANES 64
+#define CVR_HIP_PREC_F64 0   /* FP64 values, x, y; FP64 accumulation */
+#define CVR_HIP_PREC_F32 1   /* FP32 values, x, y; FP32 compensated accumulation */
+#define CVR_HIP_PREC_BF16 2  /* BF16 values, FP32 x, y; FP32 compensated accumulation */
+
+typedef struct cvr_hip_opts {
+    int steps_per_chunk;     /* S: 64-lane steps per chunk; multiple of 8, 8..4096; 0 -> 256 */
+    int precision;           /* CVR_HIP_PREC_* */
+    int col_index_bits;      /* 32, or 16 (chunk-relative offsets; mixed precision only) */
+    int recompute;           /* mixed precision: recompute rows with kappa > kappa_threshold in FP64 */
+    double kappa_threshold;  /* sum|a*x| / |sum a*x| above which a row is recomputed (default 1e3) */
+    int keep_fp64;           /* keep the FP64 CSR resident (required by recompute) */
+    int device;              /* HIP device ordinal */
+} cvr_hip_opts;
+
+typedef struct cvr_hip_info {
+    int numRows, numCols, nnz;
+    int nchunks, steps_per_chunk, precision, col_index_bits;
+    long long layout_elems;  /* 64 * total steps (nnz + padding) */
+    long long seg_capacity;  /* entries allocated for the segment list */
+    int max_slots;           /* largest number of split-segment slots in one chunk */
+    int has_empty_rows;
+    int n_fixup_rows;        /* rows spanning more than one chunk */
+    long long device_bytes;  /* device memory held by the handle */
+    double convert_ms;       /* GPU time of the CSR->CVR conversion (hipEvent) */
+} cvr_hip_info;
+
+typedef struct cvr_hip_handle cvr_hip_handle;
+
+void cvr_hip_default_opts(cvr_hip_opts *o);
+/* Host CSR -> device upload -> GPU CSR->CVR conversion. Synchronous. */
+int cvr_hip_create(const pava_csr_matrix *A, const cvr_hip_opts *opts, cvr_hip_handle **out);
+/* Same from a CSR already in device memory (the arrays are not retained). */
+int cvr_hip_create_device(int numRows, int numCols, int nnz, const int *d_rowOffsets,
+                          const int *d_cols, const double *d_vals, const cvr_hip_opts *opts,
+                          cvr_hip_handle **out);
+int cvr_hip_destroy(cvr_hip_handle *h);
+int cvr_hip_get_info(const cvr_hip_handle *h, cvr_hip_info *info);
+
+/* y = A*x, FP64 handle, device pointers, enqueued on `stream`, asynchronous.
+ * y is fully written (empty rows get 0); no pre-zeroing needed. */
+int cvr_hip_spmv(cvr_hip_handle *h, const double *d_x, double *d_y, void *stream);
+/* Host-pointer variant (copies in and out, synchronous). */
+int cvr_hip_spmv_host(cvr_hip_handle *h, const double *x, double *y);
+/* y = A*x for FP32/BF16 handles: x and y FP32. d_x64 (may be NULL) is the FP64
+ * x used by the FP64 recompute of high-cancellation rows. */
+int cvr_hip_spmv_mixed(cvr_hip_handle *h, const float *d_x32, const double *d_x64, float *d_y,
+                       void *stream);
+/* Y = A*X with k right-hand sides, X (numCols x k) and Y (numRows x k) row-major
+ * FP32, device pointers; 1 <= k <= 32. Any precision of handle. */
+int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *stream);
+/* Number of rows recomputed in FP64 by the last mixed SpMV (synchronises). */
+int cvr_hip_last_recompute_count(cvr_hip_handle *h, int *count);
+
+/* Timing of `iters` back-to-back SpMVs after `warmup`, with hipEvents on the
+ * handle's stream: ms_total = average wall per SpMV (all kernels), ms_main =
+ * average duration of the main SpMV kernel alone (events around it). */
+int cvr_hip_time_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, int warmup, int iters,
+                      double *ms_total, double *ms_main);
+
+/* Copies of the layout for checking (host buffers sized from cvr_hip_info):
+ * vals (layout_elems * value size), cols (layout_elems int32, or uint16 when
+ * col_index_bits == 16), chunk_rows (nchunks+1), seg (seg_capacity),
+ * seg_count (nchunks), split_n (nchunks), split_tab (2*nchunks*CVR_HIP_SPLIT_CAP),
+ * fix (2*nchunks), col_base (nchunks, 16-bit layouts only), end_mask
+ * (layout_elems/64 uint64, 16-bit layouts only). Any pointer may be NULL. */
+#define CVR_HIP_SPLIT_CAP 72
+#define CVR_HIP_SEG_PER_CHUNK_EXTRA 66
+int cvr_hip_export_layout(const cvr_hip_handle *h, void *vals, void *cols, int *chunk_rows,
+                          int *seg, int *seg_count, int *split_n, int *split_tab, int *fix,
+                          int *col_base, uint64_t *end_mask);
+
+/* Drop-in for conduct_cvr (cvr/cvr.h:1953): same arguments; converts (timed),
+ * runs one verification SpMV into refOut, numIterations timed SpMVs, prints the
+ * printPerformance line with format "CVR_HIP". Owns refOut completely (no
+ * pre-zeroing needed, empty rows written). nItems_raw needs no padding; alpha
+ * is accepted and ignored as in the reference. Returns 0 or PAVA_HIP_E*. */
+int conduct_cvr_hip(int numRows, int numCols, int nItems_raw, int *h_rowDelimiters, int *h_cols,
+                    double *h_val, double *h_vec, double *refOut, int alpha, char *matrixName,
+                    int numIterations);
+
+/* benchFile for the CVR block: read, size filters (main.cpp:82-99), x = i%3,
+ * referenceSpMV, conduct_cvr_hip, checkResults. Returns the error count
+ * (>= 0), -1 if the matrix was skipped, or a PAVA_HIP_E* code. */
+int pava_bench_file(const char *path, int numIterations);
+
+const char *pava_hip_last_error(void);
+/* Library version string. */
+const char *pava_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PAVA_HIP_H */

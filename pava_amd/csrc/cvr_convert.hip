@@ -1,0 +1,386 @@
+// cvr_convert.hip -- GPU CSR -> CVR-64 conversion for gfx950.
+//
+// The layout re-derives pava's CVR (cvr/cvr.h:583-1060, pre_processing) for a
+// 64-lane wavefront:
+//   * the nonzeros are cut into chunks of C = 64*S consecutive CSR elements
+//     (pava: per-thread nnz-balanced slices, cvr/cvr.h:602-648); one wave owns
+//     one chunk;
+//   * each of the 64 lanes streams one row at a time; when a lane finishes its
+//     row, the next unassigned non-empty row of the chunk goes to it, lanes
+//     served in ascending order (cvr/cvr.h:839-880);
+//   * once the rows run out, an idle lane takes the LAST R elements of the
+//     lowest lane holding more than R, where R is the number of steps left
+//     (pava takes the first ave = R elements, cvr/cvr.h:909-978); every lane
+//     therefore ends on the chunk's last step;
+//   * values and columns are stored step-major, lane-minor (element (s, l) at
+//     chunk_base + 64*s + l), so each step is one coalesced 512 B (FP64) load;
+//   * the sign bit of a column marks the last element of a lane segment
+//     (replaces pava's record pairs, cvr/cvr.h:861-873, 919-955);
+//   * seg[] lists, in the order segments end (step, then lane), where each
+//     segment's sum goes: a row id, or ~slot for segments of split rows whose
+//     partial sums are reduced at the end of the chunk (pava's t_rets /
+//     final_2 / split arrays, cvr/cvr.h:884-894, 1823-1868).
+// A segment never jumps: a thief's elements are the tail of the victim's
+// remaining run, so every segment is a contiguous CSR range and partial sums
+// of a split row are added in element order.
+//
+// One wavefront simulates its chunk's schedule with ballots: the control is a
+// scan over steps with wave-uniform state, the data movement is batched 8
+// steps at a time so the gathers of a batch are in flight together.
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include "cvr_kernels.h"
+
+namespace cvr {
+
+__device__ __forceinline__ int mbcnt64(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ unsigned short f32_to_bf16_rne(float f) {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40);  // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+template <typename VT>
+__device__ __forceinline__ VT cvt_val(double v);
+template <>
+__device__ __forceinline__ double cvt_val<double>(double v) { return v; }
+template <>
+__device__ __forceinline__ float cvt_val<float>(double v) { return (float)v; }
+template <>
+__device__ __forceinline__ unsigned short cvt_val<unsigned short>(double v) { return f32_to_bf16_rne((float)v); }
+
+// Largest r in [0, M) with rp[r] <= b (b < nnz), i.e. the row holding element b.
+__device__ int row_of_element(const int *rp, int M, int b) {
+    int lo = 0, hi = M;  // invariant: rp[lo] <= b < rp[hi]
+    while (hi - lo > 1) {
+        int mid = lo + ((hi - lo) >> 1);
+        if (rp[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Empty-row bitmap: bit j of word w is set when row 32w+j has no entries.
+__global__ void empty_rows_kernel(const int *__restrict__ rp, int M, unsigned *__restrict__ bits,
+                                  int *__restrict__ status) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    int nw = (M + 31) >> 5;
+    if (w >= nw) return;
+    unsigned m = 0;
+    int r0 = w << 5;
+    for (int j = 0; j < 32; ++j) {
+        int r = r0 + j;
+        if (r < M && rp[r + 1] == rp[r]) m |= 1u << j;
+    }
+    bits[w] = m;
+    if (m) atomicOr(&status[4], 1);
+}
+
+template <typename VT, bool COL16>
+__global__ __launch_bounds__(256) void convert_kernel(ConvertArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int c = blockIdx.x * kWavesPerBlock + wv;
+    if (c >= a.nchunks) return;
+
+    __shared__ int q_start[kWavesPerBlock][kLanes];
+    __shared__ int q_len[kWavesPerBlock][kLanes];
+    __shared__ int q_row[kWavesPerBlock][kLanes];
+    __shared__ int q_kind[kWavesPerBlock][kLanes];
+    __shared__ int sp_start[kWavesPerBlock][kSlotCap];
+    __shared__ int sp_idx[kWavesPerBlock][kSlotCap];
+    __shared__ int sp_row[kWavesPerBlock][kSlotCap];
+    __shared__ int sp_kind[kWavesPerBlock][kSlotCap];
+    __shared__ int srt_row[kWavesPerBlock][kSlotCap];
+    __shared__ int srt_kind[kWavesPerBlock][kSlotCap];
+
+    const int *__restrict__ rp = a.rp;
+    const long long C = 64LL * a.S;
+    const int b = (int)(C * c);
+    const int e = (int)min<long long>((long long)b + C, (long long)a.nnz);
+    const int n = e - b;
+    int steps = a.S;
+    if (c == a.nchunks - 1) {
+        steps = (n + 63) / 64;
+        steps = (steps + kStepGroup - 1) / kStepGroup * kStepGroup;
+    }
+    const int pad = 64 * steps - n;
+
+    const int row_lo = row_of_element(rp, a.M, b);
+    const int zr = (c == 0) ? 0 : row_lo;
+    const int seg_base = zr + kSegExtra * c;
+    if (lane == 0) {
+        a.chunk_rows[c] = zr;
+        if (c == a.nchunks - 1) a.chunk_rows[a.nchunks] = a.M;
+        const bool head = rp[row_lo] < b;
+        const bool tail = rp[row_lo + 1] > e;
+        if (head && !tail) {
+            a.fix[2 * c] = row_lo;
+            a.fix[2 * c + 1] = (int)(rp[row_lo] / C);
+            atomicAdd(&a.status[3], 1);
+        } else {
+            a.fix[2 * c] = -1;
+            a.fix[2 * c + 1] = -1;
+        }
+    }
+
+    // 16-bit layouts: column base = smallest column of the chunk
+    int cbase = 0;
+    if (COL16) {
+        int mn = 0x7fffffff, mx = -1;
+        for (int k = b + lane; k < e; k += 64) {
+            int cc = a.ci[k];
+            mn = min(mn, cc);
+            mx = max(mx, cc);
+        }
+        if (pad > 0) mn = min(mn, 0), mx = max(mx, 0);
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = min(mn, __shfl_xor(mn, off));
+            mx = max(mx, __shfl_xor(mx, off));
+        }
+        cbase = mn;
+        if (lane == 0) {
+            a.col_base[c] = cbase;
+            if (mx - mn > 0xffff) atomicOr(&a.status[2], 1);
+        }
+    }
+
+    // ---------------------------------------------------------- state
+    int cnt = 0, pos = 0, prow = 0, pkind = 0, sstart = 0;
+    bool split = false;
+    int qh = 0, qn = 0;              // piece queue head / size (wave-uniform)
+    int next_row = row_lo;           // next row to scan for pieces
+    bool rows_done = false, pad_done = (pad == 0);
+    int segcount = 0, nsp = 0;
+
+    VT *__restrict__ ov = reinterpret_cast<VT *>(a.ovals) + (size_t)b;
+
+    for (int s0 = 0; s0 < steps; s0 += kStepGroup) {
+        int P[kStepGroup];
+        bool E[kStepGroup];
+#pragma unroll
+        for (int i = 0; i < kStepGroup; ++i) {
+            const int s = s0 + i;
+            // ---- assignment of idle lanes
+            unsigned long long idle = __ballot(cnt == 0);
+            while (idle) {
+                if (qh == qn) {
+                    // refill the piece queue
+                    qh = qn = 0;
+                    while (qn == 0 && !rows_done) {
+                        int r = next_row + lane;
+                        bool inrange = (r < a.M) && (rp[min(r, a.M)] < e);
+                        int st = 0, ln = 0, kd = 0;
+                        if (inrange) {
+                            int r0 = rp[r], r1 = rp[r + 1];
+                            st = max(r0, b);
+                            ln = min(r1, e) - st;
+                            kd = (r0 < b ? kHead : 0) | (r1 > e ? kTail : 0);
+                        }
+                        bool ne = inrange && ln > 0;
+                        unsigned long long m = __ballot(ne);
+                        if (ne) {
+                            int k = mbcnt64(m);
+                            q_start[wv][k] = st;
+                            q_len[wv][k] = ln;
+                            q_row[wv][k] = r;
+                            q_kind[wv][k] = kd;
+                        }
+                        qn = __popcll(m);
+                        unsigned long long in = __ballot(inrange);
+                        if (in != ~0ull) rows_done = true;  // a row at/after the chunk end was seen
+                        next_row += 64;
+                    }
+                    if (qn == 0 && !pad_done) {
+                        if (lane == 0) {
+                            q_start[wv][0] = a.nnz;
+                            q_len[wv][0] = pad;
+                            q_row[wv][0] = -1;
+                            q_kind[wv][0] = kPad;
+                        }
+                        qn = 1;
+                        pad_done = true;
+                    }
+                    wave_lds_sync();
+                    if (qn == 0) break;  // no pieces left: tail
+                }
+                const int avail = qn - qh;
+                const bool me_idle = (idle >> lane) & 1ull;
+                const int rank = mbcnt64(idle);
+                if (me_idle && rank < avail) {
+                    int k = qh + rank;
+                    pos = q_start[wv][k];
+                    cnt = q_len[wv][k];
+                    prow = q_row[wv][k];
+                    pkind = q_kind[wv][k];
+                    split = (pkind != 0);
+                    sstart = pos;
+                }
+                unsigned long long taken = __ballot(me_idle && rank < avail);
+                qh += __popcll(taken);
+                idle &= ~taken;
+            }
+            // ---- tail: steal for the remaining idle lanes, in lane order
+            while (idle) {
+                const int l = __ffsll((long long)idle) - 1;
+                const int R = steps - s;
+                unsigned long long cand = __ballot(cnt > R);
+                const int v = __ffsll((long long)cand) - 1;  // exists: sum(cnt) = 64 R and lane l holds 0
+                const int pv = __shfl(pos, v), cv = __shfl(cnt, v);
+                const int rv = __shfl(prow, v), kv = __shfl(pkind, v);
+                if (lane == l) {
+                    pos = pv + cv - R;
+                    cnt = R;
+                    prow = rv;
+                    pkind = kv;
+                    split = true;
+                    sstart = pos;
+                }
+                if (lane == v) {
+                    cnt = cv - R;
+                    split = true;
+                }
+                idle &= idle - 1;
+            }
+            // ---- emit one element per lane
+            P[i] = pos;
+            E[i] = (cnt == 1);
+            ++pos;
+            --cnt;
+            // ---- segment ends, in lane order
+            const bool ended = (cnt == 0);
+            unsigned long long mend = __ballot(ended);
+            if (mend) {
+                unsigned long long msp = __ballot(ended && split);
+                if (ended) {
+                    int idx = segcount + mbcnt64(mend);
+                    if (!split) {
+                        a.seg[seg_base + idx] = prow;
+                    } else {
+                        int k = nsp + mbcnt64(msp);
+                        if (k < kSlotCap) {
+                            sp_start[wv][k] = sstart;
+                            sp_idx[wv][k] = idx;
+                            sp_row[wv][k] = prow;
+                            sp_kind[wv][k] = pkind;
+                        }
+                    }
+                }
+                segcount += __popcll(mend);
+                nsp += __popcll(msp);
+            }
+        }
+        // ---- data movement for the batch
+        int cv[kStepGroup];
+        double vv[kStepGroup];
+#pragma unroll
+        for (int i = 0; i < kStepGroup; ++i) {
+            const bool real = P[i] < a.nnz;
+            cv[i] = real ? a.ci[P[i]] : 0;
+            vv[i] = real ? reinterpret_cast<const double *>(a.va)[P[i]] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < kStepGroup; ++i) {
+            const int s = s0 + i;
+            ov[(size_t)s * 64 + lane] = cvt_val<VT>(vv[i]);
+            if (COL16) {
+                reinterpret_cast<unsigned short *>(a.ocols)[(size_t)b + (size_t)s * 64 + lane] =
+                    (unsigned short)(cv[i] - cbase);
+                unsigned long long em = __ballot(E[i]);
+                if (lane == 0) a.end_mask[(size_t)b / 64 + s] = em;
+            } else {
+                reinterpret_cast<int *>(a.ocols)[(size_t)b + (size_t)s * 64 + lane] = cv[i] | (E[i] ? kEndFlag : 0);
+            }
+        }
+    }
+
+    // ---------------------------------------------------- finalisation
+    if (nsp > kSlotCap) {
+        if (lane == 0) atomicOr(&a.status[1], 1);
+        return;
+    }
+    if (lane == 0) {
+        a.seg_count[c] = segcount;
+        atomicMax(&a.status[0], nsp);
+    }
+    wave_lds_sync();
+    // rank of each split segment by start position (starts are distinct)
+    for (int k = lane; k < nsp; k += 64) {
+        const int sk = sp_start[wv][k];
+        int rank = 0;
+        for (int j = 0; j < nsp; ++j) rank += (sp_start[wv][j] < sk);
+        a.seg[seg_base + sp_idx[wv][k]] = ~rank;
+        srt_row[wv][rank] = sp_row[wv][k];
+        srt_kind[wv][rank] = sp_kind[wv][k];
+    }
+    wave_lds_sync();
+    int2 *tab = a.split_tab + (size_t)c * kSplitCap;
+    int gbase = 0;
+    for (int i0 = 0; i0 < nsp; i0 += 64) {
+        const int i = i0 + lane;
+        bool newg = false;
+        int code = 0;
+        if (i < nsp) {
+            newg = (i == 0) || (srt_row[wv][i] != srt_row[wv][i - 1]);
+            const int kd = srt_kind[wv][i];
+            code = (kd & kPad) ? kCodeDiscard : (kd & kTail) ? kCodeTail : (kd & kHead) ? kCodeHead : srt_row[wv][i];
+        }
+        unsigned long long mg = __ballot(newg);
+        if (newg) {
+            const int g = gbase + mbcnt64(mg);
+            if (g < kSplitCap - 1) tab[g] = make_int2(code, i);
+        }
+        gbase += __popcll(mg);
+    }
+    if (lane == 0) {
+        if (gbase > kSplitCap - 1) {
+            atomicOr(&a.status[1], 2);
+            gbase = kSplitCap - 1;
+        }
+        tab[gbase] = make_int2(0, nsp);
+        a.split_n[c] = gbase;
+    }
+}
+
+// Host-side launchers --------------------------------------------------------
+hipError_t launch_empty_rows(const int *rp, int M, unsigned *bits, int *status, hipStream_t st) {
+    int nw = (M + 31) >> 5;
+    if (nw == 0) return hipSuccess;
+    hipLaunchKernelGGL(empty_rows_kernel, dim3((nw + 255) / 256), dim3(256), 0, st, rp, M, bits, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(const ConvertArgs &a, hipStream_t st) {
+    if (a.nchunks == 0) return hipSuccess;
+    dim3 grid((a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
+    if (a.col16) {
+        switch (a.val_kind) {
+        case 0: hipLaunchKernelGGL((convert_kernel<double, true>), grid, block, 0, st, a); break;
+        case 1: hipLaunchKernelGGL((convert_kernel<float, true>), grid, block, 0, st, a); break;
+        default: hipLaunchKernelGGL((convert_kernel<unsigned short, true>), grid, block, 0, st, a); break;
+        }
+    } else {
+        switch (a.val_kind) {
+        case 0: hipLaunchKernelGGL((convert_kernel<double, false>), grid, block, 0, st, a); break;
+        case 1: hipLaunchKernelGGL((convert_kernel<float, false>), grid, block, 0, st, a); break;
+        default: hipLaunchKernelGGL((convert_kernel<unsigned short, false>), grid, block, 0, st, a); break;
+        }
+    }
+    return hipGetLastError();
+}
+
+}  // namespace cvr

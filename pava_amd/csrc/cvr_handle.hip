@@ -1,0 +1,475 @@
+// cvr_handle.hip -- the extern "C" CVR-HIP handle API: device upload, GPU
+// CSR->CVR conversion (pava's pre_processing, cvr/cvr.h:583-1060) and the SpMV
+// launches (pava's spmv_compute_kernel, cvr/cvr.h:1062-1884).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "cvr_kernels.h"
+#include "pava_hip.h"
+#include "pava_internal.h"
+
+
+using namespace pava;
+
+struct cvr_hip_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cvr_hip_opts opts{};
+    int M = 0, N = 0, nnz = 0, nchunks = 0, S = 256, prec = 0, col16 = 0;
+    long long layout_elems = 0, seg_cap = 0;
+    int has_empty = 0, n_fix = 0, max_slots = 0;
+    void *d_vals = nullptr, *d_cols = nullptr;
+    uint64_t *d_end_mask = nullptr;
+    int *d_col_base = nullptr;
+    int *d_chunk_rows = nullptr, *d_seg = nullptr, *d_seg_count = nullptr, *d_split_n = nullptr;
+    int2 *d_split_tab = nullptr;
+    int *d_fix = nullptr;
+    double *d_carry = nullptr;
+    unsigned *d_empty = nullptr;
+    int *d_status = nullptr;
+    // FP64 CSR kept resident (recompute); owned by the handle when set
+    int *d_rp = nullptr, *d_ci = nullptr;
+    double *d_va = nullptr;
+    // staging for the host-pointer API
+    void *d_hx = nullptr, *d_hy = nullptr;
+    long long device_bytes = 0;
+    double convert_ms = 0.0;
+    std::vector<void *> allocs;
+};
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t _e = (expr);                                                                         \
+        if (_e != hipSuccess)                                                                           \
+            return set_error(PAVA_HIP_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                             __FILE__, __LINE__);                                                       \
+    } while (0)
+
+static int dev_alloc(cvr_hip_handle *h, void **p, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_error(PAVA_HIP_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    h->allocs.push_back(*p);
+    h->device_bytes += (long long)bytes;
+    return 0;
+}
+
+extern "C" void cvr_hip_default_opts(cvr_hip_opts *o) {
+    memset(o, 0, sizeof *o);
+    o->steps_per_chunk = 256;
+    o->precision = CVR_HIP_PREC_F64;
+    o->col_index_bits = 32;
+    o->recompute = 0;
+    o->kappa_threshold = 1e3;
+    o->keep_fp64 = 0;
+    o->device = 0;
+}
+
+static int value_size(int prec) { return prec == CVR_HIP_PREC_F64 ? 8 : prec == CVR_HIP_PREC_F32 ? 4 : 2; }
+
+extern "C" int cvr_hip_destroy(cvr_hip_handle *h) {
+    if (!h) return 0;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (void *p : h->allocs) hipFree(p);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+static int validate_opts(const cvr_hip_opts *o) {
+    if (o->steps_per_chunk < 8 || o->steps_per_chunk > 4096 || o->steps_per_chunk % cvr::kStepGroup)
+        return set_error(PAVA_HIP_EINVAL, "steps_per_chunk must be a multiple of 8 in [8, 4096], got %d",
+                         o->steps_per_chunk);
+    if (o->precision < 0 || o->precision > 2) return set_error(PAVA_HIP_EINVAL, "bad precision %d", o->precision);
+    if (o->col_index_bits != 32 && o->col_index_bits != 16)
+        return set_error(PAVA_HIP_EINVAL, "col_index_bits must be 32 or 16");
+    if (o->col_index_bits == 16 && o->precision == CVR_HIP_PREC_F64)
+        return set_error(PAVA_HIP_EINVAL, "16-bit columns are offered for FP32/BF16 storage only");
+    return 0;
+}
+
+// Core: conversion from a CSR in device memory.
+static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, const double *d_va,
+                       const cvr_hip_opts *o, cvr_hip_handle *h) {
+    h->opts = *o;
+    h->M = M;
+    h->N = N;
+    h->nnz = nnz;
+    h->S = o->steps_per_chunk;
+    h->prec = o->precision;
+    h->col16 = (o->col_index_bits == 16);
+    const long long C = 64LL * h->S;
+    h->nchunks = nnz > 0 ? (int)((nnz + C - 1) / C) : 0;
+    if (h->nchunks > 0) {
+        long long nlast = nnz - (long long)(h->nchunks - 1) * C;
+        long long steps_last = (nlast + 63) / 64;
+        steps_last = (steps_last + cvr::kStepGroup - 1) / cvr::kStepGroup * cvr::kStepGroup;
+        h->layout_elems = (long long)(h->nchunks - 1) * C + 64 * steps_last;
+    }
+    if (h->layout_elems > (long long)INT32_MAX + 4096)
+        return set_error(PAVA_HIP_ERANGE, "layout beyond int32 elements");
+    h->seg_cap = (long long)M + (long long)cvr::kSegExtra * h->nchunks + 256;
+    int rc;
+    const int vs = value_size(h->prec);
+    if ((rc = dev_alloc(h, &h->d_vals, (size_t)h->layout_elems * vs))) return rc;
+    if ((rc = dev_alloc(h, &h->d_cols, (size_t)h->layout_elems * (h->col16 ? 2 : 4)))) return rc;
+    if (h->col16) {
+        if ((rc = dev_alloc(h, (void **)&h->d_end_mask, (size_t)(h->layout_elems / 64) * 8))) return rc;
+        if ((rc = dev_alloc(h, (void **)&h->d_col_base, (size_t)h->nchunks * 4))) return rc;
+    }
+    if ((rc = dev_alloc(h, (void **)&h->d_chunk_rows, (size_t)(h->nchunks + 1) * 4))) return rc;
+    if ((rc = dev_alloc(h, (void **)&h->d_seg, (size_t)h->seg_cap * 4))) return rc;
+    if ((rc = dev_alloc(h, (void **)&h->d_seg_count, (size_t)h->nchunks * 4))) return rc;
+    if ((rc = dev_alloc(h, (void **)&h->d_split_n, (size_t)h->nchunks * 4))) return rc;
+    if ((rc = dev_alloc(h, (void **)&h->d_split_tab, (size_t)h->nchunks * cvr::kSplitCap * 8))) return rc;
+    if ((rc = dev_alloc(h, (void **)&h->d_fix, (size_t)h->nchunks * 8))) return rc;
+    if ((rc = dev_alloc(h, (void **)&h->d_carry, (size_t)h->nchunks * 16))) return rc;
+    if ((rc = dev_alloc(h, (void **)&h->d_empty, (size_t)((M + 31) / 32) * 4))) return rc;
+    if ((rc = dev_alloc(h, (void **)&h->d_status, 64))) return rc;
+
+    HIP_TRY(hipMemsetAsync(h->d_status, 0, 64, h->stream));
+    // the segment list beyond the written entries is read by the SpMV window
+    // (values unused); clear it once so it is defined
+    HIP_TRY(hipMemsetAsync(h->d_seg, 0, (size_t)h->seg_cap * 4, h->stream));
+    HIP_TRY(hipMemsetAsync(h->d_carry, 0, (size_t)h->nchunks * 16 + 16, h->stream));
+
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, h->stream));
+    HIP_TRY(cvr::launch_empty_rows(d_rp, M, h->d_empty, h->d_status, h->stream));
+    cvr::ConvertArgs a{};
+    a.rp = d_rp;
+    a.ci = d_ci;
+    a.va = d_va;
+    a.M = M;
+    a.N = N;
+    a.nnz = nnz;
+    a.S = h->S;
+    a.nchunks = h->nchunks;
+    a.val_kind = h->prec;
+    a.col16 = h->col16;
+    a.ovals = h->d_vals;
+    a.ocols = h->d_cols;
+    a.end_mask = h->d_end_mask;
+    a.col_base = h->d_col_base;
+    a.chunk_rows = h->d_chunk_rows;
+    a.seg = h->d_seg;
+    a.seg_count = h->d_seg_count;
+    a.split_n = h->d_split_n;
+    a.split_tab = h->d_split_tab;
+    a.fix = h->d_fix;
+    a.status = h->d_status;
+    HIP_TRY(cvr::launch_convert(a, h->stream));
+    HIP_TRY(hipEventRecord(e1, h->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    h->convert_ms = ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    int status[16];
+    HIP_TRY(hipMemcpy(status, h->d_status, sizeof status, hipMemcpyDeviceToHost));
+    h->max_slots = status[0];
+    if (status[1]) return set_error(PAVA_HIP_EDEVICE, "CVR conversion overflowed its per-chunk tables (flags %d)", status[1]);
+    if (status[2])
+        return set_error(PAVA_HIP_ERANGE, "a chunk spans more than 65536 columns: 16-bit column layout not possible");
+    h->n_fix = status[3];
+    h->has_empty = status[4] ? 1 : 0;
+    return 0;
+}
+
+static int begin_handle(const cvr_hip_opts *opts, cvr_hip_opts *o, cvr_hip_handle **out, cvr_hip_handle **hp) {
+    if (!out) return set_error(PAVA_HIP_EINVAL, "null output handle");
+    *out = nullptr;
+    if (opts)
+        *o = *opts;
+    else
+        cvr_hip_default_opts(o);
+    if (o->steps_per_chunk == 0) o->steps_per_chunk = 256;
+    if (o->kappa_threshold <= 0) o->kappa_threshold = 1e3;
+    int rc = validate_opts(o);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(PAVA_HIP_EDEVICE, "no HIP device");
+    if (o->device < 0 || o->device >= ndev) return set_error(PAVA_HIP_EINVAL, "device %d out of range", o->device);
+    HIP_TRY(hipSetDevice(o->device));
+    cvr_hip_handle *h = new cvr_hip_handle();
+    h->device = o->device;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return set_error(PAVA_HIP_EDEVICE, "hipStreamCreate failed");
+    }
+    *hp = h;
+    return 0;
+}
+
+extern "C" int cvr_hip_create(const pava_csr_matrix *A, const cvr_hip_opts *opts, cvr_hip_handle **out) {
+    int rc = pava_csr_validate(A);
+    if (rc) return rc;
+    cvr_hip_opts o;
+    cvr_hip_handle *h = nullptr;
+    if ((rc = begin_handle(opts, &o, out, &h))) return rc;
+    const int M = A->numRows, nnz = A->nnz;
+    int *d_rp = nullptr, *d_ci = nullptr;
+    double *d_va = nullptr;
+    const bool keep = o.keep_fp64 || o.recompute;
+    auto fail = [&](int code) {
+        if (!keep) {
+            hipFree(d_rp);
+            hipFree(d_ci);
+            hipFree(d_va);
+        }
+        cvr_hip_destroy(h);
+        return code;
+    };
+    if (keep) {
+        if ((rc = dev_alloc(h, (void **)&d_rp, (size_t)(M + 1) * 4)) || (rc = dev_alloc(h, (void **)&d_ci, (size_t)nnz * 4)) ||
+            (rc = dev_alloc(h, (void **)&d_va, (size_t)nnz * 8)))
+            return fail(rc);
+    } else {
+        if (hipMalloc(&d_rp, (size_t)(M + 1) * 4 + 4) != hipSuccess || hipMalloc(&d_ci, (size_t)nnz * 4 + 4) != hipSuccess ||
+            hipMalloc(&d_va, (size_t)nnz * 8 + 8) != hipSuccess)
+            return fail(set_error(PAVA_HIP_ENOMEM, "hipMalloc of the CSR upload failed"));
+    }
+    if (hipMemcpy(d_rp, A->rowOffsets, (size_t)(M + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        (nnz && hipMemcpy(d_ci, A->cols, (size_t)nnz * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        (nnz && hipMemcpy(d_va, A->vals, (size_t)nnz * 8, hipMemcpyHostToDevice) != hipSuccess))
+        return fail(set_error(PAVA_HIP_EDEVICE, "CSR upload failed"));
+    rc = create_core(M, A->numCols, nnz, d_rp, d_ci, d_va, &o, h);
+    if (rc) return fail(rc);
+    if (keep) {
+        h->d_rp = d_rp;
+        h->d_ci = d_ci;
+        h->d_va = d_va;
+    } else {
+        hipFree(d_rp);
+        hipFree(d_ci);
+        hipFree(d_va);
+    }
+    *out = h;
+    return 0;
+}
+
+extern "C" int cvr_hip_create_device(int numRows, int numCols, int nnz, const int *d_rowOffsets, const int *d_cols,
+                                     const double *d_vals, const cvr_hip_opts *opts, cvr_hip_handle **out) {
+    if (numRows < 0 || numCols < 0 || nnz < 0) return set_error(PAVA_HIP_EINVAL, "negative size");
+    if (!d_rowOffsets || (nnz > 0 && (!d_cols || !d_vals))) return set_error(PAVA_HIP_EINVAL, "null device CSR");
+    cvr_hip_opts o;
+    cvr_hip_handle *h = nullptr;
+    int rc = begin_handle(opts, &o, out, &h);
+    if (rc) return rc;
+    if (o.recompute || o.keep_fp64) {
+        // keep a private copy of the FP64 CSR for the recompute pass
+        if ((rc = dev_alloc(h, (void **)&h->d_rp, (size_t)(numRows + 1) * 4)) ||
+            (rc = dev_alloc(h, (void **)&h->d_ci, (size_t)nnz * 4)) || (rc = dev_alloc(h, (void **)&h->d_va, (size_t)nnz * 8))) {
+            cvr_hip_destroy(h);
+            return rc;
+        }
+        hipMemcpyAsync(h->d_rp, d_rowOffsets, (size_t)(numRows + 1) * 4, hipMemcpyDeviceToDevice, h->stream);
+        if (nnz) {
+            hipMemcpyAsync(h->d_ci, d_cols, (size_t)nnz * 4, hipMemcpyDeviceToDevice, h->stream);
+            hipMemcpyAsync(h->d_va, d_vals, (size_t)nnz * 8, hipMemcpyDeviceToDevice, h->stream);
+        }
+    }
+    rc = create_core(numRows, numCols, nnz, d_rowOffsets, d_cols, d_vals, &o, h);
+    if (rc) {
+        cvr_hip_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return 0;
+}
+
+extern "C" int cvr_hip_get_info(const cvr_hip_handle *h, cvr_hip_info *info) {
+    if (!h || !info) return set_error(PAVA_HIP_EINVAL, "null argument");
+    info->numRows = h->M;
+    info->numCols = h->N;
+    info->nnz = h->nnz;
+    info->nchunks = h->nchunks;
+    info->steps_per_chunk = h->S;
+    info->precision = h->prec;
+    info->col_index_bits = h->col16 ? 16 : 32;
+    info->layout_elems = h->layout_elems;
+    info->seg_capacity = h->seg_cap;
+    info->max_slots = h->max_slots;
+    info->has_empty_rows = h->has_empty;
+    info->n_fixup_rows = h->n_fix;
+    info->device_bytes = h->device_bytes;
+    info->convert_ms = h->convert_ms;
+    return 0;
+}
+
+static int enqueue_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, hipStream_t st) {
+    cvr::SpmvArgs a{};
+    a.vals = h->d_vals;
+    a.cols = h->d_cols;
+    a.end_mask = h->d_end_mask;
+    a.col_base = h->d_col_base;
+    a.chunk_rows = h->d_chunk_rows;
+    a.seg = h->d_seg;
+    a.split_n = h->d_split_n;
+    a.split_tab = h->d_split_tab;
+    a.empty_bits = h->d_empty;
+    a.x = d_x;
+    a.y = d_y;
+    a.carry = h->d_carry;
+    a.layout_elems = h->layout_elems;
+    a.nchunks = h->nchunks;
+    a.S = h->S;
+    a.M = h->M;
+    a.has_empty = h->has_empty;
+    HIP_TRY(cvr::launch_spmv(a, h->prec, h->col16, st));
+    if (h->n_fix > 0) HIP_TRY(cvr::launch_fixup(h->d_fix, h->d_carry, d_y, h->nchunks, h->prec, st));
+    return 0;
+}
+
+extern "C" int cvr_hip_spmv(cvr_hip_handle *h, const double *d_x, double *d_y, void *stream) {
+    if (!h) return set_error(PAVA_HIP_EINVAL, "null handle");
+    if (h->prec != CVR_HIP_PREC_F64) return set_error(PAVA_HIP_ESTATE, "cvr_hip_spmv needs an FP64 handle");
+    if ((!d_x && h->N > 0) || (!d_y && h->M > 0)) return set_error(PAVA_HIP_EINVAL, "null vector");
+    HIP_TRY(hipSetDevice(h->device));
+    return enqueue_spmv(h, d_x, d_y, (hipStream_t)stream);
+}
+
+extern "C" int cvr_hip_spmv_mixed(cvr_hip_handle *h, const float *d_x32, const double *d_x64, float *d_y, void *stream) {
+    (void)d_x64;
+    if (!h) return set_error(PAVA_HIP_EINVAL, "null handle");
+    if (h->prec == CVR_HIP_PREC_F64) return set_error(PAVA_HIP_ESTATE, "cvr_hip_spmv_mixed needs an FP32/BF16 handle");
+    if ((!d_x32 && h->N > 0) || (!d_y && h->M > 0)) return set_error(PAVA_HIP_EINVAL, "null vector");
+    HIP_TRY(hipSetDevice(h->device));
+    return enqueue_spmv(h, d_x32, d_y, (hipStream_t)stream);
+}
+
+extern "C" int cvr_hip_spmv_host(cvr_hip_handle *h, const double *x, double *y) {
+    if (!h || (!x && h->N > 0) || (!y && h->M > 0)) return set_error(PAVA_HIP_EINVAL, "null argument");
+    HIP_TRY(hipSetDevice(h->device));
+    int rc;
+    if (!h->d_hx) {
+        if ((rc = dev_alloc(h, &h->d_hx, (size_t)std::max(h->N, 1) * 8))) return rc;
+        if ((rc = dev_alloc(h, &h->d_hy, (size_t)std::max(h->M, 1) * 8))) return rc;
+    }
+    if (h->prec == CVR_HIP_PREC_F64) {
+        HIP_TRY(hipMemcpyAsync(h->d_hx, x, (size_t)h->N * 8, hipMemcpyHostToDevice, h->stream));
+        if ((rc = enqueue_spmv(h, h->d_hx, h->d_hy, h->stream))) return rc;
+        HIP_TRY(hipMemcpyAsync(y, h->d_hy, (size_t)h->M * 8, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    std::vector<float> xf((size_t)h->N), yf((size_t)h->M);
+    for (int i = 0; i < h->N; ++i) xf[(size_t)i] = (float)x[i];
+    HIP_TRY(hipMemcpyAsync(h->d_hx, xf.data(), (size_t)h->N * 4, hipMemcpyHostToDevice, h->stream));
+    if ((rc = enqueue_spmv(h, h->d_hx, h->d_hy, h->stream))) return rc;
+    HIP_TRY(hipMemcpyAsync(yf.data(), h->d_hy, (size_t)h->M * 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < h->M; ++i) y[i] = yf[(size_t)i];
+    return 0;
+}
+
+extern "C" int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *stream) {
+    (void)h;
+    (void)k;
+    (void)d_X;
+    (void)d_Y;
+    (void)stream;
+    return set_error(PAVA_HIP_ESTATE, "cvr_hip_spmm: not built yet");
+}
+
+extern "C" int cvr_hip_last_recompute_count(cvr_hip_handle *h, int *count) {
+    if (!h || !count) return set_error(PAVA_HIP_EINVAL, "null argument");
+    *count = 0;
+    return 0;
+}
+
+extern "C" int cvr_hip_time_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, int warmup, int iters, double *ms_total,
+                                 double *ms_main) {
+    if (!h || iters <= 0) return set_error(PAVA_HIP_EINVAL, "bad timing arguments");
+    HIP_TRY(hipSetDevice(h->device));
+    int rc;
+    for (int i = 0; i < warmup; ++i)
+        if ((rc = enqueue_spmv(h, d_x, d_y, h->stream))) return rc;
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, h->stream));
+    for (int i = 0; i < iters; ++i)
+        if ((rc = enqueue_spmv(h, d_x, d_y, h->stream))) return rc;
+    HIP_TRY(hipEventRecord(b, h->stream));
+    HIP_TRY(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, a, b));
+    if (ms_total) *ms_total = ms / iters;
+    if (ms_main) {
+        // main kernel alone: events bracket each launch of the SpMV kernel
+        std::vector<hipEvent_t> ev(2 * (size_t)iters);
+        for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+        cvr::SpmvArgs sa{};
+        sa.vals = h->d_vals;
+        sa.cols = h->d_cols;
+        sa.end_mask = h->d_end_mask;
+        sa.col_base = h->d_col_base;
+        sa.chunk_rows = h->d_chunk_rows;
+        sa.seg = h->d_seg;
+        sa.split_n = h->d_split_n;
+        sa.split_tab = h->d_split_tab;
+        sa.empty_bits = h->d_empty;
+        sa.x = d_x;
+        sa.y = d_y;
+        sa.carry = h->d_carry;
+        sa.layout_elems = h->layout_elems;
+        sa.nchunks = h->nchunks;
+        sa.S = h->S;
+        sa.M = h->M;
+        sa.has_empty = h->has_empty;
+        for (int i = 0; i < iters; ++i) {
+            HIP_TRY(hipEventRecord(ev[2 * (size_t)i], h->stream));
+            HIP_TRY(cvr::launch_spmv(sa, h->prec, h->col16, h->stream));
+            HIP_TRY(hipEventRecord(ev[2 * (size_t)i + 1], h->stream));
+            if (h->n_fix > 0) HIP_TRY(cvr::launch_fixup(h->d_fix, h->d_carry, d_y, h->nchunks, h->prec, h->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        double tot = 0;
+        for (int i = 0; i < iters; ++i) {
+            float t = 0.f;
+            HIP_TRY(hipEventElapsedTime(&t, ev[2 * (size_t)i], ev[2 * (size_t)i + 1]));
+            tot += t;
+        }
+        *ms_main = tot / iters;
+        for (auto &e : ev) hipEventDestroy(e);
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return 0;
+}
+
+extern "C" int cvr_hip_export_layout(const cvr_hip_handle *h, void *vals, void *cols, int *chunk_rows, int *seg,
+                                     int *seg_count, int *split_n, int *split_tab, int *fix, int *col_base,
+                                     uint64_t *end_mask) {
+    if (!h) return set_error(PAVA_HIP_EINVAL, "null handle");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const size_t L = (size_t)h->layout_elems;
+    if (vals && L) HIP_TRY(hipMemcpy(vals, h->d_vals, L * value_size(h->prec), hipMemcpyDeviceToHost));
+    if (cols && L) HIP_TRY(hipMemcpy(cols, h->d_cols, L * (h->col16 ? 2 : 4), hipMemcpyDeviceToHost));
+    if (chunk_rows) HIP_TRY(hipMemcpy(chunk_rows, h->d_chunk_rows, (size_t)(h->nchunks + 1) * 4, hipMemcpyDeviceToHost));
+    if (seg) HIP_TRY(hipMemcpy(seg, h->d_seg, (size_t)h->seg_cap * 4, hipMemcpyDeviceToHost));
+    if (h->nchunks) {
+        if (seg_count) HIP_TRY(hipMemcpy(seg_count, h->d_seg_count, (size_t)h->nchunks * 4, hipMemcpyDeviceToHost));
+        if (split_n) HIP_TRY(hipMemcpy(split_n, h->d_split_n, (size_t)h->nchunks * 4, hipMemcpyDeviceToHost));
+        if (split_tab)
+            HIP_TRY(hipMemcpy(split_tab, h->d_split_tab, (size_t)h->nchunks * cvr::kSplitCap * 8, hipMemcpyDeviceToHost));
+        if (fix) HIP_TRY(hipMemcpy(fix, h->d_fix, (size_t)h->nchunks * 8, hipMemcpyDeviceToHost));
+        if (col_base && h->col16) HIP_TRY(hipMemcpy(col_base, h->d_col_base, (size_t)h->nchunks * 4, hipMemcpyDeviceToHost));
+        if (end_mask && h->col16 && L) HIP_TRY(hipMemcpy(end_mask, h->d_end_mask, L / 64 * 8, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}

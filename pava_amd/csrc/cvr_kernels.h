@@ -1,0 +1,72 @@
+// cvr_kernels.h -- layout constants and kernel launch declarations shared by
+// the CVR-64 device code and the host handle code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace cvr {
+
+constexpr int kLanes = 64;          // one wavefront = one CVR "SIMD unit"
+constexpr int kWavesPerBlock = 4;   // 256-thread workgroups, one chunk per wave
+constexpr int kStepGroup = 8;       // steps per chunk are a multiple of this
+constexpr int kSplitCap = 72;       // split-table entries per chunk (groups + sentinel)
+constexpr int kSegExtra = 66;       // segment-list slack per chunk (steals + padding piece)
+constexpr int kSlotCap = 160;       // split-segment slots per chunk (bound: 64 thieves + 67 owners)
+constexpr int kEndFlag = (int)0x80000000u;  // sign bit of a column: last element of a segment
+
+// Piece kinds (bit set)
+constexpr int kHead = 1;  // row started in an earlier chunk
+constexpr int kTail = 2;  // row continues in a later chunk
+constexpr int kPad = 4;   // padding piece of the last chunk (discarded)
+
+// Split-table codes (row ids are >= 0)
+constexpr int kCodeHead = -1;     // partial of the chunk's first row -> carry[2c]
+constexpr int kCodeTail = -2;     // partial of the chunk's last row  -> carry[2c+1]
+constexpr int kCodeDiscard = -3;  // padding
+
+struct ConvertArgs {
+    const int *rp;
+    const int *ci;
+    const void *va;      // const double*
+    int M, N, nnz;
+    int S;               // steps per full chunk
+    int nchunks;
+    int val_kind;        // 0: f64, 1: f32, 2: bf16
+    int col16;           // 1: 16-bit chunk-relative columns + end mask
+    void *ovals;         // layout values
+    void *ocols;         // int32 (flagged) or uint16
+    uint64_t *end_mask;  // 16-bit layouts: one 64-bit mask per step
+    int *col_base;       // 16-bit layouts: per-chunk column base
+    int *chunk_rows;     // nchunks + 1
+    int *seg;            // segment destinations
+    int *seg_count;      // per chunk
+    int *split_n;        // per chunk
+    int2 *split_tab;     // per chunk kSplitCap x (code, first slot)
+    int *fix;            // per chunk (row or -1, first chunk of that row)
+    int *status;         // [0] max slots, [1] error flags, [2] col16 overflow, [3] n fixup rows
+};
+
+struct SpmvArgs {
+    const void *vals;
+    const void *cols;
+    const uint64_t *end_mask;
+    const int *col_base;
+    const int *chunk_rows;
+    const int *seg;
+    const int *split_n;
+    const int2 *split_tab;
+    const unsigned *empty_bits;
+    const void *x;
+    void *y;
+    double *carry;
+    long long layout_elems;
+    int nchunks, S, M, has_empty;
+};
+
+hipError_t launch_empty_rows(const int *rp, int M, unsigned *bits, int *status, hipStream_t st);
+hipError_t launch_convert(const ConvertArgs &a, hipStream_t st);
+hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st);
+hipError_t launch_fixup(const int *fix, const double *carry, void *y, int nchunks, int prec, hipStream_t st);
+
+}  // namespace cvr

@@ -1,0 +1,280 @@
+// cvr_spmv.hip -- CVR-64 SpMV kernels for gfx950.
+//
+// Replaces pava's spmv_compute_kernel (cvr/cvr.h:1062-1884): one wavefront
+// walks one chunk of the CVR-64 layout step by step. Per step each lane loads
+// one value and one column (coalesced: the step is 64 consecutive elements),
+// gathers x, and does one FMA into its own accumulator (pava: the 8-lane
+// AVX-512 FMA of phase 3, cvr/cvr.h:1512-1598). A lane whose column carries the
+// end flag closes its segment: the wave ballots the flags, and each ending
+// lane picks its destination from the chunk's segment list at
+// (segments ended so far + its rank among this step's ending lanes) -- a row
+// of y (pava's record-driven store, cvr/cvr.h:1303-1372) or an LDS slot for a
+// segment of a split row. The segment list is read through a 128-entry
+// register window (two VGPRs per lane, ds_bpermute lookups) refilled 64
+// entries ahead. After the last step the wave sums the slots of each split row
+// in element order (pava's t_rets / final_2 merge, cvr/cvr.h:1823-1868):
+// rows wholly inside the chunk are written, the partial of a row shared with
+// a neighbouring chunk goes to a per-chunk carry, which the fixup kernel adds
+// in chunk order (pava uses omp atomic +=, cvr/cvr.h:1413-1421, 1717, 1855;
+// this build is deterministic: the same inputs give bit-identical y).
+// Finally the wave zeroes the empty rows of its row range (pava leaves them
+// unwritten and relies on the caller's zeroing, main.cpp:561).
+
+#include <hip/hip_runtime.h>
+
+#include "cvr_kernels.h"
+
+namespace cvr {
+
+
+
+__device__ __forceinline__ int mbcnt64s(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of chunks so neighbouring chunks (shared rows, nearby x)
+// meet in the same L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+    const int q = nblocks >> 3, r = nblocks & 7;
+    const int x = bid & 7, i = bid >> 3;
+    return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+template <typename VT>
+__device__ __forceinline__ float val_to_f32(VT v);
+template <>
+__device__ __forceinline__ float val_to_f32<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float val_to_f32<unsigned short>(unsigned short v) {
+    return __uint_as_float((unsigned)v << 16);
+}
+
+// Accumulator: FP64 plain FMA chain (bit-identical to a sequential fma sum),
+// or FP32 with compensated summation (TwoProduct by FMA + TwoSum, i.e. the
+// Ogita-Rump-Oishi Dot2 scheme) for the reduced-precision storage.
+template <typename VT, typename XT, bool COMP>
+struct Acc;
+
+template <>
+struct Acc<double, double, false> {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double v, double x) { s = __builtin_fma(v, x, s); }
+    __device__ __forceinline__ double value() const { return s; }
+    __device__ __forceinline__ void reset() { s = 0.0; }
+};
+
+template <typename VT>
+struct Acc<VT, float, true> {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(VT vr, float x) {
+        const float v = val_to_f32<VT>(vr);
+        const float p = v * x;
+        const float pe = __builtin_fmaf(v, x, -p);
+        const float t = s + p;
+        const float z = t - s;
+        const float te = (s - (t - z)) + (p - z);
+        s = t;
+        c += te + pe;
+    }
+    __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+    __device__ __forceinline__ void reset() { s = 0.f, c = 0.f; }
+};
+
+template <typename YT>
+__device__ __forceinline__ void store_y(void *y, int r, double v) {
+    reinterpret_cast<YT *>(y)[r] = (YT)v;
+}
+
+template <typename VT, typename XT, typename YT, bool COMP, bool COL16>
+__global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int c = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wv;
+    if (c >= a.nchunks) return;
+
+    __shared__ double slots[kWavesPerBlock][kSlotCap];
+
+    const long long C = 64LL * a.S;
+    const size_t base = (size_t)C * (size_t)c;
+    const int steps = (c == a.nchunks - 1) ? (int)((a.layout_elems - (long long)base) >> 6) : a.S;
+    const int zr = a.chunk_rows[c];
+    const int *__restrict__ segs = a.seg + zr + kSegExtra * c;
+    const XT *__restrict__ x = reinterpret_cast<const XT *>(a.x);
+
+    int w0 = segs[lane], w1 = segs[64 + lane];
+    int wbase = 0, segp = 0;
+
+    const VT *__restrict__ vp = reinterpret_cast<const VT *>(a.vals) + base + lane;
+    const int *__restrict__ cp32 = reinterpret_cast<const int *>(a.cols) + base + lane;
+    const unsigned short *__restrict__ cp16 = reinterpret_cast<const unsigned short *>(a.cols) + base + lane;
+    const uint64_t *__restrict__ emp = a.end_mask + (base >> 6);
+    const int cbase = COL16 ? a.col_base[c] : 0;
+
+    Acc<VT, XT, COMP> acc;
+
+    VT vb[kStepGroup];
+    int cb[kStepGroup];
+#pragma unroll
+    for (int i = 0; i < kStepGroup; ++i) {
+        vb[i] = vp[i * 64];
+        cb[i] = COL16 ? (int)cp16[i * 64] : cp32[i * 64];
+    }
+    for (int g = 0; g < steps; g += kStepGroup) {
+        XT xv[kStepGroup];
+#pragma unroll
+        for (int i = 0; i < kStepGroup; ++i) xv[i] = COL16 ? x[cbase + cb[i]] : x[cb[i] & 0x7fffffff];
+        uint64_t em[kStepGroup];
+        if (COL16) {
+#pragma unroll
+            for (int i = 0; i < kStepGroup; ++i) em[i] = emp[g + i];
+        }
+        VT vn[kStepGroup];
+        int cn[kStepGroup];
+        const bool more = (g + kStepGroup) < steps;
+        if (more) {
+#pragma unroll
+            for (int i = 0; i < kStepGroup; ++i) {
+                vn[i] = vp[(size_t)(g + kStepGroup + i) * 64];
+                cn[i] = COL16 ? (int)cp16[(size_t)(g + kStepGroup + i) * 64] : cp32[(size_t)(g + kStepGroup + i) * 64];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kStepGroup; ++i) {
+            acc.add(vb[i], xv[i]);
+            const bool end = COL16 ? ((em[i] >> lane) & 1ull) : (cb[i] < 0);
+            const unsigned long long m = COL16 ? em[i] : __ballot(end);
+            if (m) {
+                const int rel = segp - wbase + mbcnt64s(m);  // < 128
+                const int d0 = __shfl(w0, rel & 63);
+                const int d1 = __shfl(w1, rel & 63);
+                const int d = rel < 64 ? d0 : d1;
+                if (end) {
+                    const double v = acc.value();
+                    if (d >= 0)
+                        store_y<YT>(a.y, d, v);
+                    else
+                        slots[wv][~d] = v;
+                    acc.reset();
+                }
+                segp += __popcll(m);
+                if (segp - wbase >= 64) {
+                    w0 = w1;
+                    w1 = segs[wbase + 128 + lane];
+                    wbase += 64;
+                }
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int i = 0; i < kStepGroup; ++i) {
+                vb[i] = vn[i];
+                cb[i] = cn[i];
+            }
+        }
+    }
+
+    // split rows: sum slots in element order
+    wave_sync_lds();
+    const int ns = a.split_n[c];
+    const int2 *tab = a.split_tab + (size_t)c * kSplitCap;
+    for (int k = lane; k < ns; k += 64) {
+        const int2 t = tab[k];
+        const int se = tab[k + 1].y;
+        double s = slots[wv][t.y];
+        for (int j = t.y + 1; j < se; ++j) s += slots[wv][j];
+        if (t.x >= 0)
+            store_y<YT>(a.y, t.x, s);
+        else if (t.x == kCodeHead)
+            a.carry[2 * c] = s;
+        else if (t.x == kCodeTail)
+            a.carry[2 * c + 1] = s;
+    }
+
+    // empty rows of [chunk_rows[c], chunk_rows[c+1])
+    if (a.has_empty) {
+        const int r0 = zr, r1 = a.chunk_rows[c + 1];
+        if (r1 > r0) {
+            const int wlo = r0 >> 5, whi = (r1 - 1) >> 5;
+            for (int w = wlo + lane; w <= whi; w += 64) {
+                unsigned bits = a.empty_bits[w];
+                const int rb = w << 5;
+                if (rb < r0) bits &= ~0u << (r0 - rb);
+                if (rb + 32 > r1) bits &= (r1 - rb >= 32) ? ~0u : ((1u << (r1 - rb)) - 1u);
+                while (bits) {
+                    const int j = __ffs(bits) - 1;
+                    store_y<YT>(a.y, rb + j, 0.0);
+                    bits &= bits - 1;
+                }
+            }
+        }
+    }
+}
+
+// Rows spanning several chunks: y[row] = tail partial of its first chunk +
+// partials of the chunks it covers entirely + head partial of its last chunk,
+// added in chunk order.
+template <typename YT>
+__global__ void fixup_kernel(const int *__restrict__ fix, const double *__restrict__ carry, void *y, int nchunks) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchunks) return;
+    const int row = fix[2 * c];
+    if (row < 0) return;
+    const int c0 = fix[2 * c + 1];
+    double s = carry[2 * c0 + 1];
+    for (int k = c0 + 1; k < c; ++k) s += carry[2 * k + 1];
+    s += carry[2 * c];
+    reinterpret_cast<YT *>(y)[row] = (YT)s;
+}
+
+template <typename YT>
+__global__ void zero_kernel(void *y, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) reinterpret_cast<YT *>(y)[i] = (YT)0;
+}
+
+// --------------------------------------------------------------- launchers
+hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st) {
+    if (a.nchunks == 0) {
+        if (a.M > 0) {
+            if (prec == 0)
+                hipLaunchKernelGGL(zero_kernel<double>, dim3((a.M + 255) / 256), dim3(256), 0, st, a.y, a.M);
+            else
+                hipLaunchKernelGGL(zero_kernel<float>, dim3((a.M + 255) / 256), dim3(256), 0, st, a.y, a.M);
+        }
+        return hipGetLastError();
+    }
+    dim3 grid((a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
+    if (prec == 0) {
+        hipLaunchKernelGGL((spmv_kernel<double, double, double, false, false>), grid, block, 0, st, a);
+    } else if (prec == 1) {
+        if (col16)
+            hipLaunchKernelGGL((spmv_kernel<float, float, float, true, true>), grid, block, 0, st, a);
+        else
+            hipLaunchKernelGGL((spmv_kernel<float, float, float, true, false>), grid, block, 0, st, a);
+    } else {
+        if (col16)
+            hipLaunchKernelGGL((spmv_kernel<unsigned short, float, float, true, true>), grid, block, 0, st, a);
+        else
+            hipLaunchKernelGGL((spmv_kernel<unsigned short, float, float, true, false>), grid, block, 0, st, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fixup(const int *fix, const double *carry, void *y, int nchunks, int prec, hipStream_t st) {
+    if (nchunks == 0) return hipSuccess;
+    dim3 grid((nchunks + 255) / 256), block(256);
+    if (prec == 0)
+        hipLaunchKernelGGL(fixup_kernel<double>, grid, block, 0, st, fix, carry, y, nchunks);
+    else
+        hipLaunchKernelGGL(fixup_kernel<float>, grid, block, 0, st, fix, carry, y, nchunks);
+    return hipGetLastError();
+}
+
+}  // namespace cvr

@@ -1,0 +1,112 @@
+// driver.hip -- the cvr_hip plugin's place in pava's benchmark protocol.
+//
+// conduct_cvr_hip follows conduct_cvr (cvr/cvr.h:1953-2186): the CSR->CVR
+// conversion is timed on its own, one verification SpMV writes refOut, then
+// numIterations SpMVs are timed and the " Performance ..." line is printed
+// (cvr/cvr.h:2075-2115). pava_bench_file follows benchFile's size filters and
+// the CVR block of conduct_benchmark (main.cpp:61-114, 203-207, 554-571).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pava_hip.h"
+#include "pava_internal.h"
+
+using namespace pava;
+
+extern "C" int conduct_cvr_hip(int numRows, int numCols, int nItems_raw, int *h_rowDelimiters, int *h_cols,
+                               double *h_val, double *h_vec, double *refOut, int alpha, char *matrixName,
+                               int numIterations) {
+    (void)alpha;  // accepted and ignored, as in the reference (cvr/cvr.h:1953)
+    pava_csr_matrix A{nItems_raw, numRows, numCols, h_val, h_cols, h_rowDelimiters};
+    int rc = pava_csr_validate(&A);
+    if (rc) {
+        pava_print_performance(matrixName ? matrixName : "?", "CVR_HIP", 0, -1, -1);
+        return rc;
+    }
+    cvr_hip_opts o;
+    cvr_hip_default_opts(&o);
+    printf(" Converting CSR->CVR (HIP)\n");
+    cvr_hip_handle *h = nullptr;
+    rc = cvr_hip_create(&A, &o, &h);
+    if (rc) {
+        fprintf(stderr, "cvr_hip_create: %s\n", pava_hip_last_error());
+        pava_print_performance(matrixName ? matrixName : "?", "CVR_HIP", 0, -1, -1);
+        return rc;
+    }
+    cvr_hip_info info;
+    cvr_hip_get_info(h, &info);
+    double *d_x = nullptr, *d_y = nullptr, *d_yi = nullptr;
+    auto done = [&](int code) {
+        hipFree(d_x);
+        hipFree(d_y);
+        hipFree(d_yi);
+        cvr_hip_destroy(h);
+        return code;
+    };
+    if (hipMalloc(&d_x, (size_t)std::max(numCols, 1) * 8) != hipSuccess ||
+        hipMalloc(&d_y, (size_t)std::max(numRows, 1) * 8) != hipSuccess ||
+        hipMalloc(&d_yi, (size_t)std::max(numRows, 1) * 8) != hipSuccess)
+        return done(set_error(PAVA_HIP_ENOMEM, "vector allocation failed"));
+    if (hipMemcpy(d_x, h_vec, (size_t)numCols * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return done(set_error(PAVA_HIP_EDEVICE, "x upload failed"));
+    printf(" Executing CVR (HIP)\n");
+    if ((rc = cvr_hip_spmv(h, d_x, d_y, nullptr))) return done(rc);
+    if (hipMemcpy(refOut, d_y, (size_t)numRows * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return done(set_error(PAVA_HIP_EDEVICE, "y download failed"));
+    hipDeviceSynchronize();
+    double t3 = pava_microtime();
+    for (int it = 0; it < numIterations; ++it)
+        if ((rc = cvr_hip_spmv(h, d_x, d_yi, nullptr))) return done(rc);
+    if (hipDeviceSynchronize() != hipSuccess) return done(set_error(PAVA_HIP_EDEVICE, "SpMV loop failed"));
+    double t4 = pava_microtime();
+    // the thread-count column reports the wavefronts (one per chunk)
+    pava_print_performance(matrixName ? matrixName : "?", "CVR_HIP", info.nchunks, info.convert_ms * 1e-3,
+                           numIterations > 0 ? (t4 - t3) / numIterations : 0.0);
+    return done(0);
+}
+
+extern "C" int pava_bench_file(const char *path, int numIterations) {
+    const int minSize = 4096;
+    const long long maxSize = (1024LL + 128) * 1024 * 1024;
+    pava_csr_matrix A;
+    int rc = pava_read_csr_matrix(path, &A);
+    if (rc) {
+        fprintf(stderr, "Reading COO matrix in matrix market format failed\n");
+        return rc;
+    }
+    printf("Basic informations\n        numRows = %d\n        numCols = %d\n            nnz = %d\n", A.numRows,
+           A.numCols, A.nnz);
+    if (A.nnz < minSize || (long long)A.nnz > maxSize || A.numRows < 2048 || A.numCols < 2048) {
+        printf(" FileWarning! %s is too small or too large\n", path);
+        pava_delete_csr_matrix(&A);
+        return -1;
+    }
+    // matrix name as main.cpp:212-228 derives it: "<group> <file>" without extension
+    std::string name(path);
+    size_t dot = name.rfind('.');
+    if (dot != std::string::npos && dot > name.rfind('/') + 1) name = name.substr(0, dot);
+    size_t sl = name.rfind('/');
+    if (sl != std::string::npos) {
+        name[sl] = ' ';
+        size_t sl2 = name.rfind('/');
+        if (sl2 != std::string::npos) name = name.substr(sl2 + 1);
+    }
+    std::vector<double> x((size_t)A.numCols), y((size_t)A.numRows), y_ref((size_t)A.numRows);
+    pava_init_vectors(A.numRows, A.numCols, x.data(), y.data(), y_ref.data());
+    pava_reference_spmv(&A, x.data(), y_ref.data());
+    printf("**************\n    CVR_HIP    \n**************\n");
+    std::vector<char> nm(name.begin(), name.end());
+    nm.push_back(0);
+    rc = conduct_cvr_hip(A.numRows, A.numCols, A.nnz, A.rowOffsets, A.cols, A.vals, x.data(), y.data(), 1, nm.data(),
+                         numIterations);
+    int errors = rc ? rc : pava_check_results(A.numRows, y.data(), y_ref.data());
+    pava_delete_csr_matrix(&A);
+    printf("\nCongratulations! This File comes to an Normal End! Flag[NormalEnding]\n\n");
+    return errors;
+}
