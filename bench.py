@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark of the cvr_hip SpMV path on one MI355X.
+
+Workload (default): FP64 CVR-HIP SpMV on an R-MAT scale-22 graph (4,194,304
+rows/cols, edge factor 16, a/b/c/d = .57/.19/.19/.05, duplicates summed,
+values U(0,1]), x ~ U[-1,1] from seed 0. A "step" is one y = A*x over the
+whole matrix with A, x and y resident in HBM.
+
+Prints ONE JSON line. `value` is the SpMV's algorithmic HBM throughput in GB/s:
+bytes = nnz*(8+4) + (rows+1)*4 + cols*8 + rows*8 per SpMV (values, column
+indices, row metadata, x, y), divided by the wall time per step. `roofline`
+prices the main SpMV kernel alone (hipEvent-timed launches) against 8 TB/s.
+`cpu_baseline` is the oracle's FP64 CSR SpMV (OpenMP over rows) timed on the
+host cores on the same matrix.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "rmat22_fp64": dict(kind="rmat", scale=22, prec=0),
+    "rmat20_fp64": dict(kind="rmat", scale=20, prec=0),
+    "stencil27_128_fp64": dict(kind="stencil", n=128, prec=0),
+    "laplacian256_fp64": dict(kind="laplacian", n=256, prec=0),
+}
+
+
+def algorithmic_bytes(m: int, n: int, nnz: int, s_val=8, s_idx=4, s_x=8, s_y=8) -> int:
+    return nnz * (s_val + s_idx) + (m + 1) * 4 + n * s_x + m * s_y
+
+
+def make_matrix(P, w):
+    if w["kind"] == "rmat":
+        return P.rmat(w["scale"], 16, 0.57, 0.19, 0.19, seed=0)
+    if w["kind"] == "stencil":
+        return P.stencil27(w["n"], 0.01, 0)
+    return P.laplacian2d(w["n"], w["n"])
+
+
+def cpu_baseline(O, A, x, budget_s: float = 10.0):
+    """Oracle CSR SpMV (FP64, OpenMP over rows) on the host: a bounded sample
+    of repeated full SpMVs until ~budget_s of wall time."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = max(1, min(threads, 16))
+    O.spmv_omp(A, x, threads)  # warm
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        O.spmv_omp(A, x, threads)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 2000:
+            break
+    per = el / n
+    return {
+        "value": algorithmic_bytes(A.numRows, A.numCols, A.nnz) / per / 1e9,
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} full FP64 CSR SpMVs of the same matrix ({el:.1f} s), oracle orc_csr_spmv_omp, "
+                  f"{per * 1e3:.2f} ms each, {A.nnz / per:.3e} nnz/s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="rmat22_fp64", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps-per-chunk", type=int, default=0, help="CVR chunk steps (0 = default)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures the single-GPU SpMV path only; this build has no multi-GPU path "
+              "(see DESIGN.md for how the path would shard). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import pava_amd as P
+
+    w = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    A = make_matrix(P, w)
+    t_gen = time.perf_counter() - t0
+    x = P.fill_uniform(A.numCols, 0)
+    print(f"[bench] {args.workload}: rows={A.numRows} nnz={A.nnz} max_row={int(np.diff(A.rowOffsets).max())} "
+          f"gen {t_gen:.1f}s", file=sys.stderr, flush=True)
+
+    S = args.steps_per_chunk or 256
+    h = P.CvrHip(A, steps_per_chunk=S, precision=w["prec"])
+    info = h.info()
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(x).to(dev)
+    dy = torch.empty(A.numRows, dtype=torch.float64, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    sp = stream.cuda_stream
+
+    for _ in range(args.warmup):
+        h.spmv(dx.data_ptr(), dy.data_ptr(), sp)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        h.spmv(dx.data_ptr(), dy.data_ptr(), sp)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+
+    # per-kernel timing with hipEvents on the handle's own stream
+    ms_total_ev, ms_main = h.time_spmv(dx.data_ptr(), dy.data_ptr(), 5, max(50, min(args.steps, 500)))
+
+    B = algorithmic_bytes(A.numRows, A.numCols, A.nnz)
+    value = B / (ms_step * 1e-3) / 1e9
+    achieved = B / (ms_main * 1e-3) / 1e9
+
+    result = {
+        "metric": "SpMV HBM GB/s (% peak) + nnz/s; FP64 CVR-HIP on R-MAT scale 22"
+        if w["kind"] == "rmat" and w["scale"] == 22 else f"SpMV HBM GB/s ({args.workload})",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if w["prec"] == 0 else "f32",
+        "data": "synthetic (seeded generator; x ~ U[-1,1] seed 0)",
+        "config": {"workload": args.workload, "rows": A.numRows, "cols": A.numCols, "nnz": A.nnz,
+                   "max_row_len": int(np.diff(A.rowOffsets).max()), "steps_per_chunk": S,
+                   "chunks": info["nchunks"], "algorithmic_bytes": B},
+        "nnz_per_s": A.nnz / (ms_step * 1e-3),
+        "gflops": 2.0 * A.nnz / (ms_step * 1e-3) / 1e9,
+        "hbm_frac": value / HBM_PEAK_GBS,
+        "convert_ms": round(info["convert_ms"], 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "spmv_kernel<double,double,double,false,false>",
+                     "kernel_ms": round(ms_main, 5)},
+    }
+
+    if not args.no_check:
+        import oracle as O
+        y = dy.cpu().numpy()
+        yref = O.spmv_fma(A, x)
+        st = O.error_stats(y, yref, O.row_abs(A, x))
+        st["bit_identical_rows"] = float(np.mean(y == yref))
+        st["rows_rel_le_1e-12"] = float(np.mean(np.abs(y - yref) <= 1e-12 * np.abs(yref)))
+        result["error_vs_fp64_oracle"] = st
+    if not args.no_cpu_baseline:
+        import oracle as O
+        result["cpu_baseline"] = cpu_baseline(O, A, x)
+    else:
+        result["cpu_baseline"] = None
+    h.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,148 @@
+"""CPU oracle for the CVR SpMV path -- TEST INFRASTRUCTURE ONLY.
+
+Python loader for ``oracle/liboracle.so`` (built from ``oracle/oracle.c`` by
+``oracle/build.py``). Used by tests/, ``__graft_entry__.smoke()`` and the
+cpu_baseline leg of bench.py as the checker; the product never imports it.
+Each function's reference citation and what pins it are in oracle.c's header.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "liboracle.so"
+SPLIT_CAP = 72
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            from . import build as _b
+            _b.build()
+        L = C.CDLL(str(LIB_PATH))
+        P, i = C.c_void_p, C.c_int
+        for f in ["orc_csr_spmv_fma", "orc_csr_spmv_muladd", "orc_csr_spmv_dd", "orc_row_abs"]:
+            getattr(L, f).argtypes = [i, P, P, P, P, P]
+            getattr(L, f).restype = None
+        L.orc_csr_spmv_omp.argtypes = [i, P, P, P, P, P, i]
+        L.orc_csr_spmv_omp.restype = None
+        L.orc_check_results.argtypes = [i, P, P]
+        L.orc_cvr64_sizes.argtypes = [i, i, i, C.POINTER(i), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.orc_cvr64_sizes.restype = None
+        L.orc_cvr64_build.argtypes = [i, i, i, P, P, P, i, i] + [P] * 9
+        L.orc_cvr64_spmv.argtypes = [i, i, i, i] + [P] * 10
+        L.orc_cvr64_spmv.restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _csr(A):
+    return (np.ascontiguousarray(A.rowOffsets, np.int32), np.ascontiguousarray(A.cols, np.int32),
+            np.ascontiguousarray(A.vals, np.float64))
+
+
+def _spmv(fn, A, x):
+    rp, ci, va = _csr(A)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty(A.numRows)
+    fn(A.numRows, _p(rp), _p(ci), _p(va), _p(x), _p(y))
+    return y
+
+
+def spmv_fma(A, x):
+    """referenceSpMV (utils/basicMatrix.cpp:538-556), sequential fma per row."""
+    return _spmv(lib().orc_csr_spmv_fma, A, x)
+
+
+def spmv_muladd(A, x):
+    return _spmv(lib().orc_csr_spmv_muladd, A, x)
+
+
+def spmv_dd(A, x):
+    """Near-exact (double-double accumulated) y."""
+    return _spmv(lib().orc_csr_spmv_dd, A, x)
+
+
+def row_abs(A, x):
+    return _spmv(lib().orc_row_abs, A, x)
+
+
+def spmv_omp(A, x, threads: int):
+    rp, ci, va = _csr(A)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty(A.numRows)
+    lib().orc_csr_spmv_omp(A.numRows, _p(rp), _p(ci), _p(va), _p(x), _p(y), threads)
+    return y
+
+
+def check_results(y, y_ref) -> int:
+    y = np.ascontiguousarray(y, np.float64)
+    y_ref = np.ascontiguousarray(y_ref, np.float64)
+    return lib().orc_check_results(int(y.shape[0]), _p(y), _p(y_ref))
+
+
+def cvr64_sizes(m: int, nnz: int, S: int):
+    nc, L, sc = C.c_int(), C.c_longlong(), C.c_longlong()
+    lib().orc_cvr64_sizes(m, nnz, S, C.byref(nc), C.byref(L), C.byref(sc))
+    return nc.value, L.value, sc.value
+
+
+def cvr64_build(A, S: int = 256, prec: int = 0) -> dict:
+    """CPU emulation of the GPU CVR-64 layout (same arrays as CvrHip.export_layout)."""
+    rp, ci, va = _csr(A)
+    nc, L, sc = cvr64_sizes(A.numRows, A.nnz, S)
+    vdt = {0: np.float64, 1: np.float32, 2: np.uint16}[prec]
+    out = {"vals": np.zeros(L, vdt), "cols": np.zeros(L, np.int32), "chunk_rows": np.zeros(nc + 1, np.int32),
+           "seg": np.zeros(sc, np.int32), "seg_count": np.zeros(max(nc, 1), np.int32),
+           "split_n": np.zeros(max(nc, 1), np.int32), "split_tab": np.zeros(2 * SPLIT_CAP * max(nc, 1), np.int32),
+           "fix": np.zeros(2 * max(nc, 1), np.int32), "info": np.zeros(3, np.int32)}
+    rc = lib().orc_cvr64_build(A.numRows, A.numCols, A.nnz, _p(rp), _p(ci), _p(va), S, prec,
+                               *[_p(out[k]) for k in ["vals", "cols", "chunk_rows", "seg", "seg_count", "split_n",
+                                                      "split_tab", "fix", "info"]])
+    if rc != 0:
+        raise RuntimeError(f"orc_cvr64_build failed: {rc}")
+    out.update(nchunks=nc, layout_elems=L, seg_capacity=sc, S=S, prec=prec)
+    return out
+
+
+def cvr64_spmv(A, lay: dict, x):
+    """y over an emulated layout, in the GPU kernel's summation order."""
+    rp = np.ascontiguousarray(A.rowOffsets, np.int32)
+    prec = lay["prec"]
+    if prec == 0:
+        xx = np.ascontiguousarray(x, np.float64)
+        y = np.empty(A.numRows, np.float64)
+    else:
+        xx = np.ascontiguousarray(x, np.float32)
+        y = np.empty(A.numRows, np.float32)
+    lib().orc_cvr64_spmv(A.numRows, A.nnz, lay["S"], prec, *[_p(lay[k]) for k in
+                                                              ["vals", "cols", "chunk_rows", "seg", "split_n",
+                                                               "split_tab", "fix"]], _p(rp), _p(xx), _p(y))
+    return y
+
+
+def error_stats(y, y_ref, scale=None) -> dict:
+    """Per-row relative error |y - y_ref| / |y_ref| (rows with y_ref == 0 use
+    the absolute error), and, with scale = sum_j |a_ij x_j|, the normalised
+    error |y - y_ref| / scale."""
+    y = np.asarray(y, np.float64)
+    y_ref = np.asarray(y_ref, np.float64)
+    d = np.abs(y - y_ref)
+    den = np.abs(y_ref)
+    rel = np.where(den > 0, d / np.where(den > 0, den, 1.0), d)
+    out = {"max_rel": float(rel.max()) if rel.size else 0.0,
+           "median_rel": float(np.median(rel)) if rel.size else 0.0}
+    if scale is not None:
+        sc = np.asarray(scale, np.float64)
+        nrm = np.where(sc > 0, d / np.where(sc > 0, sc, 1.0), d)
+        out["max_scaled"] = float(nrm.max()) if nrm.size else 0.0
+    return out

@@ -239,7 +239,22 @@ __global__ __launch_bounds__(256) void convert_kernel(ConvertArgs a) {
                 const int l = __ffsll((long long)idle) - 1;
                 const int R = steps - s;
                 unsigned long long cand = __ballot(cnt > R);
-                const int v = __ffsll((long long)cand) - 1;  // exists: sum(cnt) = 64 R and lane l holds 0
+                if (cand == 0) {
+                    // unreachable (sum(cnt) = 64 R while lane l holds 0); keep every
+                    // position inside the padding range and report it
+                    if (lane == 0) atomicOr(&a.status[1], 4);
+                    if (lane == l) {
+                        pos = a.nnz;
+                        cnt = R;
+                        prow = -1;
+                        pkind = kPad;
+                        split = true;
+                        sstart = pos;
+                    }
+                    idle &= idle - 1;
+                    continue;
+                }
+                const int v = __ffsll((long long)cand) - 1;
                 const int pv = __shfl(pos, v), cv = __shfl(cnt, v);
                 const int rv = __shfl(prow, v), kv = __shfl(pkind, v);
                 if (lane == l) {

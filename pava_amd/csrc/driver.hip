@@ -84,6 +84,7 @@ extern "C" int pava_bench_file(const char *path, int numIterations) {
            A.numCols, A.nnz);
     if (A.nnz < minSize || (long long)A.nnz > maxSize || A.numRows < 2048 || A.numCols < 2048) {
         printf(" FileWarning! %s is too small or too large\n", path);
+        fflush(stdout);
         pava_delete_csr_matrix(&A);
         return -1;
     }
@@ -108,5 +109,6 @@ extern "C" int pava_bench_file(const char *path, int numIterations) {
     int errors = rc ? rc : pava_check_results(A.numRows, y.data(), y_ref.data());
     pava_delete_csr_matrix(&A);
     printf("\nCongratulations! This File comes to an Normal End! Flag[NormalEnding]\n\n");
+    fflush(stdout);
     return errors;
 }
