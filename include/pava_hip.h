@@ -158,7 +158,8 @@ typedef struct cvr_hip_opts {
     int precision;           /* CVR_HIP_PREC_* */
     int col_index_bits;      /* 32, or 16 (chunk-relative offsets; mixed precision only) */
     int recompute;           /* mixed precision: recompute rows with kappa > kappa_threshold in FP64 */
-    double kappa_threshold;  /* sum|a*x| / |sum a*x| above which a row is recomputed (default 1e3) */
+    double kappa_threshold;  /* sum|a*x| / |sum a*x| above which a row is recomputed (default 64:
+                                FP32 storage then stays within ~1e-5 relative of FP64) */
     int keep_fp64;           /* keep the FP64 CSR resident (required by recompute) */
     int device;              /* HIP device ordinal */
 } cvr_hip_opts;
@@ -207,6 +208,9 @@ int cvr_hip_last_recompute_count(cvr_hip_handle *h, int *count);
  * average duration of the main SpMV kernel alone (events around it). */
 int cvr_hip_time_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, int warmup, int iters,
                       double *ms_total, double *ms_main);
+/* Same for a mixed-precision handle built with recompute (d_x64 = FP64 x). */
+int cvr_hip_time_spmv_x64(cvr_hip_handle *h, const void *d_x, const double *d_x64, void *d_y, int warmup,
+                          int iters, double *ms_total, double *ms_main);
 
 /* Copies of the layout for checking (host buffers sized from cvr_hip_info):
  * vals (layout_elems * value size), cols (layout_elems int32, or uint16 when

@@ -76,7 +76,7 @@ EXPORTED = [
     "pava_microtime", "pava_gen_laplacian2d", "pava_gen_stencil27", "pava_gen_rmat", "pava_gen_synthetic",
     "cvr_hip_default_opts", "cvr_hip_create", "cvr_hip_create_device", "cvr_hip_destroy", "cvr_hip_get_info",
     "cvr_hip_spmv", "cvr_hip_spmv_host", "cvr_hip_spmv_mixed", "cvr_hip_spmm", "cvr_hip_last_recompute_count",
-    "cvr_hip_time_spmv", "cvr_hip_export_layout", "conduct_cvr_hip", "pava_bench_file", "pava_hip_last_error",
+    "cvr_hip_time_spmv", "cvr_hip_time_spmv_x64", "cvr_hip_export_layout", "conduct_cvr_hip", "pava_bench_file", "pava_hip_last_error",
     "pava_hip_version",
 ]
 
@@ -122,6 +122,7 @@ def lib() -> C.CDLL:
     L.cvr_hip_spmm.argtypes = [P, i, P, P, P]
     L.cvr_hip_last_recompute_count.argtypes = [P, C.POINTER(i)]
     L.cvr_hip_time_spmv.argtypes = [P, P, P, i, i, C.POINTER(d), C.POINTER(d)]
+    L.cvr_hip_time_spmv_x64.argtypes = [P, P, P, P, i, i, C.POINTER(d), C.POINTER(d)]
     L.cvr_hip_export_layout.argtypes = [P] + [P] * 10
     L.conduct_cvr_hip.argtypes = [i, i, i, P, P, P, P, P, i, C.c_char_p, i]
     L.pava_bench_file.argtypes = [C.c_char_p, i]
@@ -252,7 +253,7 @@ class CvrHip:
     """A CVR-64 matrix resident on one GPU (the cvr_hip format plugin)."""
 
     def __init__(self, A: CSR, steps_per_chunk: int = 256, precision: int = PREC_F64, col_index_bits: int = 32,
-                 recompute: bool = False, kappa_threshold: float = 1e3, keep_fp64: bool = False, device: int = 0):
+                 recompute: bool = False, kappa_threshold: float = 64.0, keep_fp64: bool = False, device: int = 0):
         L = lib()
         o = _Opts()
         L.cvr_hip_default_opts(C.byref(o))
@@ -307,10 +308,17 @@ class CvrHip:
     def spmm(self, k: int, X_ptr: int, Y_ptr: int, stream: int = 0) -> None:
         _check(lib().cvr_hip_spmm(self._h, k, X_ptr, Y_ptr, stream))
 
-    def time_spmv(self, x_ptr: int, y_ptr: int, warmup: int = 10, iters: int = 100):
+    def time_spmv(self, x_ptr: int, y_ptr: int, warmup: int = 10, iters: int = 100, x64_ptr: int = 0):
+        """(ms per SpMV, ms of the main kernel alone), hipEvents on the handle's stream."""
         tot, main = C.c_double(), C.c_double()
-        _check(lib().cvr_hip_time_spmv(self._h, x_ptr, y_ptr, warmup, iters, C.byref(tot), C.byref(main)))
+        _check(lib().cvr_hip_time_spmv_x64(self._h, x_ptr, x64_ptr or None, y_ptr, warmup, iters, C.byref(tot),
+                                           C.byref(main)))
         return tot.value, main.value
+
+    def last_recompute_count(self) -> int:
+        n = C.c_int()
+        _check(lib().cvr_hip_last_recompute_count(self._h, C.byref(n)))
+        return n.value
 
     def export_layout(self) -> dict:
         inf = self.info()

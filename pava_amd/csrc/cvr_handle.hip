@@ -29,14 +29,16 @@ struct cvr_hip_handle {
     int *d_chunk_rows = nullptr, *d_seg = nullptr, *d_seg_count = nullptr, *d_split_n = nullptr;
     int2 *d_split_tab = nullptr;
     int *d_fix = nullptr;
-    double *d_carry = nullptr;
+    double *d_carry = nullptr, *d_carry_abs = nullptr;
+    unsigned char *d_rflag = nullptr;
+    int *d_rc_count = nullptr;
     unsigned *d_empty = nullptr;
     int *d_status = nullptr;
     // FP64 CSR kept resident (recompute); owned by the handle when set
     int *d_rp = nullptr, *d_ci = nullptr;
     double *d_va = nullptr;
     // staging for the host-pointer API
-    void *d_hx = nullptr, *d_hy = nullptr;
+    void *d_hx = nullptr, *d_hy = nullptr, *d_hx64 = nullptr;
     long long device_bytes = 0;
     double convert_ms = 0.0;
     std::vector<void *> allocs;
@@ -68,7 +70,7 @@ extern "C" void cvr_hip_default_opts(cvr_hip_opts *o) {
     o->precision = CVR_HIP_PREC_F64;
     o->col_index_bits = 32;
     o->recompute = 0;
-    o->kappa_threshold = 1e3;
+    o->kappa_threshold = 64.0;
     o->keep_fp64 = 0;
     o->device = 0;
 }
@@ -133,6 +135,11 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     if ((rc = dev_alloc(h, (void **)&h->d_split_tab, (size_t)h->nchunks * cvr::kSplitCap * 8))) return rc;
     if ((rc = dev_alloc(h, (void **)&h->d_fix, (size_t)h->nchunks * 8))) return rc;
     if ((rc = dev_alloc(h, (void **)&h->d_carry, (size_t)h->nchunks * 16))) return rc;
+    if (o->recompute && h->prec != CVR_HIP_PREC_F64) {
+        if ((rc = dev_alloc(h, (void **)&h->d_carry_abs, (size_t)h->nchunks * 16))) return rc;
+        if ((rc = dev_alloc(h, (void **)&h->d_rflag, (size_t)M + 64))) return rc;
+        if ((rc = dev_alloc(h, (void **)&h->d_rc_count, (size_t)h->nchunks * 4 + 4))) return rc;
+    }
     if ((rc = dev_alloc(h, (void **)&h->d_empty, (size_t)((M + 31) / 32) * 4))) return rc;
     if ((rc = dev_alloc(h, (void **)&h->d_status, 64))) return rc;
 
@@ -141,6 +148,11 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     // (values unused); clear it once so it is defined
     HIP_TRY(hipMemsetAsync(h->d_seg, 0, (size_t)h->seg_cap * 4, h->stream));
     HIP_TRY(hipMemsetAsync(h->d_carry, 0, (size_t)h->nchunks * 16 + 16, h->stream));
+    if (h->d_rflag) {
+        HIP_TRY(hipMemsetAsync(h->d_rflag, 0, (size_t)M + 64, h->stream));
+        HIP_TRY(hipMemsetAsync(h->d_carry_abs, 0, (size_t)h->nchunks * 16 + 16, h->stream));
+        HIP_TRY(hipMemsetAsync(h->d_rc_count, 0, (size_t)h->nchunks * 4 + 4, h->stream));
+    }
 
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -196,7 +208,7 @@ static int begin_handle(const cvr_hip_opts *opts, cvr_hip_opts *o, cvr_hip_handl
     else
         cvr_hip_default_opts(o);
     if (o->steps_per_chunk == 0) o->steps_per_chunk = 256;
-    if (o->kappa_threshold <= 0) o->kappa_threshold = 1e3;
+    if (o->kappa_threshold <= 0) o->kappa_threshold = 64.0;
     int rc = validate_opts(o);
     if (rc) return rc;
     int ndev = 0;
@@ -309,7 +321,7 @@ extern "C" int cvr_hip_get_info(const cvr_hip_handle *h, cvr_hip_info *info) {
     return 0;
 }
 
-static int enqueue_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, hipStream_t st) {
+static cvr::SpmvArgs spmv_args(const cvr_hip_handle *h, const void *d_x, void *d_y) {
     cvr::SpmvArgs a{};
     a.vals = h->d_vals;
     a.cols = h->d_cols;
@@ -323,14 +335,47 @@ static int enqueue_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, hipStream
     a.x = d_x;
     a.y = d_y;
     a.carry = h->d_carry;
+    a.carry_abs = h->d_carry_abs;
+    a.rflag = h->d_rflag;
+    a.kappa = (float)h->opts.kappa_threshold;
     a.layout_elems = h->layout_elems;
     a.nchunks = h->nchunks;
     a.S = h->S;
     a.M = h->M;
     a.has_empty = h->has_empty;
-    HIP_TRY(cvr::launch_spmv(a, h->prec, h->col16, st));
-    if (h->n_fix > 0) HIP_TRY(cvr::launch_fixup(h->d_fix, h->d_carry, d_y, h->nchunks, h->prec, st));
+    a.recompute = h->d_rflag != nullptr;
+    return a;
+}
+
+// Kernels after the main SpMV: cross-chunk rows, and for mixed precision with
+// recompute the FP64 pass over rows flagged for cancellation.
+static int enqueue_tail(cvr_hip_handle *h, void *d_y, const double *d_x64, hipStream_t st) {
+    if (h->d_rflag) {
+        cvr::FinishArgs f{};
+        f.fix = h->d_fix;
+        f.carry = h->d_carry;
+        f.carry_abs = h->d_carry_abs;
+        f.chunk_rows = h->d_chunk_rows;
+        f.rp = h->d_rp;
+        f.ci = h->d_ci;
+        f.va = h->d_va;
+        f.x64 = d_x64;
+        f.rflag = h->d_rflag;
+        f.y = (float *)d_y;
+        f.rc_count = h->d_rc_count;
+        f.kappa = (float)h->opts.kappa_threshold;
+        f.nchunks = h->nchunks;
+        HIP_TRY(cvr::launch_finish(f, st));
+    } else if (h->n_fix > 0) {
+        HIP_TRY(cvr::launch_fixup(h->d_fix, h->d_carry, d_y, h->nchunks, h->prec, st));
+    }
     return 0;
+}
+
+static int enqueue_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, hipStream_t st, const double *d_x64 = nullptr) {
+    cvr::SpmvArgs a = spmv_args(h, d_x, d_y);
+    HIP_TRY(cvr::launch_spmv(a, h->prec, h->col16, st));
+    return enqueue_tail(h, d_y, d_x64, st);
 }
 
 extern "C" int cvr_hip_spmv(cvr_hip_handle *h, const double *d_x, double *d_y, void *stream) {
@@ -342,12 +387,13 @@ extern "C" int cvr_hip_spmv(cvr_hip_handle *h, const double *d_x, double *d_y, v
 }
 
 extern "C" int cvr_hip_spmv_mixed(cvr_hip_handle *h, const float *d_x32, const double *d_x64, float *d_y, void *stream) {
-    (void)d_x64;
     if (!h) return set_error(PAVA_HIP_EINVAL, "null handle");
     if (h->prec == CVR_HIP_PREC_F64) return set_error(PAVA_HIP_ESTATE, "cvr_hip_spmv_mixed needs an FP32/BF16 handle");
     if ((!d_x32 && h->N > 0) || (!d_y && h->M > 0)) return set_error(PAVA_HIP_EINVAL, "null vector");
+    if (h->d_rflag && !d_x64 && h->N > 0)
+        return set_error(PAVA_HIP_EINVAL, "a handle built with recompute needs the FP64 x");
     HIP_TRY(hipSetDevice(h->device));
-    return enqueue_spmv(h, d_x32, d_y, (hipStream_t)stream);
+    return enqueue_spmv(h, d_x32, d_y, (hipStream_t)stream, d_x64);
 }
 
 extern "C" int cvr_hip_spmv_host(cvr_hip_handle *h, const double *x, double *y) {
@@ -368,7 +414,11 @@ extern "C" int cvr_hip_spmv_host(cvr_hip_handle *h, const double *x, double *y) 
     std::vector<float> xf((size_t)h->N), yf((size_t)h->M);
     for (int i = 0; i < h->N; ++i) xf[(size_t)i] = (float)x[i];
     HIP_TRY(hipMemcpyAsync(h->d_hx, xf.data(), (size_t)h->N * 4, hipMemcpyHostToDevice, h->stream));
-    if ((rc = enqueue_spmv(h, h->d_hx, h->d_hy, h->stream))) return rc;
+    if (h->d_rflag) {
+        if (!h->d_hx64 && (rc = dev_alloc(h, &h->d_hx64, (size_t)std::max(h->N, 1) * 8))) return rc;
+        HIP_TRY(hipMemcpyAsync(h->d_hx64, x, (size_t)h->N * 8, hipMemcpyHostToDevice, h->stream));
+    }
+    if ((rc = enqueue_spmv(h, h->d_hx, h->d_hy, h->stream, (const double *)h->d_hx64))) return rc;
     HIP_TRY(hipMemcpyAsync(yf.data(), h->d_hy, (size_t)h->M * 4, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     for (int i = 0; i < h->M; ++i) y[i] = yf[(size_t)i];
@@ -387,22 +437,36 @@ extern "C" int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d
 extern "C" int cvr_hip_last_recompute_count(cvr_hip_handle *h, int *count) {
     if (!h || !count) return set_error(PAVA_HIP_EINVAL, "null argument");
     *count = 0;
+    if (!h->d_rc_count || h->nchunks == 0) return 0;
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipDeviceSynchronize());
+    std::vector<int> v((size_t)h->nchunks);
+    HIP_TRY(hipMemcpy(v.data(), h->d_rc_count, v.size() * 4, hipMemcpyDeviceToHost));
+    long long t = 0;
+    for (int c : v) t += c;
+    *count = (int)t;
     return 0;
 }
 
 extern "C" int cvr_hip_time_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, int warmup, int iters, double *ms_total,
                                  double *ms_main) {
+    return cvr_hip_time_spmv_x64(h, d_x, nullptr, d_y, warmup, iters, ms_total, ms_main);
+}
+
+extern "C" int cvr_hip_time_spmv_x64(cvr_hip_handle *h, const void *d_x, const double *d_x64, void *d_y, int warmup,
+                                     int iters, double *ms_total, double *ms_main) {
     if (!h || iters <= 0) return set_error(PAVA_HIP_EINVAL, "bad timing arguments");
+    if (h->d_rflag && !d_x64) return set_error(PAVA_HIP_EINVAL, "a handle built with recompute needs the FP64 x");
     HIP_TRY(hipSetDevice(h->device));
     int rc;
     for (int i = 0; i < warmup; ++i)
-        if ((rc = enqueue_spmv(h, d_x, d_y, h->stream))) return rc;
+        if ((rc = enqueue_spmv(h, d_x, d_y, h->stream, d_x64))) return rc;
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
     HIP_TRY(hipEventRecord(a, h->stream));
     for (int i = 0; i < iters; ++i)
-        if ((rc = enqueue_spmv(h, d_x, d_y, h->stream))) return rc;
+        if ((rc = enqueue_spmv(h, d_x, d_y, h->stream, d_x64))) return rc;
     HIP_TRY(hipEventRecord(b, h->stream));
     HIP_TRY(hipEventSynchronize(b));
     float ms = 0.f;
@@ -412,29 +476,12 @@ extern "C" int cvr_hip_time_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, 
         // main kernel alone: events bracket each launch of the SpMV kernel
         std::vector<hipEvent_t> ev(2 * (size_t)iters);
         for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
-        cvr::SpmvArgs sa{};
-        sa.vals = h->d_vals;
-        sa.cols = h->d_cols;
-        sa.end_mask = h->d_end_mask;
-        sa.col_base = h->d_col_base;
-        sa.chunk_rows = h->d_chunk_rows;
-        sa.seg = h->d_seg;
-        sa.split_n = h->d_split_n;
-        sa.split_tab = h->d_split_tab;
-        sa.empty_bits = h->d_empty;
-        sa.x = d_x;
-        sa.y = d_y;
-        sa.carry = h->d_carry;
-        sa.layout_elems = h->layout_elems;
-        sa.nchunks = h->nchunks;
-        sa.S = h->S;
-        sa.M = h->M;
-        sa.has_empty = h->has_empty;
+        cvr::SpmvArgs sa = spmv_args(h, d_x, d_y);
         for (int i = 0; i < iters; ++i) {
             HIP_TRY(hipEventRecord(ev[2 * (size_t)i], h->stream));
             HIP_TRY(cvr::launch_spmv(sa, h->prec, h->col16, h->stream));
             HIP_TRY(hipEventRecord(ev[2 * (size_t)i + 1], h->stream));
-            if (h->n_fix > 0) HIP_TRY(cvr::launch_fixup(h->d_fix, h->d_carry, d_y, h->nchunks, h->prec, h->stream));
+            if ((rc = enqueue_tail(h, d_y, (const double *)d_x64, h->stream))) return rc;
         }
         HIP_TRY(hipStreamSynchronize(h->stream));
         double tot = 0;

@@ -59,14 +59,35 @@ struct SpmvArgs {
     const unsigned *empty_bits;
     const void *x;
     void *y;
-    double *carry;
+    double *carry;       // 2 per chunk: head / tail partial sums
+    double *carry_abs;   // 2 per chunk: head / tail partial sum|a x| (recompute only)
+    unsigned char *rflag;  // per row: 1 = recompute in FP64 (recompute only)
+    float kappa;         // recompute threshold on sum|a x| / |sum a x|
     long long layout_elems;
-    int nchunks, S, M, has_empty;
+    int nchunks, S, M, has_empty, recompute;
+};
+
+// FP64 recompute of flagged rows, one wave per chunk (mixed precision only).
+struct FinishArgs {
+    const int *fix;
+    const double *carry;
+    const double *carry_abs;
+    const int *chunk_rows;
+    const int *rp;
+    const int *ci;
+    const double *va;
+    const double *x64;
+    unsigned char *rflag;
+    float *y;
+    int *rc_count;       // per chunk: rows recomputed by the last SpMV
+    float kappa;
+    int nchunks;
 };
 
 hipError_t launch_empty_rows(const int *rp, int M, unsigned *bits, int *status, hipStream_t st);
 hipError_t launch_convert(const ConvertArgs &a, hipStream_t st);
 hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st);
 hipError_t launch_fixup(const int *fix, const double *carry, void *y, int nchunks, int prec, hipStream_t st);
+hipError_t launch_finish(const FinishArgs &a, hipStream_t st);
 
 }  // namespace cvr

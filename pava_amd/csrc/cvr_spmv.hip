@@ -59,20 +59,23 @@ __device__ __forceinline__ float val_to_f32<unsigned short>(unsigned short v) {
 // Accumulator: FP64 plain FMA chain (bit-identical to a sequential fma sum),
 // or FP32 with compensated summation (TwoProduct by FMA + TwoSum, i.e. the
 // Ogita-Rump-Oishi Dot2 scheme) for the reduced-precision storage.
-template <typename VT, typename XT, bool COMP>
+template <typename VT, typename XT, bool COMP, bool KAPPA>
 struct Acc;
 
 template <>
-struct Acc<double, double, false> {
+struct Acc<double, double, false, false> {
     double s = 0.0;
     __device__ __forceinline__ void add(double v, double x) { s = __builtin_fma(v, x, s); }
     __device__ __forceinline__ double value() const { return s; }
+    __device__ __forceinline__ double absval() const { return 0.0; }
     __device__ __forceinline__ void reset() { s = 0.0; }
 };
 
-template <typename VT>
-struct Acc<VT, float, true> {
-    float s = 0.f, c = 0.f;
+// KAPPA also accumulates sum|v x| (the cancellation scale) for the FP64
+// recompute decision.
+template <typename VT, bool KAPPA>
+struct Acc<VT, float, true, KAPPA> {
+    float s = 0.f, c = 0.f, ab = 0.f;
     __device__ __forceinline__ void add(VT vr, float x) {
         const float v = val_to_f32<VT>(vr);
         const float p = v * x;
@@ -82,9 +85,11 @@ struct Acc<VT, float, true> {
         const float te = (s - (t - z)) + (p - z);
         s = t;
         c += te + pe;
+        if (KAPPA) ab += __builtin_fabsf(p);
     }
     __device__ __forceinline__ double value() const { return (double)s + (double)c; }
-    __device__ __forceinline__ void reset() { s = 0.f, c = 0.f; }
+    __device__ __forceinline__ double absval() const { return (double)ab; }
+    __device__ __forceinline__ void reset() { s = 0.f, c = 0.f, ab = 0.f; }
 };
 
 template <typename YT>
@@ -92,7 +97,7 @@ __device__ __forceinline__ void store_y(void *y, int r, double v) {
     reinterpret_cast<YT *>(y)[r] = (YT)v;
 }
 
-template <typename VT, typename XT, typename YT, bool COMP, bool COL16>
+template <typename VT, typename XT, typename YT, bool COMP, bool COL16, bool KAPPA>
 __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -100,6 +105,7 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
     if (c >= a.nchunks) return;
 
     __shared__ double slots[kWavesPerBlock][kSlotCap];
+    __shared__ double slots_abs[KAPPA ? kWavesPerBlock : 1][KAPPA ? kSlotCap : 1];
 
     const long long C = 64LL * a.S;
     const size_t base = (size_t)C * (size_t)c;
@@ -117,7 +123,7 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
     const uint64_t *__restrict__ emp = a.end_mask + (base >> 6);
     const int cbase = COL16 ? a.col_base[c] : 0;
 
-    Acc<VT, XT, COMP> acc;
+    Acc<VT, XT, COMP, KAPPA> acc;
 
     VT vb[kStepGroup];
     int cb[kStepGroup];
@@ -157,10 +163,13 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
                 const int d = rel < 64 ? d0 : d1;
                 if (end) {
                     const double v = acc.value();
-                    if (d >= 0)
+                    if (d >= 0) {
                         store_y<YT>(a.y, d, v);
-                    else
+                        if (KAPPA && acc.absval() > (double)a.kappa * __builtin_fabs(v)) a.rflag[d] = 1;
+                    } else {
                         slots[wv][~d] = v;
+                        if (KAPPA) slots_abs[wv][~d] = acc.absval();
+                    }
                     acc.reset();
                 }
                 segp += __popcll(m);
@@ -189,12 +198,19 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
         const int se = tab[k + 1].y;
         double s = slots[wv][t.y];
         for (int j = t.y + 1; j < se; ++j) s += slots[wv][j];
-        if (t.x >= 0)
+        double sa = 0.0;
+        if (KAPPA)
+            for (int j = t.y; j < se; ++j) sa += slots_abs[wv][j];
+        if (t.x >= 0) {
             store_y<YT>(a.y, t.x, s);
-        else if (t.x == kCodeHead)
+            if (KAPPA && sa > (double)a.kappa * __builtin_fabs(s)) a.rflag[t.x] = 1;
+        } else if (t.x == kCodeHead) {
             a.carry[2 * c] = s;
-        else if (t.x == kCodeTail)
+            if (KAPPA) a.carry_abs[2 * c] = sa;
+        } else if (t.x == kCodeTail) {
             a.carry[2 * c + 1] = s;
+            if (KAPPA) a.carry_abs[2 * c + 1] = sa;
+        }
     }
 
     // empty rows of [chunk_rows[c], chunk_rows[c+1])
@@ -239,7 +255,70 @@ __global__ void zero_kernel(void *y, int n) {
     if (i < n) reinterpret_cast<YT *>(y)[i] = (YT)0;
 }
 
+// Mixed precision, after the SpMV: one wave per chunk (1) closes the chunk's
+// cross-chunk row (as fixup_kernel) and flags it when its cancellation ratio
+// exceeds kappa, then (2) recomputes every flagged row of the chunk's row range
+// [chunk_rows[c], chunk_rows[c+1]) from the resident FP64 CSR and FP64 x:
+// lanes stride the row with FMA, a fixed xor-butterfly adds the lane partials
+// (deterministic), the flag is cleared for the next SpMV.
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ double row_fp64(const FinishArgs &a, int row, int lane) {
+    double s = 0.0;
+    for (int j = a.rp[row] + lane; j < a.rp[row + 1]; j += 64) s = __builtin_fma(a.va[j], a.x64[a.ci[j]], s);
+    return wave_sum_f64(s);
+}
+
+__global__ __launch_bounds__(256) void finish_kernel(FinishArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (c >= a.nchunks) return;
+    int count = 0;
+    const int frow = a.fix[2 * c];
+    if (frow >= 0) {
+        const int c0 = a.fix[2 * c + 1];
+        double s = a.carry[2 * c0 + 1], sa = a.carry_abs[2 * c0 + 1];
+        for (int k = c0 + 1; k < c; ++k) s += a.carry[2 * k + 1], sa += a.carry_abs[2 * k + 1];
+        s += a.carry[2 * c];
+        sa += a.carry_abs[2 * c];
+        if (a.x64 && sa > (double)a.kappa * __builtin_fabs(s)) {
+            s = row_fp64(a, frow, lane);
+            ++count;
+        }
+        if (lane == 0) a.y[frow] = (float)s;
+    }
+    if (a.x64) {
+        const int r0 = a.chunk_rows[c], r1 = a.chunk_rows[c + 1];
+        for (int rb = r0; rb < r1; rb += 64) {
+            const int r = rb + lane;
+            const bool f = (r < r1) && a.rflag[r];
+            unsigned long long m = __ballot(f);
+            if (f) a.rflag[r] = 0;
+            while (m) {
+                const int j = __ffsll((long long)m) - 1;
+                const int row = rb + j;
+                const double s = row_fp64(a, row, lane);
+                if (lane == 0) a.y[row] = (float)s;
+                ++count;
+                m &= m - 1;
+            }
+        }
+    }
+    if (lane == 0) a.rc_count[c] = count;
+}
+
 // --------------------------------------------------------------- launchers
+template <typename VT, bool COL16>
+static void launch_mixed(const SpmvArgs &a, dim3 grid, dim3 block, hipStream_t st) {
+    if (a.recompute)
+        hipLaunchKernelGGL((spmv_kernel<VT, float, float, true, COL16, true>), grid, block, 0, st, a);
+    else
+        hipLaunchKernelGGL((spmv_kernel<VT, float, float, true, COL16, false>), grid, block, 0, st, a);
+}
+
 hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st) {
     if (a.nchunks == 0) {
         if (a.M > 0) {
@@ -252,18 +331,25 @@ hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st) {
     }
     dim3 grid((a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
     if (prec == 0) {
-        hipLaunchKernelGGL((spmv_kernel<double, double, double, false, false>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((spmv_kernel<double, double, double, false, false, false>), grid, block, 0, st, a);
     } else if (prec == 1) {
         if (col16)
-            hipLaunchKernelGGL((spmv_kernel<float, float, float, true, true>), grid, block, 0, st, a);
+            launch_mixed<float, true>(a, grid, block, st);
         else
-            hipLaunchKernelGGL((spmv_kernel<float, float, float, true, false>), grid, block, 0, st, a);
+            launch_mixed<float, false>(a, grid, block, st);
     } else {
         if (col16)
-            hipLaunchKernelGGL((spmv_kernel<unsigned short, float, float, true, true>), grid, block, 0, st, a);
+            launch_mixed<unsigned short, true>(a, grid, block, st);
         else
-            hipLaunchKernelGGL((spmv_kernel<unsigned short, float, float, true, false>), grid, block, 0, st, a);
+            launch_mixed<unsigned short, false>(a, grid, block, st);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_finish(const FinishArgs &a, hipStream_t st) {
+    if (a.nchunks == 0) return hipSuccess;
+    dim3 grid((a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
+    hipLaunchKernelGGL(finish_kernel, grid, block, 0, st, a);
     return hipGetLastError();
 }
 

@@ -156,3 +156,79 @@ def test_invalid_options_rejected(P):
                dict(col_index_bits=16)]:
         with pytest.raises(P.PavaError):
             P.CvrHip(A, **kw)
+
+
+def _mixed_run(P, A, x, prec, recompute, kappa=64.0, col_bits=32, S=64):
+    import torch
+    with P.CvrHip(A, steps_per_chunk=S, precision=prec, recompute=recompute, kappa_threshold=kappa,
+                  col_index_bits=col_bits) as h:
+        dx32 = torch.from_numpy(x.astype(np.float32)).cuda()
+        dx64 = torch.from_numpy(x).cuda()
+        dy = torch.full((A.numRows,), float("nan"), dtype=torch.float32, device="cuda")
+        h.spmv_mixed(dx32.data_ptr(), dy.data_ptr(), dx64.data_ptr() if recompute else 0)
+        torch.cuda.synchronize()
+        y = dy.cpu().numpy().astype(np.float64)
+        n = h.last_recompute_count()
+        # second call: flags were cleared, same result
+        h.spmv_mixed(dx32.data_ptr(), dy.data_ptr(), dx64.data_ptr() if recompute else 0)
+        torch.cuda.synchronize()
+        assert np.array_equal(y, dy.cpu().numpy().astype(np.float64))
+    return y, n
+
+
+def test_fp32_recompute_meets_1e5_on_cancelling_rows(P, O):
+    """27-point stencil with x = 1: interior rows cancel (26 - sum of 26 values ~ -1)."""
+    A = P.stencil27(20, 0.01, 0)
+    x = np.ones(A.numCols)
+    yref = O.spmv_fma(A, x)
+    sabs = O.row_abs(A, x)
+    kappa = sabs / np.maximum(np.abs(yref), 1e-300)
+    y_plain, n0 = _mixed_run(P, A, x, 1, False)
+    y_rc, n1 = _mixed_run(P, A, x, 1, True)
+    assert n0 == 0 and n1 > 0
+    rel_plain = np.abs(y_plain - yref) / np.abs(yref)
+    rel_rc = np.abs(y_rc - yref) / np.abs(yref)
+    _say("kappa max", kappa.max(), "rows > 64:", int((kappa > 64).sum()), "recomputed", n1,
+         "max rel plain", rel_plain.max(), "with recompute", rel_rc.max())
+    assert rel_rc.max() <= 1e-5
+    # rows with kappa well below the threshold are left to the FP32 path and its bound
+    low = kappa < 32
+    assert np.all(np.abs(y_rc[low] - yref[low]) <= 2.0 ** -22 * sabs[low] + 2.0 ** -23 * np.abs(yref[low]))
+
+
+def test_fp32_recompute_random_x_and_rmat(P, O):
+    for name, A in [("stencil", P.stencil27(24, 0.01, 3)), ("rmat", P.rmat(15, 16, seed=2))]:
+        x = P.fill_uniform(A.numCols, 0)
+        yref = O.spmv_fma(A, x)
+        y, n = _mixed_run(P, A, x, 1, True, S=256)
+        rel = np.abs(y - yref) / np.maximum(np.abs(yref), 1e-300)
+        _say(name, "recomputed", n, "max rel", rel.max(), "median", np.median(rel))
+        assert rel.max() <= 1e-5
+
+
+def test_col16_layout_and_spmv(P, O):
+    A = P.stencil27(24, 0.01, 1)
+    x = P.fill_uniform(A.numCols, 0)
+    y16, _ = _mixed_run(P, A, x, 1, False, col_bits=16, S=64)
+    y32, _ = _mixed_run(P, A, x, 1, False, col_bits=32, S=64)
+    assert np.array_equal(y16, y32)  # same layout order, only the column encoding differs
+    with P.CvrHip(A, steps_per_chunk=64, precision=1, col_index_bits=16) as h:
+        lay = h.export_layout()
+    emu = O.cvr64_build(A, 64, 1)
+    nc = lay["info"]["nchunks"]
+    # rebuild 32-bit flagged columns from base + offset + end mask and compare with the emulation
+    base = np.repeat(lay["col_base"][:nc], 64 * 64)[: lay["cols"].size]
+    full = base + lay["cols"].astype(np.int64)
+    ends = np.unpackbits(lay["end_mask"].view(np.uint8), bitorder="little").astype(bool)[: full.size]
+    assert np.array_equal(full, emu["cols"] & 0x7FFFFFFF)
+    assert np.array_equal(ends, emu["cols"] < 0)
+
+
+def test_recompute_needs_fp64_x(P):
+    import torch
+    A = P.laplacian2d(30, 30)
+    with P.CvrHip(A, precision=1, recompute=True) as h:
+        dx = torch.zeros(A.numCols, dtype=torch.float32, device="cuda")
+        dy = torch.zeros(A.numRows, dtype=torch.float32, device="cuda")
+        with pytest.raises(P.PavaError):
+            h.spmv_mixed(dx.data_ptr(), dy.data_ptr(), 0)
