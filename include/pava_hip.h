@@ -200,6 +200,24 @@ int cvr_hip_spmv_mixed(cvr_hip_handle *h, const float *d_x32, const double *d_x6
 /* Y = A*X with k right-hand sides, X (numCols x k) and Y (numRows x k) row-major
  * FP32, device pointers; 1 <= k <= 32. Any precision of handle. */
 int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *stream);
+/* Batched multi-RHS SpMM: the chunks of many matrices (same device and
+ * precision, 32-bit column layouts, nnz > 0) form one work queue consumed by a
+ * persistent grid sized to the GPU's resident capacity; a fixup kernel closes
+ * rows spanning chunks and rearms the queue, so the launch pair can be captured
+ * once in a hipGraph and replayed. The handles must outlive the batch.
+ * Accumulation is FP64 over FP32 inputs (one rounding per output). */
+typedef struct cvr_hip_batch cvr_hip_batch;
+int cvr_hip_batch_create(int nmat, cvr_hip_handle *const *handles, int k, cvr_hip_batch **out);
+/* Device pointers of each matrix's X (numCols x k) and Y (numRows x k), FP32
+ * row-major, 16-byte aligned when k is 8, 16 or 32. Synchronous upload. */
+int cvr_hip_batch_set_operands(cvr_hip_batch *b, const float *const *d_X, float *const *d_Y);
+int cvr_hip_batch_spmm(cvr_hip_batch *b, void *stream);
+/* Captures one batched SpMM on `stream` (non-null) into a hipGraph. */
+int cvr_hip_batch_graph(cvr_hip_batch *b, void *stream);
+int cvr_hip_batch_replay(cvr_hip_batch *b, void *stream);
+int cvr_hip_batch_info(const cvr_hip_batch *b, int *nchunks_total, int *grid, long long *nnz);
+int cvr_hip_batch_destroy(cvr_hip_batch *b);
+
 /* Number of rows recomputed in FP64 by the last mixed SpMV (synchronises). */
 int cvr_hip_last_recompute_count(cvr_hip_handle *h, int *count);
 

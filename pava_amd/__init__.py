@@ -76,8 +76,10 @@ EXPORTED = [
     "pava_microtime", "pava_gen_laplacian2d", "pava_gen_stencil27", "pava_gen_rmat", "pava_gen_synthetic",
     "cvr_hip_default_opts", "cvr_hip_create", "cvr_hip_create_device", "cvr_hip_destroy", "cvr_hip_get_info",
     "cvr_hip_spmv", "cvr_hip_spmv_host", "cvr_hip_spmv_mixed", "cvr_hip_spmm", "cvr_hip_last_recompute_count",
-    "cvr_hip_time_spmv", "cvr_hip_time_spmv_x64", "cvr_hip_export_layout", "conduct_cvr_hip", "pava_bench_file", "pava_hip_last_error",
-    "pava_hip_version",
+    "cvr_hip_time_spmv", "cvr_hip_time_spmv_x64", "cvr_hip_export_layout", "conduct_cvr_hip", "pava_bench_file",
+    "pava_hip_last_error", "pava_hip_version", "cvr_hip_batch_create", "cvr_hip_batch_set_operands",
+    "cvr_hip_batch_spmm", "cvr_hip_batch_graph", "cvr_hip_batch_replay", "cvr_hip_batch_info",
+    "cvr_hip_batch_destroy",
 ]
 
 
@@ -126,6 +128,13 @@ def lib() -> C.CDLL:
     L.cvr_hip_export_layout.argtypes = [P] + [P] * 10
     L.conduct_cvr_hip.argtypes = [i, i, i, P, P, P, P, P, i, C.c_char_p, i]
     L.pava_bench_file.argtypes = [C.c_char_p, i]
+    L.cvr_hip_batch_create.argtypes = [i, C.POINTER(P), i, C.POINTER(P)]
+    L.cvr_hip_batch_set_operands.argtypes = [P, C.POINTER(P), C.POINTER(P)]
+    L.cvr_hip_batch_spmm.argtypes = [P, P]
+    L.cvr_hip_batch_graph.argtypes = [P, P]
+    L.cvr_hip_batch_replay.argtypes = [P, P]
+    L.cvr_hip_batch_info.argtypes = [P, C.POINTER(i), C.POINTER(i), C.POINTER(C.c_longlong)]
+    L.cvr_hip_batch_destroy.argtypes = [P]
     _lib = L
     return L
 
@@ -350,3 +359,45 @@ def conduct_cvr_hip(A: CSR, x: np.ndarray, name: str = "matrix", iterations: int
 
 def version() -> str:
     return lib().pava_hip_version().decode()
+
+
+class Batch:
+    """Batched multi-RHS SpMM over several CvrHip matrices (persistent launch)."""
+
+    def __init__(self, mats: list, k: int):
+        arr = (C.c_void_p * len(mats))(*[m.handle for m in mats])
+        self._b = C.c_void_p()
+        self.mats = mats  # keep the handles alive
+        self.k = k
+        _check(lib().cvr_hip_batch_create(len(mats), arr, k, C.byref(self._b)))
+
+    def set_operands(self, X_ptrs: list, Y_ptrs: list) -> None:
+        n = len(self.mats)
+        xs = (C.c_void_p * n)(*X_ptrs)
+        ys = (C.c_void_p * n)(*Y_ptrs)
+        _check(lib().cvr_hip_batch_set_operands(self._b, xs, ys))
+
+    def spmm(self, stream: int = 0) -> None:
+        _check(lib().cvr_hip_batch_spmm(self._b, stream))
+
+    def capture(self, stream: int) -> None:
+        _check(lib().cvr_hip_batch_graph(self._b, stream))
+
+    def replay(self, stream: int) -> None:
+        _check(lib().cvr_hip_batch_replay(self._b, stream))
+
+    def info(self) -> dict:
+        t, g, n = C.c_int(), C.c_int(), C.c_longlong()
+        _check(lib().cvr_hip_batch_info(self._b, C.byref(t), C.byref(g), C.byref(n)))
+        return {"chunks": t.value, "grid": g.value, "nnz": n.value}
+
+    def close(self) -> None:
+        if self._b:
+            lib().cvr_hip_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -30,6 +30,7 @@ struct cvr_hip_handle {
     int2 *d_split_tab = nullptr;
     int *d_fix = nullptr;
     double *d_carry = nullptr, *d_carry_abs = nullptr;
+    double *d_carry_mm = nullptr;  // SpMM carries: 2 * nchunks * 32
     unsigned char *d_rflag = nullptr;
     int *d_rc_count = nullptr;
     unsigned *d_empty = nullptr;
@@ -425,13 +426,187 @@ extern "C" int cvr_hip_spmv_host(cvr_hip_handle *h, const double *x, double *y) 
     return 0;
 }
 
+static int spmm_desc(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, cvr::SpmmDesc *a) {
+    if (k < 1 || k > 32) return set_error(PAVA_HIP_EINVAL, "k must be in [1, 32], got %d", k);
+    if (h->col16) return set_error(PAVA_HIP_ESTATE, "SpMM needs a 32-bit column layout");
+    if ((!d_X && h->N > 0) || (!d_Y && h->M > 0)) return set_error(PAVA_HIP_EINVAL, "null X or Y");
+    if (k == cvr::spmm_kp(k) && (((uintptr_t)d_X | (uintptr_t)d_Y) & 15))
+        return set_error(PAVA_HIP_EINVAL, "X and Y must be 16-byte aligned");
+    const size_t need = (size_t)h->nchunks * 2 * 32 * 8 + 64;
+    if (!h->d_carry_mm) {
+        int rc = dev_alloc(h, (void **)&h->d_carry_mm, need);
+        if (rc) return rc;
+    }
+    *a = cvr::SpmmDesc{};
+    a->vals = h->d_vals;
+    a->cols = (const int *)h->d_cols;
+    a->chunk_rows = h->d_chunk_rows;
+    a->seg = h->d_seg;
+    a->split_n = h->d_split_n;
+    a->split_tab = h->d_split_tab;
+    a->empty_bits = h->d_empty;
+    a->fix = h->d_fix;
+    a->X = d_X;
+    a->Y = d_Y;
+    a->carry = h->d_carry_mm;
+    a->layout_elems = h->layout_elems;
+    a->nchunks = h->nchunks;
+    a->S = h->S;
+    a->M = h->M;
+    a->k = k;
+    a->has_empty = h->has_empty;
+    return 0;
+}
+
+static hipError_t zero_rows_k(float *d_Y, long long n, hipStream_t st) {
+    return n > 0 ? hipMemsetAsync(d_Y, 0, (size_t)n * 4, st) : hipSuccess;
+}
+
 extern "C" int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *stream) {
-    (void)h;
-    (void)k;
-    (void)d_X;
-    (void)d_Y;
-    (void)stream;
-    return set_error(PAVA_HIP_ESTATE, "cvr_hip_spmm: not built yet");
+    if (!h) return set_error(PAVA_HIP_EINVAL, "null handle");
+    HIP_TRY(hipSetDevice(h->device));
+    cvr::SpmmDesc a;
+    int rc = spmm_desc(h, k, d_X, d_Y, &a);
+    if (rc) return rc;
+    if (h->nchunks == 0) {
+        HIP_TRY(zero_rows_k(d_Y, (long long)h->M * k, (hipStream_t)stream));
+        return 0;
+    }
+    HIP_TRY(cvr::launch_spmm(a, h->prec, h->max_slots, (hipStream_t)stream));
+    return 0;
+}
+
+// ------------------------------------------------------------ batched SpMM
+struct cvr_hip_batch {
+    int device = 0, nmat = 0, k = 0, prec = 0, total = 0, grid = 0, max_slots = 1;
+    size_t lds = 0;
+    std::vector<cvr_hip_handle *> mats;
+    cvr::SpmmDesc *d_descs = nullptr;
+    int2 *d_queue = nullptr;
+    int *d_counter = nullptr;
+    std::vector<cvr::SpmmDesc> descs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long long nnz = 0;
+};
+
+extern "C" int cvr_hip_batch_create(int nmat, cvr_hip_handle *const *handles, int k, cvr_hip_batch **out) {
+    if (!out || nmat <= 0 || !handles) return set_error(PAVA_HIP_EINVAL, "bad batch arguments");
+    *out = nullptr;
+    if (k < 1 || k > 32) return set_error(PAVA_HIP_EINVAL, "k must be in [1, 32]");
+    cvr_hip_batch *b = new cvr_hip_batch();
+    b->nmat = nmat;
+    b->k = k;
+    b->mats.assign(handles, handles + nmat);
+    b->device = handles[0]->device;
+    b->prec = handles[0]->prec;
+    std::vector<int2> queue;
+    for (int i = 0; i < nmat; ++i) {
+        cvr_hip_handle *h = handles[i];
+        if (!h || h->device != b->device || h->prec != b->prec || h->col16 || h->nchunks == 0) {
+            delete b;
+            return set_error(PAVA_HIP_EINVAL,
+                             "batch matrices must be non-empty and share device and precision, 32-bit columns");
+        }
+        b->max_slots = std::max(b->max_slots, h->max_slots);
+        b->nnz += h->nnz;
+        for (int c = 0; c < h->nchunks; ++c) queue.push_back(make_int2(i, c));
+    }
+    b->total = (int)queue.size();
+    auto fail = [&](int code) {
+        if (b->d_descs) hipFree(b->d_descs);
+        if (b->d_queue) hipFree(b->d_queue);
+        if (b->d_counter) hipFree(b->d_counter);
+        delete b;
+        return code;
+    };
+    if (hipSetDevice(b->device) != hipSuccess) return fail(set_error(PAVA_HIP_EDEVICE, "hipSetDevice failed"));
+    if (hipMalloc(&b->d_descs, sizeof(cvr::SpmmDesc) * nmat) != hipSuccess ||
+        hipMalloc(&b->d_queue, sizeof(int2) * std::max(b->total, 1)) != hipSuccess ||
+        hipMalloc(&b->d_counter, 256) != hipSuccess)
+        return fail(set_error(PAVA_HIP_ENOMEM, "batch allocation failed"));
+    if (hipMemcpy(b->d_queue, queue.data(), sizeof(int2) * queue.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(b->d_counter, 0, 256) != hipSuccess)
+        return fail(set_error(PAVA_HIP_EDEVICE, "batch upload failed"));
+    b->lds = cvr::spmm_lds_bytes(b->max_slots, k);
+    if (cvr::spmm_batch_grid(b->prec, k, b->lds, &b->grid) != hipSuccess)
+        return fail(set_error(PAVA_HIP_EDEVICE, "occupancy query failed"));
+    b->descs.resize(nmat);
+    *out = b;
+    return 0;
+}
+
+extern "C" int cvr_hip_batch_set_operands(cvr_hip_batch *b, const float *const *d_X, float *const *d_Y) {
+    if (!b || !d_X || !d_Y) return set_error(PAVA_HIP_EINVAL, "null argument");
+    HIP_TRY(hipSetDevice(b->device));
+    for (int i = 0; i < b->nmat; ++i) {
+        int rc = spmm_desc(b->mats[i], b->k, d_X[i], d_Y[i], &b->descs[i]);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpy(b->d_descs, b->descs.data(), sizeof(cvr::SpmmDesc) * b->nmat, hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int batch_enqueue(cvr_hip_batch *b, hipStream_t st) {
+    HIP_TRY(cvr::launch_spmm_batch(b->d_descs, b->d_queue, b->total, b->d_counter, b->prec, b->k, b->lds, b->grid, st));
+    return 0;
+}
+
+extern "C" int cvr_hip_batch_spmm(cvr_hip_batch *b, void *stream) {
+    if (!b) return set_error(PAVA_HIP_EINVAL, "null batch");
+    HIP_TRY(hipSetDevice(b->device));
+    return batch_enqueue(b, (hipStream_t)stream);
+}
+
+extern "C" int cvr_hip_batch_graph(cvr_hip_batch *b, void *stream) {
+    if (!b || !stream) return set_error(PAVA_HIP_EINVAL, "graph capture needs a non-null stream");
+    HIP_TRY(hipSetDevice(b->device));
+    if (b->exec) {
+        hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = batch_enqueue(b, st);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_error(PAVA_HIP_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    return 0;
+}
+
+extern "C" int cvr_hip_batch_replay(cvr_hip_batch *b, void *stream) {
+    if (!b || !b->exec) return set_error(PAVA_HIP_ESTATE, "no captured graph");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return 0;
+}
+
+extern "C" int cvr_hip_batch_info(const cvr_hip_batch *b, int *nchunks_total, int *grid, long long *nnz) {
+    if (!b) return set_error(PAVA_HIP_EINVAL, "null batch");
+    if (nchunks_total) *nchunks_total = b->total;
+    if (grid) *grid = b->grid;
+    if (nnz) *nnz = b->nnz;
+    return 0;
+}
+
+extern "C" int cvr_hip_batch_destroy(cvr_hip_batch *b) {
+    if (!b) return 0;
+    hipSetDevice(b->device);
+    hipDeviceSynchronize();
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    hipFree(b->d_descs);
+    hipFree(b->d_queue);
+    hipFree(b->d_counter);
+    delete b;
+    return 0;
 }
 
 extern "C" int cvr_hip_last_recompute_count(cvr_hip_handle *h, int *count) {

@@ -84,6 +84,30 @@ struct FinishArgs {
     int nchunks;
 };
 
+// One matrix of a (batched) multi-RHS SpMM. 32-bit flagged columns only.
+struct SpmmDesc {
+    const void *vals;
+    const int *cols;
+    const int *chunk_rows;
+    const int *seg;
+    const int *split_n;
+    const int2 *split_tab;
+    const unsigned *empty_bits;
+    const int *fix;
+    const float *X;   // numCols x k, row-major
+    float *Y;         // numRows x k, row-major
+    double *carry;    // 2 * nchunks * k
+    long long layout_elems;
+    int nchunks, S, M, k, has_empty, pad_;
+};
+
+int spmm_kp(int k);
+size_t spmm_lds_bytes(int max_slots, int k);
+hipError_t launch_spmm(const SpmmDesc &a, int prec, int max_slots, hipStream_t st);
+hipError_t spmm_batch_grid(int prec, int k, size_t lds, int *grid);
+hipError_t launch_spmm_batch(const SpmmDesc *d_descs, const int2 *d_queue, int total, int *d_counter, int prec, int k,
+                             size_t lds, int grid, hipStream_t st);
+
 hipError_t launch_empty_rows(const int *rp, int M, unsigned *bits, int *status, hipStream_t st);
 hipError_t launch_convert(const ConvertArgs &a, hipStream_t st);
 hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st);

@@ -1,0 +1,107 @@
+"""GPU multi-RHS SpMM and the batched persistent launch (needs an MI355X).
+
+Reference: k independent FP64 oracle SpMVs on the same (FP32-valued) inputs.
+The GPU accumulates exact FP32 products in FP64 and rounds once, so each
+output is within one FP32 rounding (2^-24 relative) plus the FP64 summation
+bound 2 (n+64) 2^-53 sum|a x| of the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from matrices import edge_cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _f32_matrix(A):
+    A.vals = A.vals.astype(np.float32).astype(np.float64)
+    return A
+
+
+def _oracle_mm(O, A, X):
+    return np.stack([O.spmv_fma(A, X[:, j].astype(np.float64)) for j in range(X.shape[1])], axis=1)
+
+
+def _bound(O, A, X, yref):
+    lens = np.diff(A.rowOffsets)[:, None]
+    sabs = np.stack([O.row_abs(A, np.abs(X[:, j]).astype(np.float64)) for j in range(X.shape[1])], axis=1)
+    return 2.0 ** -24 * np.abs(yref) + 2 * (lens + 64) * 2.0 ** -53 * sabs + 1e-38
+
+
+@pytest.mark.parametrize("k", [1, 5, 8, 16, 32])
+def test_spmm_single_matrix(P, O, k):
+    import torch
+    rng = np.random.default_rng(k)
+    for name, A in edge_cases(P).items():
+        if A.nnz == 0:
+            continue
+        _say(f"spmm {name} k={k}")
+        A = _f32_matrix(A)
+        X = rng.uniform(-1, 1, (A.numCols, k)).astype(np.float32)
+        yref = _oracle_mm(O, A, X)
+        with P.CvrHip(A, steps_per_chunk=16, precision=P.PREC_F32) as h:
+            dX = torch.from_numpy(X).cuda()
+            dY = torch.full((A.numRows, k), float("nan"), dtype=torch.float32, device="cuda")
+            h.spmm(k, dX.data_ptr(), dY.data_ptr())
+            torch.cuda.synchronize()
+            Y = dY.cpu().numpy().astype(np.float64)
+        assert not np.isnan(Y).any(), name
+        assert np.all(np.abs(Y - yref) <= _bound(O, A, X, yref)), name
+
+
+def test_spmm_fp64_handle(P, O):
+    import torch
+    A = _f32_matrix(P.rmat(13, 8, seed=4))
+    X = np.random.default_rng(0).uniform(-1, 1, (A.numCols, 8)).astype(np.float32)
+    with P.CvrHip(A, steps_per_chunk=64) as h:
+        dX = torch.from_numpy(X).cuda()
+        dY = torch.empty((A.numRows, 8), dtype=torch.float32, device="cuda")
+        h.spmm(8, dX.data_ptr(), dY.data_ptr())
+        torch.cuda.synchronize()
+        Y = dY.cpu().numpy().astype(np.float64)
+    yref = _oracle_mm(O, A, X)
+    assert np.all(np.abs(Y - yref) <= _bound(O, A, X, yref))
+
+
+@pytest.mark.parametrize("k", [8, 32])
+def test_batched_persistent_and_graph_replay(P, O, k):
+    import torch
+    rng = np.random.default_rng(100 + k)
+    mats = []
+    for i in range(12):
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(2e4))))
+        A = P.synthetic(n, P.RL_POISSON, float(rng.uniform(5, 60)), col_struct=P.COL_MIXED, half_band=1000,
+                        band_frac=0.7, seed=1000 + i)
+        mats.append(_f32_matrix(A))
+    hs = [P.CvrHip(A, steps_per_chunk=32, precision=P.PREC_F32) for A in mats]
+    b = P.Batch(hs, k)
+    Xs = [torch.from_numpy(rng.uniform(-1, 1, (A.numCols, k)).astype(np.float32)).cuda() for A in mats]
+    Ys = [torch.full((A.numRows, k), float("nan"), dtype=torch.float32, device="cuda") for A in mats]
+    b.set_operands([x.data_ptr() for x in Xs], [y.data_ptr() for y in Ys])
+    _say("batch", b.info())
+    s = torch.cuda.Stream()
+    b.spmm(s.cuda_stream)
+    s.synchronize()
+    first = [y.cpu().numpy().astype(np.float64) for y in Ys]
+    for A, X, Y in zip(mats, Xs, first):
+        yref = _oracle_mm(O, A, X.cpu().numpy())
+        assert np.all(np.abs(Y - yref) <= _bound(O, A, X.cpu().numpy(), yref))
+    # graph: capture once, replay several times with eager work in between
+    b.capture(s.cuda_stream)
+    for rep in range(4):
+        for y in Ys:
+            y.fill_(float("nan"))
+        torch.cuda.synchronize()
+        b.replay(s.cuda_stream)
+        s.synchronize()
+        for Y0, y in zip(first, Ys):
+            assert np.array_equal(Y0, y.cpu().numpy().astype(np.float64)), rep
+    b.close()
+    for h in hs:
+        h.close()
