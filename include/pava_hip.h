@@ -218,6 +218,15 @@ int cvr_hip_batch_replay(cvr_hip_batch *b, void *stream);
 int cvr_hip_batch_info(const cvr_hip_batch *b, int *nchunks_total, int *grid, long long *nnz);
 int cvr_hip_batch_destroy(cvr_hip_batch *b);
 
+/* "csr_hip": plain CSR SpMV with lanes_per_row (power of two, 1..64) lanes per
+ * row, FP64 or FP32 values (x, y of the same type). The GPU stand-in for the
+ * MKL CSR block benchmark_CSR_SpMV (utils/pava_formats.cpp:39-47) and the
+ * selector's second format. */
+typedef struct csr_hip_handle csr_hip_handle;
+int csr_hip_create(const pava_csr_matrix *A, int lanes_per_row, int precision, int device, csr_hip_handle **out);
+int csr_hip_spmv(csr_hip_handle *h, const void *d_x, void *d_y, void *stream);
+int csr_hip_destroy(csr_hip_handle *h);
+
 /* Number of rows recomputed in FP64 by the last mixed SpMV (synchronises). */
 int cvr_hip_last_recompute_count(cvr_hip_handle *h, int *count);
 

@@ -79,7 +79,7 @@ EXPORTED = [
     "cvr_hip_time_spmv", "cvr_hip_time_spmv_x64", "cvr_hip_export_layout", "conduct_cvr_hip", "pava_bench_file",
     "pava_hip_last_error", "pava_hip_version", "cvr_hip_batch_create", "cvr_hip_batch_set_operands",
     "cvr_hip_batch_spmm", "cvr_hip_batch_graph", "cvr_hip_batch_replay", "cvr_hip_batch_info",
-    "cvr_hip_batch_destroy",
+    "cvr_hip_batch_destroy", "csr_hip_create", "csr_hip_spmv", "csr_hip_destroy",
 ]
 
 
@@ -401,3 +401,38 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class CsrHip:
+    """The csr_hip format: CSR SpMV with `lanes_per_row` lanes per row."""
+
+    def __init__(self, A: CSR, lanes_per_row: int = 8, precision: int = PREC_F64, device: int = 0):
+        L = lib()
+        L.csr_hip_create.argtypes = [C.POINTER(_CSR), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.csr_hip_spmv.argtypes = [C.c_void_p] * 4
+        L.csr_hip_destroy.argtypes = [C.c_void_p]
+        self._h = C.c_void_p()
+        s = A.c_struct()
+        _check(L.csr_hip_create(C.byref(s), lanes_per_row, precision, device, C.byref(self._h)))
+        self.shape = (A.numRows, A.numCols)
+        self.precision = precision
+
+    def spmv(self, x_ptr: int, y_ptr: int, stream: int = 0) -> None:
+        _check(lib().csr_hip_spmv(self._h, x_ptr, y_ptr, stream))
+
+    def close(self) -> None:
+        if self._h:
+            lib().csr_hip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -19,7 +19,7 @@ OBJDIR = HERE / "_build"
 LIB = HERE / "libpava_hip.so"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
-HIP_SOURCES = ["cvr_convert.hip", "cvr_spmv.hip", "cvr_spmm.hip", "cvr_handle.hip", "driver.hip"]
+HIP_SOURCES = ["cvr_convert.hip", "cvr_spmv.hip", "cvr_spmm.hip", "cvr_handle.hip", "csr_vector.hip", "driver.hip"]
 CXX_SOURCES = ["host_io.cpp", "gen.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}", "-Wall", "-Wno-unused-function"]

@@ -232,3 +232,22 @@ def test_recompute_needs_fp64_x(P):
         dy = torch.zeros(A.numRows, dtype=torch.float32, device="cuda")
         with pytest.raises(P.PavaError):
             h.spmv_mixed(dx.data_ptr(), dy.data_ptr(), 0)
+
+
+@pytest.mark.parametrize("L", [1, 4, 16, 64])
+def test_csr_hip_format(P, O, mats, L):
+    import torch
+    for name, A in mats.items():
+        x = P.fill_uniform(A.numCols, 9)
+        yref = O.spmv_fma(A, x)
+        lens = np.diff(A.rowOffsets)
+        sabs = O.row_abs(A, x)
+        with P.CsrHip(A, lanes_per_row=L) as h:
+            dx = torch.from_numpy(x).cuda()
+            dy = torch.full((A.numRows,), float("nan"), dtype=torch.float64, device="cuda")
+            h.spmv(dx.data_ptr(), dy.data_ptr())
+            torch.cuda.synchronize()
+            y = dy.cpu().numpy()
+        if L == 1:
+            assert np.array_equal(y, yref), name  # one lane per row: the oracle's order
+        assert np.all(np.abs(y - yref) <= 2 * (lens + 64) * 2.0 ** -53 * sabs), name
