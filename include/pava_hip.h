@@ -227,6 +227,30 @@ int csr_hip_create(const pava_csr_matrix *A, int lanes_per_row, int precision, i
 int csr_hip_spmv(csr_hip_handle *h, const void *d_x, void *d_y, void *stream);
 int csr_hip_destroy(csr_hip_handle *h);
 
+/* ------------------------------------------------------------- selector
+ * No counterpart in pava (it benchmarks every format instead of choosing).
+ * 16 cheap CSR features (log sizes, row-length statistics, bandwidth, gather
+ * locality), then a 3-layer MLP (n_in -> n_hidden -> n_hidden -> n_out, ReLU,
+ * FP32) trained and run on the GPU by this library's own kernels. Parameter
+ * layout: W1[n_hidden x n_in], b1, W2[n_hidden x n_hidden], b2,
+ * W3[n_out x n_hidden], b3 (row-major). Loss: masked mean squared error. */
+#define PAVA_N_FEATURES 16
+int pava_csr_features(const pava_csr_matrix *A, float *out /* PAVA_N_FEATURES */);
+typedef struct pava_mlp pava_mlp;
+int pava_mlp_create(int n_in, int n_hidden, int n_out, unsigned long long seed, int device, pava_mlp **out);
+int pava_mlp_num_params(const pava_mlp *m);
+int pava_mlp_get_params(const pava_mlp *m, float *host);
+int pava_mlp_set_params(pava_mlp *m, const float *host);
+/* Y[batch x n_out] = MLP(X[batch x n_in]); host pointers. */
+int pava_mlp_predict(pava_mlp *m, int batch, const float *X, float *Y);
+/* Loss and flat gradient of one batch (host pointers; T targets, M mask). */
+int pava_mlp_loss_grad(pava_mlp *m, int batch, const float *X, const float *T, const float *M, float *loss,
+                       float *grad);
+/* Adam on shuffled minibatches; loss_hist[epoch] = mean minibatch loss. */
+int pava_mlp_train(pava_mlp *m, int n, const float *X, const float *T, const float *M, int epochs, int batch,
+                   float lr, unsigned long long seed, float *loss_hist);
+int pava_mlp_destroy(pava_mlp *m);
+
 /* Number of rows recomputed in FP64 by the last mixed SpMV (synchronises). */
 int cvr_hip_last_recompute_count(cvr_hip_handle *h, int *count);
 

@@ -79,7 +79,9 @@ EXPORTED = [
     "cvr_hip_time_spmv", "cvr_hip_time_spmv_x64", "cvr_hip_export_layout", "conduct_cvr_hip", "pava_bench_file",
     "pava_hip_last_error", "pava_hip_version", "cvr_hip_batch_create", "cvr_hip_batch_set_operands",
     "cvr_hip_batch_spmm", "cvr_hip_batch_graph", "cvr_hip_batch_replay", "cvr_hip_batch_info",
-    "cvr_hip_batch_destroy", "csr_hip_create", "csr_hip_spmv", "csr_hip_destroy",
+    "cvr_hip_batch_destroy", "csr_hip_create", "csr_hip_spmv", "csr_hip_destroy", "pava_csr_features",
+    "pava_mlp_create", "pava_mlp_num_params", "pava_mlp_get_params", "pava_mlp_set_params", "pava_mlp_predict",
+    "pava_mlp_loss_grad", "pava_mlp_train", "pava_mlp_destroy",
 ]
 
 

@@ -135,9 +135,9 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     if ((rc = dev_alloc(h, (void **)&h->d_split_n, (size_t)h->nchunks * 4))) return rc;
     if ((rc = dev_alloc(h, (void **)&h->d_split_tab, (size_t)h->nchunks * cvr::kSplitCap * 8))) return rc;
     if ((rc = dev_alloc(h, (void **)&h->d_fix, (size_t)h->nchunks * 8))) return rc;
-    if ((rc = dev_alloc(h, (void **)&h->d_carry, (size_t)h->nchunks * 16))) return rc;
+    if ((rc = dev_alloc(h, (void **)&h->d_carry, (size_t)h->nchunks * 16 + 16))) return rc;
     if (o->recompute && h->prec != CVR_HIP_PREC_F64) {
-        if ((rc = dev_alloc(h, (void **)&h->d_carry_abs, (size_t)h->nchunks * 16))) return rc;
+        if ((rc = dev_alloc(h, (void **)&h->d_carry_abs, (size_t)h->nchunks * 16 + 16))) return rc;
         if ((rc = dev_alloc(h, (void **)&h->d_rflag, (size_t)M + 64))) return rc;
         if ((rc = dev_alloc(h, (void **)&h->d_rc_count, (size_t)h->nchunks * 4 + 4))) return rc;
     }

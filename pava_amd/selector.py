@@ -1,0 +1,227 @@
+"""Autotuner and learned selector for the SpMV path (no pava counterpart).
+
+* ``CONFIGS`` -- the candidate space: format (cvr_hip / csr_hip), value
+  precision (FP64; FP32 with FP64 recompute of rows whose cancellation ratio
+  exceeds 64, which keeps every row within ~1e-5 of FP64; the same with 16-bit
+  chunk-relative columns) and tile (CVR steps per chunk S, or CSR lanes per
+  row L).
+* ``autotune`` -- exhaustive search: builds each candidate on the GPU, times
+  it with hipEvents, returns per-config times (inf where a candidate does not
+  apply, e.g. a chunk spanning more than 65536 columns for 16-bit columns).
+* ``features`` -- 16 CSR features (C ABI ``pava_csr_features``).
+* ``Mlp`` -- 3-layer MLP trained on the GPU by the library's own kernels
+  (``pava_mlp_*``): it regresses log2(time / best time) of every config from
+  the standardised features; the selector picks the config with the lowest
+  prediction.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import (CSR, CvrHip, CsrHip, PREC_F32, PREC_F64, PavaError, _check, lib)
+
+N_FEATURES = 16
+
+
+@dataclass(frozen=True)
+class Config:
+    fmt: str        # "cvr" | "csr"
+    prec: str       # "f64" | "f32rc" | "f32rc16"
+    tile: int       # S for cvr, L for csr
+
+    @property
+    def name(self) -> str:
+        return f"{self.fmt}-{self.prec}-{self.tile}"
+
+
+CONFIGS = ([Config("cvr", "f64", s) for s in (32, 128, 512)]
+           + [Config("cvr", "f32rc", s) for s in (32, 128, 512)]
+           + [Config("cvr", "f32rc16", s) for s in (32, 128, 512)]
+           + [Config("csr", "f64", l) for l in (1, 4, 16, 64)])
+DEFAULT = Config("cvr", "f64", 128)  # the fallback without a model
+
+
+def features(A: CSR) -> np.ndarray:
+    L = lib()
+    L.pava_csr_features.argtypes = [C.c_void_p, C.c_void_p]
+    out = np.zeros(N_FEATURES, np.float32)
+    s = A.c_struct()
+    _check(L.pava_csr_features(C.byref(s), out.ctypes.data))
+    return out
+
+
+def _time(fn, iters: int):
+    import torch
+    s = torch.cuda.current_stream()
+    for _ in range(2):
+        fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(iters):
+        fn()
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / iters
+
+
+def build(A: CSR, cfg: Config):
+    if cfg.fmt == "csr":
+        return CsrHip(A, lanes_per_row=cfg.tile, precision=PREC_F64)
+    if cfg.prec == "f64":
+        return CvrHip(A, steps_per_chunk=cfg.tile)
+    return CvrHip(A, steps_per_chunk=cfg.tile, precision=PREC_F32, recompute=True, kappa_threshold=64.0,
+                  col_index_bits=16 if cfg.prec == "f32rc16" else 32)
+
+
+def autotune(A: CSR, x: np.ndarray, iters: int = 5, configs=CONFIGS) -> np.ndarray:
+    """Milliseconds per SpMV of every config (inf where it does not apply).
+    Timed with events on torch's current stream (the launches use it)."""
+    import torch
+    dev = torch.device("cuda:0")
+    x64 = torch.from_numpy(np.ascontiguousarray(x, np.float64)).to(dev)
+    x32 = x64.float()
+    y64 = torch.empty(A.numRows, dtype=torch.float64, device=dev)
+    y32 = torch.empty(A.numRows, dtype=torch.float32, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    out = np.full(len(configs), np.inf)
+    for i, cfg in enumerate(configs):
+        try:
+            h = build(A, cfg)
+        except PavaError:
+            continue
+        try:
+            if cfg.fmt == "csr":
+                fn = lambda: h.spmv(x64.data_ptr(), y64.data_ptr(), st)
+            elif cfg.prec == "f64":
+                fn = lambda: h.spmv(x64.data_ptr(), y64.data_ptr(), st)
+            else:
+                fn = lambda: h.spmv_mixed(x32.data_ptr(), y32.data_ptr(), x64.data_ptr(), st)
+            out[i] = _time(fn, iters)
+        finally:
+            h.close()
+    return out
+
+
+class Mlp:
+    """3-layer MLP on the GPU (pava_mlp_* kernels)."""
+
+    def __init__(self, n_in: int = N_FEATURES, n_hidden: int = 64, n_out: int = len(CONFIGS), seed: int = 0,
+                 device: int = 0):
+        L = lib()
+        P = C.c_void_p
+        L.pava_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(P)]
+        L.pava_mlp_num_params.argtypes = [P]
+        L.pava_mlp_get_params.argtypes = [P, P]
+        L.pava_mlp_set_params.argtypes = [P, P]
+        L.pava_mlp_predict.argtypes = [P, C.c_int, P, P]
+        L.pava_mlp_loss_grad.argtypes = [P, C.c_int, P, P, P, P, P]
+        L.pava_mlp_train.argtypes = [P, C.c_int, P, P, P, C.c_int, C.c_int, C.c_float, C.c_ulonglong, P]
+        L.pava_mlp_destroy.argtypes = [P]
+        self.shape = (n_in, n_hidden, n_out)
+        self._m = P()
+        _check(L.pava_mlp_create(n_in, n_hidden, n_out, seed, device, C.byref(self._m)))
+        self.np = L.pava_mlp_num_params(self._m)
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.np, np.float32)
+        _check(lib().pava_mlp_get_params(self._m, p.ctypes.data))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.np
+        _check(lib().pava_mlp_set_params(self._m, p.ctypes.data))
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.empty((X.shape[0], self.shape[2]), np.float32)
+        _check(lib().pava_mlp_predict(self._m, X.shape[0], X.ctypes.data, Y.ctypes.data))
+        return Y
+
+    def loss_grad(self, X, T, M):
+        X, T, M = (np.ascontiguousarray(a, np.float32) for a in (X, T, M))
+        loss = np.zeros(1, np.float32)
+        g = np.empty(self.np, np.float32)
+        _check(lib().pava_mlp_loss_grad(self._m, X.shape[0], X.ctypes.data, T.ctypes.data, M.ctypes.data,
+                                        loss.ctypes.data, g.ctypes.data))
+        return float(loss[0]), g
+
+    def train(self, X, T, M, epochs: int = 200, batch: int = 64, lr: float = 2e-3, seed: int = 1) -> np.ndarray:
+        X, T, M = (np.ascontiguousarray(a, np.float32) for a in (X, T, M))
+        hist = np.zeros(epochs, np.float32)
+        _check(lib().pava_mlp_train(self._m, X.shape[0], X.ctypes.data, T.ctypes.data, M.ctypes.data, epochs, batch,
+                                    lr, seed, hist.ctypes.data))
+        return hist
+
+    def close(self):
+        if self._m:
+            lib().pava_mlp_destroy(self._m)
+            self._m = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def targets(times: np.ndarray):
+    """log2(time / best) per config and a validity mask (rows: matrices)."""
+    t = np.asarray(times, np.float64)
+    best = np.min(np.where(np.isfinite(t), t, np.inf), axis=1, keepdims=True)
+    mask = np.isfinite(t)
+    T = np.where(mask, np.log2(np.where(mask, t, 1.0) / best), 0.0)
+    return T.astype(np.float32), mask.astype(np.float32)
+
+
+class Selector:
+    """Standardised features -> MLP -> argmin predicted slowdown (valid configs)."""
+
+    def __init__(self, mlp: Mlp, mu: np.ndarray, sd: np.ndarray):
+        self.mlp, self.mu, self.sd = mlp, mu, sd
+
+    @staticmethod
+    def fit(F: np.ndarray, times: np.ndarray, epochs: int = 300, seed: int = 0, lr: float = 2e-3) -> tuple:
+        mu = F.mean(0)
+        sd = F.std(0) + 1e-6
+        T, M = targets(times)
+        mlp = Mlp(seed=seed)
+        hist = mlp.train((F - mu) / sd, T, M, epochs=epochs, batch=64, lr=lr, seed=seed + 1)
+        return Selector(mlp, mu, sd), hist
+
+    def choose(self, F: np.ndarray, valid: np.ndarray | None = None) -> np.ndarray:
+        P = self.mlp.predict((np.atleast_2d(F) - self.mu) / self.sd)
+        if valid is not None:
+            P = np.where(valid, P, np.inf)
+        return np.argmin(P, axis=1)
+
+
+def slowdowns(times: np.ndarray, choice: np.ndarray) -> np.ndarray:
+    """Measured time of the chosen config over the best config, per matrix."""
+    t = np.where(np.isfinite(times), times, np.inf)
+    best = t.min(1)
+    return t[np.arange(t.shape[0]), choice] / best
+
+
+def gen_params(rng: np.random.Generator, max_rows: int = 1_000_000) -> dict:
+    """One random member of the selector's synthetic family: rows log-uniform in
+    [1e4, max_rows], row lengths constant / Poisson / lognormal (sigma
+    0.5-2) / power law (alpha 1.8-2.5), columns banded / block-diagonal (block
+    32-512) / uniform."""
+    rows = int(math.exp(rng.uniform(math.log(1e4), math.log(max_rows))))
+    dist = int(rng.integers(0, 4))
+    mean = float(math.exp(rng.uniform(math.log(4), math.log(64))))
+    shape = 1.0
+    if dist == 2:
+        shape = float(rng.uniform(0.5, 2.0))
+    elif dist == 3:
+        shape = float(rng.uniform(1.8, 2.5))
+        mean = float(rng.uniform(1, 8))  # x_min of the Pareto law
+    col = int(rng.integers(0, 3))
+    return dict(rows=rows, row_dist=dist, len_param=mean, shape=shape, col_struct=col,
+                half_band=int(rng.integers(8, 4096)), block=int(2 ** rng.integers(5, 10)), band_frac=0.7,
+                value_dist=0, max_len=min(rows, 20000), seed=int(rng.integers(0, 2 ** 31)))

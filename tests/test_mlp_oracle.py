@@ -1,0 +1,62 @@
+"""Pins the CPU MLP oracle (oracle/mlp_ref.py) by finite differences, and
+checks the selector's pure-host helpers (CPU only)."""
+import numpy as np
+
+from oracle import mlp_ref
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    I, H, O, B = 5, 7, 3, 11
+    n = H * I + H + H * H + H + O * H + O
+    p = rng.normal(scale=0.5, size=n)
+    X = rng.normal(size=(B, I))
+    T = rng.normal(size=(B, O))
+    M = (rng.random((B, O)) < 0.8).astype(float)
+    L, g = mlp_ref.loss_grad(p, X, T, M, I, H, O)
+    assert abs(L - mlp_ref.loss(p, X, T, M, I, H, O)) < 1e-12
+    eps = 1e-6
+    for i in rng.choice(n, 40, replace=False):
+        e = np.zeros(n)
+        e[i] = eps
+        fd = (mlp_ref.loss(p + e, X, T, M, I, H, O) - mlp_ref.loss(p - e, X, T, M, I, H, O)) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_mlp_oracle_layout_sizes():
+    I, H, O = 16, 64, 13
+    n = H * I + H + H * H + H + O * H + O
+    parts = mlp_ref.unpack(np.arange(n, dtype=float), I, H, O)
+    assert [a.shape for a in parts] == [(H, I), (H,), (H, H), (H,), (O, H), (O,)]
+    assert parts[1][0] == H * I  # b1 follows W1
+
+
+def test_targets_and_slowdowns():
+    from pava_amd import selector as S
+    t = np.array([[2.0, 1.0, np.inf], [4.0, 8.0, 2.0]])
+    T, M = S.targets(t)
+    assert np.allclose(T, [[1, 0, 0], [1, 2, 0]]) and np.array_equal(M, [[1, 1, 0], [1, 1, 1]])
+    assert np.allclose(S.slowdowns(t, np.array([0, 2])), [2.0, 1.0])
+
+
+def test_features_closed_forms(P):
+    from pava_amd import selector as S
+    A = P.laplacian2d(64, 64)
+    f = S.features(A)
+    assert abs(f[0] - 12) < 1e-6 and abs(f[15] - 12) < 1e-6       # log2 rows / cols
+    assert abs(f[1] - np.log2(A.nnz)) < 1e-5
+    mean = A.nnz / A.numRows
+    assert abs(f[2] - np.log2(1 + mean)) < 1e-5
+    assert abs(f[4] - np.log2(6)) < 1e-6                          # max row length 5
+    assert f[6] == 0 and abs(f[7] - 4 / 4096) < 1e-7               # no empty rows; 4 corners of length 3
+    assert f[10] == 1.0                                            # all within +-1024 of the diagonal
+    E = P.CSR(4, 4, np.array([0, 0, 2, 2, 3]), np.array([0, 3, 1]), np.ones(3))
+    g = S.features(E)
+    assert abs(g[6] - 0.5) < 1e-6 and abs(g[7] - 1.0) < 1e-6       # 2 of 4 rows empty, all < 4
+
+
+def test_config_space():
+    from pava_amd import selector as S
+    names = [c.name for c in S.CONFIGS]
+    assert len(names) == len(set(names)) == 13
+    assert S.DEFAULT in S.CONFIGS
