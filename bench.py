@@ -74,6 +74,128 @@ def cpu_baseline(O, A, x, budget_s: float = 10.0):
     }
 
 
+def run_precision(P, O, A, x, steps: int, S: int = 256) -> dict:
+    """FP64 vs FP32 / BF16 storage (FP32 compensated accumulation), with and
+    without FP64 recompute of rows whose cancellation ratio exceeds 64, and with
+    16-bit chunk-relative columns; per-row error vs the FP64 oracle."""
+    import torch
+    dev = torch.device("cuda:0")
+    yref = O.spmv_fma(A, x)
+    x64 = torch.from_numpy(x).to(dev)
+    x32 = x64.float()
+    variants = [("f64", dict(precision=0), 8, 4, 8),
+                ("f32", dict(precision=1), 4, 4, 4),
+                ("f32_recompute", dict(precision=1, recompute=True), 4, 4, 4),
+                ("f32_col16_recompute", dict(precision=1, recompute=True, col_index_bits=16), 4, 2.125, 4),
+                ("bf16", dict(precision=2), 2, 4, 4),
+                ("bf16_col16", dict(precision=2, col_index_bits=16), 2, 2.125, 4)]
+    out = {}
+    for name, kw, sv, si, sx in variants:
+        try:
+            h = P.CvrHip(A, steps_per_chunk=S, **kw)
+        except P.PavaError as e:
+            out[name] = {"error": str(e)}
+            continue
+        rc = kw.get("recompute", False)
+        if kw["precision"] == 0:
+            y = torch.empty(A.numRows, dtype=torch.float64, device=dev)
+            tot, main = h.time_spmv(x64.data_ptr(), y.data_ptr(), 5, steps)
+            h.spmv(x64.data_ptr(), y.data_ptr(), 0)
+        else:
+            y = torch.empty(A.numRows, dtype=torch.float32, device=dev)
+            tot, main = h.time_spmv(x32.data_ptr(), y.data_ptr(), 5, steps, x64.data_ptr() if rc else 0)
+            h.spmv_mixed(x32.data_ptr(), y.data_ptr(), x64.data_ptr() if rc else 0, 0)
+        torch.cuda.synchronize()
+        yy = y.double().cpu().numpy()
+        st = O.error_stats(yy, yref)
+        B = A.nnz * (sv + si) + (A.numRows + 1) * 4 + A.numCols * sx + A.numRows * sx
+        out[name] = {"ms": round(tot, 5), "main_kernel_ms": round(main, 5), "GB_s": round(B / tot / 1e6, 1),
+                     "nnz_per_s": A.nnz / (tot * 1e-3), "bytes": int(B), "max_rel": st["max_rel"],
+                     "median_rel": st["median_rel"],
+                     "recomputed_rows": h.last_recompute_count() if rc else 0}
+        h.close()
+    base = out["f64"]["nnz_per_s"]
+    for v in out.values():
+        if "nnz_per_s" in v:
+            v["speedup_vs_f64"] = round(v["nnz_per_s"] / base, 3)
+    return out
+
+
+def run_batch(P, O, k: int, nmat: int, steps: int, S: int = 64, check: int = 4) -> dict:
+    """nmat random matrices (rows log-uniform in [1e3, 2e5], Poisson row
+    lengths with mean U[5, 60], 70% of columns within +-1000 of the diagonal,
+    FP32 N(0,1) values) times FP32 RHS blocks of k columns, in one persistent
+    launch replayed from a hipGraph; compared with plain FP64 CVR-HIP SpMVs of
+    the same matrices (one launch each)."""
+    import torch
+    rng = np.random.default_rng(4)
+    dev = torch.device("cuda:0")
+    mats, hs, h64 = [], [], []
+    t0 = time.perf_counter()
+    for i in range(nmat):
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+        A = P.synthetic(n, P.RL_POISSON, float(rng.uniform(5, 60)), col_struct=P.COL_MIXED, half_band=1000,
+                        band_frac=0.7, value_dist=0, seed=10_000 + i)
+        A.vals = A.vals.astype(np.float32).astype(np.float64)
+        mats.append(A)
+        hs.append(P.CvrHip(A, steps_per_chunk=S, precision=P.PREC_F32))
+        h64.append(P.CvrHip(A, steps_per_chunk=256))
+        if i % 64 == 63:
+            print(f"[bench] batch: {i + 1}/{nmat} matrices built ({time.perf_counter() - t0:.0f}s)", file=sys.stderr,
+                  flush=True)
+    nnz = sum(A.nnz for A in mats)
+    Xs = [torch.rand((A.numCols, k), dtype=torch.float32, device=dev) * 2 - 1 for A in mats]
+    Ys = [torch.empty((A.numRows, k), dtype=torch.float32, device=dev) for A in mats]
+    b = P.Batch(hs, k)
+    b.set_operands([t.data_ptr() for t in Xs], [t.data_ptr() for t in Ys])
+    st = torch.cuda.Stream(device=dev)
+    b.capture(st.cuda_stream)
+    for _ in range(3):
+        b.replay(st.cuda_stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(steps):
+        b.replay(st.cuda_stream)
+    e1.record(st)
+    e1.synchronize()
+    ms_batch = e0.elapsed_time(e1) / steps
+    # plain FP64 CVR-HIP, one SpMV per matrix (x = first column of the block)
+    x64 = [X[:, 0].double().contiguous() for X in Xs]
+    y64 = [torch.empty(A.numRows, dtype=torch.float64, device=dev) for A in mats]
+    for _ in range(2):
+        for h, xx, yy in zip(h64, x64, y64):
+            h.spmv(xx.data_ptr(), yy.data_ptr(), st.cuda_stream)
+    e0.record(st)
+    reps = max(1, steps // 4)
+    for _ in range(reps):
+        for h, xx, yy in zip(h64, x64, y64):
+            h.spmv(xx.data_ptr(), yy.data_ptr(), st.cuda_stream)
+    e1.record(st)
+    e1.synchronize()
+    ms_fp64 = e0.elapsed_time(e1) / reps
+    # accuracy on a sample of matrices: FP64 oracle on the same FP32 inputs
+    errs = []
+    for i in range(min(check, nmat)):
+        X = Xs[i].cpu().numpy().astype(np.float64)
+        Y = Ys[i].cpu().numpy().astype(np.float64)
+        for j in range(k):
+            errs.append(O.error_stats(Y[:, j], O.spmv_fma(mats[i], X[:, j])))
+    rows = sum(A.numRows for A in mats)
+    cols = sum(A.numCols for A in mats)
+    Bb = nnz * 8 + (rows + 1) * 4 + cols * k * 4 + rows * k * 4
+    res = {"matrices": nmat, "k": k, "nnz": nnz, "rows": rows, "chunks": b.info()["chunks"], "grid": b.info()["grid"],
+           "ms_batch": round(ms_batch, 4), "GB_s": round(Bb / ms_batch / 1e6, 1),
+           "nnz_rhs_per_s": nnz * k / (ms_batch * 1e-3),
+           "ms_fp64_spmv_all": round(ms_fp64, 4), "fp64_nnz_per_s": nnz / (ms_fp64 * 1e-3),
+           "speedup_nnz_rhs_vs_fp64_spmv": round((nnz * k / ms_batch) / (nnz / ms_fp64), 2),
+           "max_rel": max(e["max_rel"] for e in errs), "median_rel": float(np.median([e["median_rel"] for e in errs])),
+           "build_s": round(time.perf_counter() - t0, 1)}
+    b.close()
+    for h in hs + h64:
+        h.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +205,10 @@ def main():
     ap.add_argument("--steps-per-chunk", type=int, default=0, help="CVR chunk steps (0 = default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--variant", choices=["none", "precision", "batch"], default="none",
+                    help="precision: FP64/FP32/BF16 storage on the workload's matrix; batch: 512-matrix SpMM")
+    ap.add_argument("--k", type=int, default=8, help="batch variant: right-hand sides")
+    ap.add_argument("--nmat", type=int, default=512, help="batch variant: matrices")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures the single-GPU SpMV path only; this build has no multi-GPU path "
@@ -92,7 +218,21 @@ def main():
     import torch
     import pava_amd as P
 
+    if args.variant == "batch":
+        import oracle as O
+        res = run_batch(P, O, args.k, args.nmat, args.steps)
+        print(json.dumps({"variant": "batch", **res}), flush=True)
+        return
+
     w = WORKLOADS[args.workload]
+    if args.variant == "precision":
+        import oracle as O
+        A = make_matrix(P, w)
+        x = P.fill_uniform(A.numCols, 0)
+        res = run_precision(P, O, A, x, args.steps, args.steps_per_chunk or 256)
+        print(json.dumps({"variant": "precision", "workload": args.workload, "rows": A.numRows, "nnz": A.nnz,
+                          "results": res}), flush=True)
+        return
     t0 = time.perf_counter()
     A = make_matrix(P, w)
     t_gen = time.perf_counter() - t0

@@ -121,7 +121,7 @@ void orc_cvr64_sizes(int m, int nnz, int S, int *nchunks, long long *layout_elem
     }
     *nchunks = nc;
     *layout_elems = L;
-    *seg_cap = (long long)m + (long long)SEG_EXTRA * nc + 256;
+    *seg_cap = (long long)m + (long long)SEG_EXTRA * nc + 512;
 }
 
 static int row_of_element(const int *rp, int m, int b) {

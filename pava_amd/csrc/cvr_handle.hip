@@ -120,7 +120,7 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     }
     if (h->layout_elems > (long long)INT32_MAX + 4096)
         return set_error(PAVA_HIP_ERANGE, "layout beyond int32 elements");
-    h->seg_cap = (long long)M + (long long)cvr::kSegExtra * h->nchunks + 256;
+    h->seg_cap = (long long)M + (long long)cvr::kSegExtra * h->nchunks + 512;
     int rc;
     const int vs = value_size(h->prec);
     if ((rc = dev_alloc(h, &h->d_vals, (size_t)h->layout_elems * vs))) return rc;
@@ -145,6 +145,7 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     if ((rc = dev_alloc(h, (void **)&h->d_status, 64))) return rc;
 
     HIP_TRY(hipMemsetAsync(h->d_status, 0, 64, h->stream));
+    HIP_TRY(hipMemsetAsync(h->d_chunk_rows, 0, (size_t)(h->nchunks + 1) * 4, h->stream));
     // the segment list beyond the written entries is read by the SpMV window
     // (values unused); clear it once so it is defined
     HIP_TRY(hipMemsetAsync(h->d_seg, 0, (size_t)h->seg_cap * 4, h->stream));
@@ -366,6 +367,7 @@ static int enqueue_tail(cvr_hip_handle *h, void *d_y, const double *d_x64, hipSt
         f.rc_count = h->d_rc_count;
         f.kappa = (float)h->opts.kappa_threshold;
         f.nchunks = h->nchunks;
+        f.M = h->M;
         HIP_TRY(cvr::launch_finish(f, st));
     } else if (h->n_fix > 0) {
         HIP_TRY(cvr::launch_fixup(h->d_fix, h->d_carry, d_y, h->nchunks, h->prec, st));

@@ -81,7 +81,7 @@ struct FinishArgs {
     float *y;
     int *rc_count;       // per chunk: rows recomputed by the last SpMV
     float kappa;
-    int nchunks;
+    int nchunks, M;
 };
 
 // One matrix of a (batched) multi-RHS SpMM. 32-bit flagged columns only.

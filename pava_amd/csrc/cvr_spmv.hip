@@ -97,12 +97,18 @@ __device__ __forceinline__ void store_y(void *y, int r, double v) {
     reinterpret_cast<YT *>(y)[r] = (YT)v;
 }
 
+// Picks one of four wave-uniform register copies by a uniform index.
+__device__ __forceinline__ int pick4(int r, int s0, int s1, int s2, int s3) {
+    return r == 0 ? s0 : r == 1 ? s1 : r == 2 ? s2 : s3;
+}
+
 template <typename VT, typename XT, typename YT, bool COMP, bool COL16, bool KAPPA>
 __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int c = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wv;
     if (c >= a.nchunks) return;
+    constexpr int G = kStepGroup;
 
     __shared__ double slots[kWavesPerBlock][kSlotCap];
     __shared__ double slots_abs[KAPPA ? kWavesPerBlock : 1][KAPPA ? kSlotCap : 1];
@@ -113,54 +119,83 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
     const int zr = a.chunk_rows[c];
     const int *__restrict__ segs = a.seg + zr + kSegExtra * c;
     const XT *__restrict__ x = reinterpret_cast<const XT *>(a.x);
-
-    int w0 = segs[lane], w1 = segs[64 + lane];
-    int wbase = 0, segp = 0;
-
     const VT *__restrict__ vp = reinterpret_cast<const VT *>(a.vals) + base + lane;
     const int *__restrict__ cp32 = reinterpret_cast<const int *>(a.cols) + base + lane;
     const unsigned short *__restrict__ cp16 = reinterpret_cast<const unsigned short *>(a.cols) + base + lane;
     const uint64_t *__restrict__ emp = a.end_mask + (base >> 6);
     const int cbase = COL16 ? a.col_base[c] : 0;
 
-    Acc<VT, XT, COMP, KAPPA> acc;
-
-    VT vb[kStepGroup];
-    int cb[kStepGroup];
+    // Pipeline, one 8-step group deep: while group g is processed, the x
+    // gathers and the segment-list entries of group g+1, the values of group
+    // g+1 and the columns of group g+2 are in flight. The segment entries of a
+    // group (<= 256 in four registers per lane, read with ds_bpermute) are
+    // issued together with its x gathers, so destination lookups never wait.
+    auto col_of = [&](int cv) -> int { return COL16 ? cbase + cv : (cv & 0x7fffffff); };
+    auto load_cols = [&](int g0, int (&cc)[G], uint64_t (&em)[G]) {
 #pragma unroll
-    for (int i = 0; i < kStepGroup; ++i) {
-        vb[i] = vp[i * 64];
-        cb[i] = COL16 ? (int)cp16[i * 64] : cp32[i * 64];
-    }
-    for (int g = 0; g < steps; g += kStepGroup) {
-        XT xv[kStepGroup];
-#pragma unroll
-        for (int i = 0; i < kStepGroup; ++i) xv[i] = COL16 ? x[cbase + cb[i]] : x[cb[i] & 0x7fffffff];
-        uint64_t em[kStepGroup];
+        for (int i = 0; i < G; ++i) cc[i] = COL16 ? (int)cp16[(size_t)(g0 + i) * 64] : cp32[(size_t)(g0 + i) * 64];
         if (COL16) {
 #pragma unroll
-            for (int i = 0; i < kStepGroup; ++i) em[i] = emp[g + i];
+            for (int i = 0; i < G; ++i) em[i] = emp[g0 + i];
         }
-        VT vn[kStepGroup];
-        int cn[kStepGroup];
-        const bool more = (g + kStepGroup) < steps;
-        if (more) {
+    };
+    auto group_ends = [&](const int (&cc)[G], const uint64_t (&em)[G]) -> int {
+        int e = 0;
 #pragma unroll
-            for (int i = 0; i < kStepGroup; ++i) {
-                vn[i] = vp[(size_t)(g + kStepGroup + i) * 64];
-                cn[i] = COL16 ? (int)cp16[(size_t)(g + kStepGroup + i) * 64] : cp32[(size_t)(g + kStepGroup + i) * 64];
-            }
+        for (int i = 0; i < G; ++i) e += __popcll(COL16 ? em[i] : __ballot(cc[i] < 0));
+        return e;
+    };
+
+    Acc<VT, XT, COMP, KAPPA> acc;
+    VT vb[G], vn[G];
+    int cb[G], cn[G], cnn[G];
+    uint64_t eb[G], en[G], enn[G];
+    XT xb[G], xn[G];
+    int s0, s1, s2, s3, n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+
+    // prologue: group 0 fully issued, columns of group 1 in flight
+    load_cols(0, cb, eb);
+#pragma unroll
+    for (int i = 0; i < G; ++i) vb[i] = vp[(size_t)i * 64];
+#pragma unroll
+    for (int i = 0; i < G; ++i) xb[i] = x[col_of(cb[i])];
+    s0 = segs[lane], s1 = segs[64 + lane], s2 = segs[128 + lane], s3 = segs[192 + lane];
+    if (G < steps) load_cols(G, cn, en);
+    int sbase = 0, segp = 0;
+
+    for (int g = 0; g < steps; g += G) {
+        const bool has1 = (g + G) < steps, has2 = (g + 2 * G) < steps;
+        const int sbase_next = sbase + group_ends(cb, eb);
+        if (has1) {
+#pragma unroll
+            for (int i = 0; i < G; ++i) xn[i] = x[col_of(cn[i])];
+            n0 = segs[sbase_next + lane];
+            n1 = segs[sbase_next + 64 + lane];
+            n2 = segs[sbase_next + 128 + lane];
+            n3 = segs[sbase_next + 192 + lane];
+#pragma unroll
+            for (int i = 0; i < G; ++i) vn[i] = vp[(size_t)(g + G + i) * 64];
+            if (has2) load_cols(g + 2 * G, cnn, enn);
         }
 #pragma unroll
-        for (int i = 0; i < kStepGroup; ++i) {
-            acc.add(vb[i], xv[i]);
-            const bool end = COL16 ? ((em[i] >> lane) & 1ull) : (cb[i] < 0);
-            const unsigned long long m = COL16 ? em[i] : __ballot(end);
+        for (int i = 0; i < G; ++i) {
+            acc.add(vb[i], xb[i]);
+            const bool end = COL16 ? ((eb[i] >> lane) & 1ull) : (cb[i] < 0);
+            const unsigned long long m = COL16 ? eb[i] : __ballot(end);
             if (m) {
-                const int rel = segp - wbase + mbcnt64s(m);  // < 128
-                const int d0 = __shfl(w0, rel & 63);
-                const int d1 = __shfl(w1, rel & 63);
-                const int d = rel < 64 ? d0 : d1;
+                const int off = segp - sbase;  // segments of this group already ended
+                const int rel = off + mbcnt64s(m);
+                int d;
+                if (off + __popcll(m) <= 256) {
+                    const int r0 = off >> 6;
+                    const int wa = pick4(r0, s0, s1, s2, s3);
+                    const int wb = pick4(r0 + 1, s0, s1, s2, s3);
+                    const int da = __shfl(wa, rel & 63);
+                    const int db = __shfl(wb, rel & 63);
+                    d = ((rel >> 6) == r0) ? da : db;
+                } else {
+                    d = end ? segs[segp + mbcnt64s(m)] : 0;  // > 32 row ends per step: rare
+                }
                 if (end) {
                     const double v = acc.value();
                     if (d >= 0) {
@@ -173,19 +208,20 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
                     acc.reset();
                 }
                 segp += __popcll(m);
-                if (segp - wbase >= 64) {
-                    w0 = w1;
-                    w1 = segs[wbase + 128 + lane];
-                    wbase += 64;
-                }
             }
         }
-        if (more) {
+        if (has1) {
 #pragma unroll
-            for (int i = 0; i < kStepGroup; ++i) {
+            for (int i = 0; i < G; ++i) {
                 vb[i] = vn[i];
                 cb[i] = cn[i];
+                eb[i] = en[i];
+                xb[i] = xn[i];
+                cn[i] = cnn[i];
+                en[i] = enn[i];
             }
+            s0 = n0, s1 = n1, s2 = n2, s3 = n3;
+            sbase = sbase_next;
         }
     }
 
@@ -213,21 +249,19 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
         }
     }
 
-    // empty rows of [chunk_rows[c], chunk_rows[c+1])
+    // empty rows: any wave may zero any empty row, so the bitmap words are
+    // split evenly over the waves (not by the chunks' row ranges, which are
+    // very uneven on power-law matrices)
     if (a.has_empty) {
-        const int r0 = zr, r1 = a.chunk_rows[c + 1];
-        if (r1 > r0) {
-            const int wlo = r0 >> 5, whi = (r1 - 1) >> 5;
-            for (int w = wlo + lane; w <= whi; w += 64) {
-                unsigned bits = a.empty_bits[w];
-                const int rb = w << 5;
-                if (rb < r0) bits &= ~0u << (r0 - rb);
-                if (rb + 32 > r1) bits &= (r1 - rb >= 32) ? ~0u : ((1u << (r1 - rb)) - 1u);
-                while (bits) {
-                    const int j = __ffs(bits) - 1;
-                    store_y<YT>(a.y, rb + j, 0.0);
-                    bits &= bits - 1;
-                }
+        const int nw = (a.M + 31) >> 5;
+        const int w0 = (int)((long long)nw * c / a.nchunks), w1 = (int)((long long)nw * (c + 1) / a.nchunks);
+        for (int w = w0 + lane; w < w1; w += 64) {
+            unsigned bits = a.empty_bits[w];
+            const int rb = w << 5;
+            while (bits) {
+                const int j = __ffs(bits) - 1;
+                store_y<YT>(a.y, rb + j, 0.0);
+                bits &= bits - 1;
             }
         }
     }
@@ -291,7 +325,8 @@ __global__ __launch_bounds__(256) void finish_kernel(FinishArgs a) {
         if (lane == 0) a.y[frow] = (float)s;
     }
     if (a.x64) {
-        const int r0 = a.chunk_rows[c], r1 = a.chunk_rows[c + 1];
+        // flagged rows: any wave may recompute any row; rows are split evenly
+        const int r0 = (int)((long long)a.M * c / a.nchunks), r1 = (int)((long long)a.M * (c + 1) / a.nchunks);
         for (int rb = r0; rb < r1; rb += 64) {
             const int r = rb + lane;
             const bool f = (r < r1) && a.rflag[r];
