@@ -209,6 +209,8 @@ def main():
                     help="precision: FP64/FP32/BF16 storage on the workload's matrix; batch: 512-matrix SpMM")
     ap.add_argument("--k", type=int, default=8, help="batch variant: right-hand sides")
     ap.add_argument("--nmat", type=int, default=512, help="batch variant: matrices")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="also run 3 plain device copies of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures the single-GPU SpMV path only; this build has no multi-GPU path "
@@ -217,6 +219,15 @@ def main():
 
     import torch
     import pava_amd as P
+
+    if args.copy_calib_mb > 0:
+        n = args.copy_calib_mb * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)  # an elementwise kernel: reads and writes exactly n*8 bytes
+        torch.cuda.synchronize()
+        del src, dst
 
     if args.variant == "batch":
         import oracle as O

@@ -92,6 +92,14 @@ def lib() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m pava_amd.build`")
+    # PyTorch-ROCm ships its own libamdhip64 / libhsa-runtime64 under the same
+    # sonames as /opt/rocm's. Whichever is loaded first serves the whole
+    # process; load torch's first so that torch and this library share one HIP
+    # runtime (device pointers and streams pass between them).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     P = C.c_void_p
     i, d, ull = C.c_int, C.c_double, C.c_ulonglong
