@@ -162,6 +162,9 @@ typedef struct cvr_hip_opts {
                                 FP32 storage then stays within ~1e-5 relative of FP64) */
     int keep_fp64;           /* keep the FP64 CSR resident (required by recompute) */
     int device;              /* HIP device ordinal */
+    int x_hot_kb;            /* hot-x staging: the most used columns (each used >= 2x, at most
+                                x_hot_kb KiB of x) are renumbered into a dense copy of x refreshed
+                                before each SpMV; 0 = off (default 2048; 32-bit layouts only) */
 } cvr_hip_opts;
 
 typedef struct cvr_hip_info {
@@ -174,6 +177,7 @@ typedef struct cvr_hip_info {
     int n_fixup_rows;        /* rows spanning more than one chunk */
     long long device_bytes;  /* device memory held by the handle */
     double convert_ms;       /* GPU time of the CSR->CVR conversion (hipEvent) */
+    int n_hot_cols;          /* columns staged in the hot-x copy */
 } cvr_hip_info;
 
 typedef struct cvr_hip_handle cvr_hip_handle;
@@ -269,6 +273,9 @@ int cvr_hip_time_spmv_x64(cvr_hip_handle *h, const void *d_x, const double *d_x6
  * seg_count (nchunks), split_n (nchunks), split_tab (2*nchunks*CVR_HIP_SPLIT_CAP),
  * fix (2*nchunks), col_base (nchunks, 16-bit layouts only), end_mask
  * (layout_elems/64 uint64, 16-bit layouts only). Any pointer may be NULL. */
+/* hot_cols (n_hot_cols entries): the column of each hot index; a layout column
+ * with bit 30 set refers to hot_cols[col & 0x3fffffff]. */
+int cvr_hip_export_hot(const cvr_hip_handle *h, int *hot_cols);
 #define CVR_HIP_SPLIT_CAP 72
 #define CVR_HIP_SEG_PER_CHUNK_EXTRA 66
 int cvr_hip_export_layout(const cvr_hip_handle *h, void *vals, void *cols, int *chunk_rows,

@@ -49,7 +49,7 @@ class _CSR(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("steps_per_chunk", C.c_int), ("precision", C.c_int), ("col_index_bits", C.c_int),
                 ("recompute", C.c_int), ("kappa_threshold", C.c_double), ("keep_fp64", C.c_int),
-                ("device", C.c_int)]
+                ("device", C.c_int), ("x_hot_kb", C.c_int)]
 
 
 class _Info(C.Structure):
@@ -57,7 +57,7 @@ class _Info(C.Structure):
                 ("steps_per_chunk", C.c_int), ("precision", C.c_int), ("col_index_bits", C.c_int),
                 ("layout_elems", C.c_longlong), ("seg_capacity", C.c_longlong), ("max_slots", C.c_int),
                 ("has_empty_rows", C.c_int), ("n_fixup_rows", C.c_int), ("device_bytes", C.c_longlong),
-                ("convert_ms", C.c_double)]
+                ("convert_ms", C.c_double), ("n_hot_cols", C.c_int)]
 
 
 class _GenParams(C.Structure):
@@ -81,7 +81,7 @@ EXPORTED = [
     "cvr_hip_batch_spmm", "cvr_hip_batch_graph", "cvr_hip_batch_replay", "cvr_hip_batch_info",
     "cvr_hip_batch_destroy", "csr_hip_create", "csr_hip_spmv", "csr_hip_destroy", "pava_csr_features",
     "pava_mlp_create", "pava_mlp_num_params", "pava_mlp_get_params", "pava_mlp_set_params", "pava_mlp_predict",
-    "pava_mlp_loss_grad", "pava_mlp_train", "pava_mlp_destroy",
+    "pava_mlp_loss_grad", "pava_mlp_train", "pava_mlp_destroy", "cvr_hip_export_hot",
 ]
 
 
@@ -272,12 +272,14 @@ class CvrHip:
     """A CVR-64 matrix resident on one GPU (the cvr_hip format plugin)."""
 
     def __init__(self, A: CSR, steps_per_chunk: int = 256, precision: int = PREC_F64, col_index_bits: int = 32,
-                 recompute: bool = False, kappa_threshold: float = 64.0, keep_fp64: bool = False, device: int = 0):
+                 recompute: bool = False, kappa_threshold: float = 64.0, keep_fp64: bool = False, device: int = 0,
+                 x_hot_kb: int = 2048):
         L = lib()
         o = _Opts()
         L.cvr_hip_default_opts(C.byref(o))
         o.steps_per_chunk, o.precision, o.col_index_bits = steps_per_chunk, precision, col_index_bits
         o.recompute, o.kappa_threshold, o.keep_fp64, o.device = int(recompute), kappa_threshold, int(keep_fp64), device
+        o.x_hot_kb = x_hot_kb
         self._h = C.c_void_p()
         s = A.c_struct()
         _check(L.cvr_hip_create(C.byref(s), C.byref(o), C.byref(self._h)))
@@ -338,6 +340,14 @@ class CvrHip:
         n = C.c_int()
         _check(lib().cvr_hip_last_recompute_count(self._h, C.byref(n)))
         return n.value
+
+    def hot_cols(self) -> np.ndarray:
+        """Columns staged in the hot-x copy (index = hot number)."""
+        n = self.info()["n_hot_cols"]
+        out = np.zeros(max(n, 1), np.int32)
+        lib().cvr_hip_export_hot.argtypes = [C.c_void_p, C.c_void_p]
+        _check(lib().cvr_hip_export_hot(self._h, _ptr(out)))
+        return out[:n]
 
     def export_layout(self) -> dict:
         inf = self.info()

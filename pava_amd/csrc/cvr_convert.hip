@@ -31,6 +31,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
+#include <vector>
+
 #include "cvr_kernels.h"
 
 namespace cvr {
@@ -369,6 +371,104 @@ __global__ __launch_bounds__(256) void convert_kernel(ConvertArgs a) {
         tab[gbase] = make_int2(0, nsp);
         a.split_n[c] = gbase;
     }
+}
+
+// ---------------------------------------------------------------- hot columns
+// Column use counts, a log2 histogram of them, an ordered selection of the
+// columns used at least `t` times, and the layout rewrite that renumbers those
+// columns into the dense hot-x buffer (bit 30 of the column marks them).
+__global__ void col_hist_kernel(const int *__restrict__ ci, long long nnz, int *__restrict__ cnt) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (long long)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[ci[k]], 1);
+}
+
+__global__ void cnt_bucket_kernel(const int *__restrict__ cnt, int N, int *__restrict__ buckets) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const int v = cnt[c];
+    if (v > 0) atomicAdd(&buckets[31 - __clz(v)], 1);
+}
+
+// per 1024-column block: number of columns with cnt >= t
+__global__ __launch_bounds__(1024) void hot_count_kernel(const int *__restrict__ cnt, int N, int t, int *__restrict__ bcnt) {
+    __shared__ int s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    const int c = blockIdx.x * 1024 + threadIdx.x;
+    const bool f = c < N && cnt[c] >= t;
+    const unsigned long long m = __ballot(f);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&s, __popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = s;
+}
+
+// ordered ranks: colmap[c] = hot index (column order) or -1; hot_cols[idx] = c
+__global__ __launch_bounds__(1024) void hot_assign_kernel(const int *__restrict__ cnt, int N, int t,
+                                                          const int *__restrict__ boff, int *__restrict__ colmap,
+                                                          int *__restrict__ hot_cols) {
+    __shared__ int wsum[16];
+    const int c = blockIdx.x * 1024 + threadIdx.x;
+    const int w = threadIdx.x >> 6;
+    const bool f = c < N && cnt[c] >= t;
+    const unsigned long long m = __ballot(f);
+    if ((threadIdx.x & 63) == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int off = boff[blockIdx.x];
+    for (int i = 0; i < w; ++i) off += wsum[i];
+    if (c < N) {
+        if (f) {
+            const int idx = off + mbcnt64(m);
+            colmap[c] = idx;
+            hot_cols[idx] = c;
+        } else {
+            colmap[c] = -1;
+        }
+    }
+}
+
+__global__ void remap_hot_kernel(int *__restrict__ cols, long long L, const int *__restrict__ colmap) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < L; e += (long long)gridDim.x * blockDim.x) {
+        const int v = cols[e];
+        const int m = colmap[v & 0x3fffffff];
+        if (m >= 0) cols[e] = (v & kEndFlag) | kHotFlag | m;
+    }
+}
+
+hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, int *cnt, int *buckets, int *bcnt,
+                              int *boff, int *colmap, int *hot_cols, int *cols, long long L, int *n_hot,
+                              hipStream_t st) {
+    // counts (cnt, buckets zeroed by the caller)
+    hipLaunchKernelGGL(col_hist_kernel, dim3(2048), dim3(256), 0, st, ci, nnz, cnt);
+    hipLaunchKernelGGL(cnt_bucket_kernel, dim3((N + 255) / 256), dim3(256), 0, st, cnt, N, buckets);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int hb[32];
+    if ((e = hipMemcpyAsync(hb, buckets, sizeof hb, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    // threshold t = 2^b: the largest hot set that fits, columns used at least twice
+    int b = 31, total = 0;
+    while (b >= 1 && total + hb[b] <= max_hot) total += hb[b--];
+    const int bstar = b + 1;
+    *n_hot = total;
+    if (total == 0 || bstar > 31) {
+        *n_hot = 0;
+        return hipSuccess;
+    }
+    const int t = 1 << bstar;
+    const int nb = (N + 1023) / 1024;
+    hipLaunchKernelGGL(hot_count_kernel, dim3(nb), dim3(1024), 0, st, cnt, N, t, bcnt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    std::vector<int> bc((size_t)nb), bo((size_t)nb);
+    if ((e = hipMemcpyAsync(bc.data(), bcnt, (size_t)nb * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    int run = 0;
+    for (int i = 0; i < nb; ++i) bo[(size_t)i] = run, run += bc[(size_t)i];
+    *n_hot = run;
+    if ((e = hipMemcpyAsync(boff, bo.data(), (size_t)nb * 4, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(hot_assign_kernel, dim3(nb), dim3(1024), 0, st, cnt, N, t, boff, colmap, hot_cols);
+    hipLaunchKernelGGL(remap_hot_kernel, dim3(2048), dim3(256), 0, st, cols, L, colmap);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipStreamSynchronize(st);
 }
 
 // Host-side launchers --------------------------------------------------------

@@ -42,6 +42,10 @@ struct cvr_hip_handle {
     void *d_hx = nullptr, *d_hy = nullptr, *d_hx64 = nullptr;
     long long device_bytes = 0;
     double convert_ms = 0.0;
+    // hot-x staging
+    int *d_hot_cols = nullptr;
+    void *d_xhot = nullptr;
+    int n_hot = 0;
     std::vector<void *> allocs;
 };
 
@@ -74,6 +78,7 @@ extern "C" void cvr_hip_default_opts(cvr_hip_opts *o) {
     o->kappa_threshold = 64.0;
     o->keep_fp64 = 0;
     o->device = 0;
+    o->x_hot_kb = 2048;
 }
 
 static int value_size(int prec) { return prec == CVR_HIP_PREC_F64 ? 8 : prec == CVR_HIP_PREC_F32 ? 4 : 2; }
@@ -199,6 +204,39 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
         return set_error(PAVA_HIP_ERANGE, "a chunk spans more than 65536 columns: 16-bit column layout not possible");
     h->n_fix = status[3];
     h->has_empty = status[4] ? 1 : 0;
+
+    // hot-x staging: renumber the most used columns into a dense copy of x
+    const int xsize = h->prec == CVR_HIP_PREC_F64 ? 8 : 4;
+    const long long max_hot = (long long)o->x_hot_kb * 1024 / xsize;
+    if (max_hot > 0 && !h->col16 && nnz > 0 && N < (1 << 30)) {
+        int *cnt = nullptr, *buckets = nullptr, *bcnt = nullptr, *boff = nullptr, *colmap = nullptr;
+        const int nb = (N + 1023) / 1024;
+        hipError_t e = hipSuccess;
+        int nh = 0;
+        if (hipMalloc(&cnt, (size_t)N * 4 + 4) != hipSuccess || hipMalloc(&buckets, 256) != hipSuccess ||
+            hipMalloc(&bcnt, (size_t)nb * 4 + 4) != hipSuccess || hipMalloc(&boff, (size_t)nb * 4 + 4) != hipSuccess ||
+            hipMalloc(&colmap, (size_t)N * 4 + 4) != hipSuccess ||
+            dev_alloc(h, (void **)&h->d_hot_cols, (size_t)std::min<long long>(max_hot, N) * 4 + 4) != 0) {
+            e = hipErrorOutOfMemory;
+        } else {
+            e = hipMemsetAsync(cnt, 0, (size_t)N * 4, h->stream);
+            if (e == hipSuccess) e = hipMemsetAsync(buckets, 0, 256, h->stream);
+            if (e == hipSuccess)
+                e = cvr::launch_hot_columns(d_ci, nnz, N, (int)std::min<long long>(max_hot, N), cnt, buckets, bcnt, boff,
+                                            colmap, h->d_hot_cols, (int *)h->d_cols, h->layout_elems, &nh, h->stream);
+        }
+        (void)hipFree(cnt);
+        (void)hipFree(buckets);
+        (void)hipFree(bcnt);
+        (void)hipFree(boff);
+        (void)hipFree(colmap);
+        if (e != hipSuccess) return set_error(PAVA_HIP_EDEVICE, "hot-column staging failed: %s", hipGetErrorString(e));
+        h->n_hot = nh;
+        if (nh > 0) {
+            int rc2 = dev_alloc(h, &h->d_xhot, (size_t)nh * xsize);
+            if (rc2) return rc2;
+        }
+    }
     return 0;
 }
 
@@ -320,6 +358,7 @@ extern "C" int cvr_hip_get_info(const cvr_hip_handle *h, cvr_hip_info *info) {
     info->n_fixup_rows = h->n_fix;
     info->device_bytes = h->device_bytes;
     info->convert_ms = h->convert_ms;
+    info->n_hot_cols = h->n_hot;
     return 0;
 }
 
@@ -335,6 +374,7 @@ static cvr::SpmvArgs spmv_args(const cvr_hip_handle *h, const void *d_x, void *d
     a.split_tab = h->d_split_tab;
     a.empty_bits = h->d_empty;
     a.x = d_x;
+    a.xhot = h->d_xhot;
     a.y = d_y;
     a.carry = h->d_carry;
     a.carry_abs = h->d_carry_abs;
@@ -377,6 +417,9 @@ static int enqueue_tail(cvr_hip_handle *h, void *d_y, const double *d_x64, hipSt
 
 static int enqueue_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, hipStream_t st, const double *d_x64 = nullptr) {
     cvr::SpmvArgs a = spmv_args(h, d_x, d_y);
+    if (h->n_hot > 0)
+        HIP_TRY(cvr::launch_hot_gather(d_x, h->d_hot_cols, h->d_xhot, h->n_hot,
+                                       h->prec == CVR_HIP_PREC_F64 ? 8 : 4, st));
     HIP_TRY(cvr::launch_spmv(a, h->prec, h->col16, st));
     return enqueue_tail(h, d_y, d_x64, st);
 }
@@ -448,6 +491,7 @@ static int spmm_desc(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, cvr
     a->split_tab = h->d_split_tab;
     a->empty_bits = h->d_empty;
     a->fix = h->d_fix;
+    a->hot_cols = h->d_hot_cols;
     a->X = d_X;
     a->Y = d_Y;
     a->carry = h->d_carry_mm;
@@ -655,6 +699,9 @@ extern "C" int cvr_hip_time_spmv_x64(cvr_hip_handle *h, const void *d_x, const d
         for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
         cvr::SpmvArgs sa = spmv_args(h, d_x, d_y);
         for (int i = 0; i < iters; ++i) {
+            if (h->n_hot > 0)
+                HIP_TRY(cvr::launch_hot_gather(d_x, h->d_hot_cols, h->d_xhot, h->n_hot,
+                                               h->prec == CVR_HIP_PREC_F64 ? 8 : 4, h->stream));
             HIP_TRY(hipEventRecord(ev[2 * (size_t)i], h->stream));
             HIP_TRY(cvr::launch_spmv(sa, h->prec, h->col16, h->stream));
             HIP_TRY(hipEventRecord(ev[2 * (size_t)i + 1], h->stream));
@@ -672,6 +719,14 @@ extern "C" int cvr_hip_time_spmv_x64(cvr_hip_handle *h, const void *d_x, const d
     }
     hipEventDestroy(a);
     hipEventDestroy(b);
+    return 0;
+}
+
+extern "C" int cvr_hip_export_hot(const cvr_hip_handle *h, int *hot_cols) {
+    if (!h || !hot_cols) return set_error(PAVA_HIP_EINVAL, "null argument");
+    if (h->n_hot == 0) return 0;
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipMemcpy(hot_cols, h->d_hot_cols, (size_t)h->n_hot * 4, hipMemcpyDeviceToHost));
     return 0;
 }
 

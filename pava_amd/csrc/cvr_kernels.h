@@ -14,6 +14,8 @@ constexpr int kSplitCap = 72;       // split-table entries per chunk (groups + s
 constexpr int kSegExtra = 66;       // segment-list slack per chunk (steals + padding piece)
 constexpr int kSlotCap = 160;       // split-segment slots per chunk (bound: 64 thieves + 67 owners)
 constexpr int kEndFlag = (int)0x80000000u;  // sign bit of a column: last element of a segment
+constexpr int kHotFlag = 0x40000000;        // bit 30: column renumbered into the hot-x buffer
+constexpr int kColMask = 0x3fffffff;
 
 // Piece kinds (bit set)
 constexpr int kHead = 1;  // row started in an earlier chunk
@@ -58,6 +60,7 @@ struct SpmvArgs {
     const int2 *split_tab;
     const unsigned *empty_bits;
     const void *x;
+    const void *xhot;    // dense copy of the hot columns of x (or null)
     void *y;
     double *carry;       // 2 per chunk: head / tail partial sums
     double *carry_abs;   // 2 per chunk: head / tail partial sum|a x| (recompute only)
@@ -94,6 +97,7 @@ struct SpmmDesc {
     const int2 *split_tab;
     const unsigned *empty_bits;
     const int *fix;
+    const int *hot_cols;  // column of each hot index (layouts with hot-x renumbering)
     const float *X;   // numCols x k, row-major
     float *Y;         // numRows x k, row-major
     double *carry;    // 2 * nchunks * k
@@ -113,5 +117,9 @@ hipError_t launch_convert(const ConvertArgs &a, hipStream_t st);
 hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st);
 hipError_t launch_fixup(const int *fix, const double *carry, void *y, int nchunks, int prec, hipStream_t st);
 hipError_t launch_finish(const FinishArgs &a, hipStream_t st);
+hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, int *cnt, int *buckets, int *bcnt,
+                              int *boff, int *colmap, int *hot_cols, int *cols, long long L, int *n_hot,
+                              hipStream_t st);
+hipError_t launch_hot_gather(const void *x, const int *hot_cols, void *xhot, int n_hot, int xsize, hipStream_t st);
 
 }  // namespace cvr

@@ -89,7 +89,11 @@ __device__ void spmm_chunk(const SpmmDesc &a, int c, int lane, double *slots) {
     for (int g = 0; g < steps; g += U) {
         float xv[U][KP];
 #pragma unroll
-        for (int i = 0; i < U; ++i) load_xrow<KP>(a.X, k, cb[i] & 0x7fffffff, xv[i]);
+        for (int i = 0; i < U; ++i) {
+            const int cv = cb[i];
+            const int col = (cv & kHotFlag) ? a.hot_cols[cv & kColMask] : (cv & kColMask);
+            load_xrow<KP>(a.X, k, col, xv[i]);
+        }
         VT vn[U];
         int cn[U];
         const bool more = (g + U) < steps;

@@ -130,7 +130,12 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
     // g+1 and the columns of group g+2 are in flight. The segment entries of a
     // group (<= 256 in four registers per lane, read with ds_bpermute) are
     // issued together with its x gathers, so destination lookups never wait.
-    auto col_of = [&](int cv) -> int { return COL16 ? cbase + cv : (cv & 0x7fffffff); };
+    const XT *__restrict__ xh = reinterpret_cast<const XT *>(a.xhot);
+    // x element of a stored column: hot columns read the dense hot-x copy
+    auto xval = [&](int cv) -> XT {
+        if (COL16) return x[cbase + cv];
+        return ((cv & kHotFlag) ? xh : x)[cv & kColMask];
+    };
     auto load_cols = [&](int g0, int (&cc)[G], uint64_t (&em)[G]) {
 #pragma unroll
         for (int i = 0; i < G; ++i) cc[i] = COL16 ? (int)cp16[(size_t)(g0 + i) * 64] : cp32[(size_t)(g0 + i) * 64];
@@ -158,7 +163,7 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
 #pragma unroll
     for (int i = 0; i < G; ++i) vb[i] = vp[(size_t)i * 64];
 #pragma unroll
-    for (int i = 0; i < G; ++i) xb[i] = x[col_of(cb[i])];
+    for (int i = 0; i < G; ++i) xb[i] = xval(cb[i]);
     s0 = segs[lane], s1 = segs[64 + lane], s2 = segs[128 + lane], s3 = segs[192 + lane];
     if (G < steps) load_cols(G, cn, en);
     int sbase = 0, segp = 0;
@@ -168,7 +173,7 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
         const int sbase_next = sbase + group_ends(cb, eb);
         if (has1) {
 #pragma unroll
-            for (int i = 0; i < G; ++i) xn[i] = x[col_of(cn[i])];
+            for (int i = 0; i < G; ++i) xn[i] = xval(cn[i]);
             n0 = segs[sbase_next + lane];
             n1 = segs[sbase_next + 64 + lane];
             n2 = segs[sbase_next + 128 + lane];
@@ -343,6 +348,24 @@ __global__ __launch_bounds__(256) void finish_kernel(FinishArgs a) {
         }
     }
     if (lane == 0) a.rc_count[c] = count;
+}
+
+// xhot[k] = x[hot_cols[k]] (before each SpMV on layouts with hot columns)
+template <typename XT>
+__global__ void hot_gather_kernel(const XT *__restrict__ x, const int *__restrict__ hot_cols, XT *__restrict__ xhot,
+                                  int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) xhot[k] = x[hot_cols[k]];
+}
+
+hipError_t launch_hot_gather(const void *x, const int *hot_cols, void *xhot, int n_hot, int xsize, hipStream_t st) {
+    if (n_hot <= 0) return hipSuccess;
+    dim3 grid((n_hot + 255) / 256), block(256);
+    if (xsize == 8)
+        hipLaunchKernelGGL(hot_gather_kernel<double>, grid, block, 0, st, (const double *)x, hot_cols, (double *)xhot, n_hot);
+    else
+        hipLaunchKernelGGL(hot_gather_kernel<float>, grid, block, 0, st, (const float *)x, hot_cols, (float *)xhot, n_hot);
+    return hipGetLastError();
 }
 
 // --------------------------------------------------------------- launchers

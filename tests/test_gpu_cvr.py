@@ -56,11 +56,35 @@ def _compare_layout(A, gpu, emu, S):
 def test_gpu_layout_equals_emulation(P, O, mats, S):
     for name, A in mats.items():
         _say(f"layout {name} S={S} m={A.numRows} nnz={A.nnz}")
-        with P.CvrHip(A, steps_per_chunk=S) as h:
+        with P.CvrHip(A, steps_per_chunk=S, x_hot_kb=0) as h:
             gpu = h.export_layout()
         emu = O.cvr64_build(A, S, 0)
         _compare_layout(A, gpu, emu, S)
         assert not reconstruct(A, gpu, S)
+
+
+@pytest.mark.parametrize("hot_kb", [1, 64, 2048])
+def test_hot_x_staging_decodes_and_is_bit_identical(P, O, mats, hot_kb):
+    for name, A in mats.items():
+        S = 64
+        x = P.fill_uniform(A.numCols, 11)
+        with P.CvrHip(A, steps_per_chunk=S, x_hot_kb=hot_kb) as h:
+            lay = h.export_layout()
+            hot = h.hot_cols()
+            y = h.spmv_host(x)
+        emu = O.cvr64_build(A, S, 0)
+        cols = lay["cols"].astype(np.int64)
+        is_hot = (cols & 0x40000000) != 0
+        idx = cols & 0x3FFFFFFF
+        dec = np.where(is_hot, hot[np.minimum(idx, max(len(hot) - 1, 0))] if len(hot) else idx, idx)
+        assert np.array_equal(dec, emu["cols"] & 0x7FFFFFFF), name
+        assert np.array_equal(cols < 0, emu["cols"] < 0), name
+        assert len(hot) <= hot_kb * 1024 // 8
+        if len(hot):
+            assert np.all(np.diff(hot) > 0)  # hot numbers follow column order
+            counts = np.bincount(A.cols, minlength=A.numCols)
+            assert counts[hot].min() >= 2
+        assert np.array_equal(y, O.cvr64_spmv(A, emu, x)), name
 
 
 @pytest.mark.parametrize("S", [8, 64, 256])
