@@ -209,6 +209,8 @@ def main():
                     help="precision: FP64/FP32/BF16 storage on the workload's matrix; batch: 512-matrix SpMM")
     ap.add_argument("--k", type=int, default=8, help="batch variant: right-hand sides")
     ap.add_argument("--nmat", type=int, default=512, help="batch variant: matrices")
+    ap.add_argument("--x-hot-kb", type=int, default=2048,
+                    help="hot-x staging buffer (KiB; 0 = off): the most used columns read from a dense copy")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run 3 plain device copies of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -252,7 +254,7 @@ def main():
           f"gen {t_gen:.1f}s", file=sys.stderr, flush=True)
 
     S = args.steps_per_chunk or 256
-    h = P.CvrHip(A, steps_per_chunk=S, precision=w["prec"])
+    h = P.CvrHip(A, steps_per_chunk=S, precision=w["prec"], x_hot_kb=args.x_hot_kb)
     info = h.info()
     dev = torch.device("cuda:0")
     dx = torch.from_numpy(x).to(dev)
@@ -295,7 +297,8 @@ def main():
         "data": "synthetic (seeded generator; x ~ U[-1,1] seed 0)",
         "config": {"workload": args.workload, "rows": A.numRows, "cols": A.numCols, "nnz": A.nnz,
                    "max_row_len": int(np.diff(A.rowOffsets).max()), "steps_per_chunk": S,
-                   "chunks": info["nchunks"], "algorithmic_bytes": B},
+                   "chunks": info["nchunks"], "algorithmic_bytes": B, "x_hot_kb": args.x_hot_kb,
+                   "hot_columns": info["n_hot_cols"]},
         "nnz_per_s": A.nnz / (ms_step * 1e-3),
         "gflops": 2.0 * A.nnz / (ms_step * 1e-3) / 1e9,
         "hbm_frac": value / HBM_PEAK_GBS,

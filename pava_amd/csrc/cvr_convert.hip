@@ -149,7 +149,6 @@ __global__ __launch_bounds__(256) void convert_kernel(ConvertArgs a) {
             mn = min(mn, cc);
             mx = max(mx, cc);
         }
-        if (pad > 0) mn = min(mn, 0), mx = max(mx, 0);
         for (int off = 32; off > 0; off >>= 1) {
             mn = min(mn, __shfl_xor(mn, off));
             mx = max(mx, __shfl_xor(mx, off));
@@ -307,7 +306,9 @@ __global__ __launch_bounds__(256) void convert_kernel(ConvertArgs a) {
 #pragma unroll
         for (int i = 0; i < kStepGroup; ++i) {
             const bool real = P[i] < a.nnz;
-            cv[i] = real ? a.ci[P[i]] : 0;
+            // padding elements read a column inside the chunk's range (value 0,
+            // sum discarded): column 0 for 32-bit layouts, offset 0 for 16-bit
+            cv[i] = real ? a.ci[P[i]] : (COL16 ? cbase : 0);
             vv[i] = real ? reinterpret_cast<const double *>(a.va)[P[i]] : 0.0;
         }
 #pragma unroll
