@@ -97,13 +97,10 @@ __device__ __forceinline__ void store_y(void *y, int r, double v) {
     reinterpret_cast<YT *>(y)[r] = (YT)v;
 }
 
-// Picks one of four wave-uniform register copies by a uniform index.
-__device__ __forceinline__ int pick4(int r, int s0, int s1, int s2, int s3) {
-    return r == 0 ? s0 : r == 1 ? s1 : r == 2 ? s2 : s3;
-}
-
 template <typename VT, typename XT, typename YT, bool COMP, bool COL16, bool KAPPA>
-__global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
+// 32-bit layouts: >= 4 waves per SIMD (128 VGPRs, no spills); the 16-bit
+// layouts keep their end masks in registers and are left unconstrained
+__global__ __launch_bounds__(256, COL16 ? 1 : 4) void spmv_kernel(SpmvArgs a) {
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int c = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wv;
@@ -126,37 +123,60 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
     const int cbase = COL16 ? a.col_base[c] : 0;
 
     // Pipeline, one 8-step group deep: while group g is processed, the x
-    // gathers and the segment-list entries of group g+1, the values of group
-    // g+1 and the columns of group g+2 are in flight. The segment entries of a
-    // group (<= 256 in four registers per lane, read with ds_bpermute) are
-    // issued together with its x gathers, so destination lookups never wait.
+    // gathers and the segment-list entries of group g+1 (exactly as many as
+    // its row ends, <= 512, issued together with the gathers), the values of
+    // group g+1 and the columns of group g+2 are in flight. A group's segment
+    // entries are copied to LDS when its processing starts, so each ending
+    // lane finds its destination with one LDS read and never waits on memory.
+    __shared__ int segbuf[kWavesPerBlock][64 * kStepGroup];
     const XT *__restrict__ xh = reinterpret_cast<const XT *>(a.xhot);
     // x element of a stored column: hot columns read the dense hot-x copy
+    // (an explicit 64-bit address select: a select of two pointers would be
+    // lowered to an indexed private array)
+    const uint64_t ax = reinterpret_cast<uint64_t>(x), ah = reinterpret_cast<uint64_t>(xh);
     auto xval = [&](int cv) -> XT {
-        if (COL16) return x[cbase + cv];
-        return ((cv & kHotFlag) ? xh : x)[cv & kColMask];
-    };
-    auto load_cols = [&](int g0, int (&cc)[G], uint64_t (&em)[G]) {
-#pragma unroll
-        for (int i = 0; i < G; ++i) cc[i] = COL16 ? (int)cp16[(size_t)(g0 + i) * 64] : cp32[(size_t)(g0 + i) * 64];
-        if (COL16) {
-#pragma unroll
-            for (int i = 0; i < G; ++i) em[i] = emp[g0 + i];
+        if constexpr (COL16) {
+            return x[cbase + cv];
+        } else {
+            const uint64_t b = (cv & kHotFlag) ? ah : ax;
+            return *reinterpret_cast<const XT *>(b + (uint64_t)(uint32_t)(cv & kColMask) * sizeof(XT));
         }
     };
-    auto group_ends = [&](const int (&cc)[G], const uint64_t (&em)[G]) -> int {
-        int e = 0;
+    constexpr int GE = COL16 ? G : 1;  // end masks are stored only for 16-bit layouts
+    auto load_cols = [&](int g0, int (&cc)[G], uint64_t (&em)[GE]) {
+        if constexpr (COL16) {
 #pragma unroll
-        for (int i = 0; i < G; ++i) e += __popcll(COL16 ? em[i] : __ballot(cc[i] < 0));
+            for (int i = 0; i < G; ++i) cc[i] = (int)cp16[(size_t)(g0 + i) * 64];
+#pragma unroll
+            for (int i = 0; i < G; ++i) em[i] = emp[g0 + i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < G; ++i) cc[i] = cp32[(size_t)(g0 + i) * 64];
+        }
+    };
+    auto group_ends = [&](const int (&cc)[G], const uint64_t (&em)[GE]) -> int {
+        int e = 0;
+        if constexpr (COL16) {
+#pragma unroll
+            for (int i = 0; i < G; ++i) e += __popcll(em[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < G; ++i) e += __popcll(__ballot(cc[i] < 0));
+        }
         return e;
+    };
+    auto load_segs = [&](int sb, int ne, int (&r)[G]) {
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (j * 64 < ne) r[j] = segs[sb + j * 64 + lane];
     };
 
     Acc<VT, XT, COMP, KAPPA> acc;
     VT vb[G], vn[G];
     int cb[G], cn[G], cnn[G];
-    uint64_t eb[G], en[G], enn[G];
+    uint64_t eb[GE], en[GE], enn[GE];
     XT xb[G], xn[G];
-    int s0, s1, s2, s3, n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+    int sc[G], sn[G];
 
     // prologue: group 0 fully issued, columns of group 1 in flight
     load_cols(0, cb, eb);
@@ -164,44 +184,44 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
     for (int i = 0; i < G; ++i) vb[i] = vp[(size_t)i * 64];
 #pragma unroll
     for (int i = 0; i < G; ++i) xb[i] = xval(cb[i]);
-    s0 = segs[lane], s1 = segs[64 + lane], s2 = segs[128 + lane], s3 = segs[192 + lane];
+    int ecur = group_ends(cb, eb);
+    load_segs(0, ecur, sc);
     if (G < steps) load_cols(G, cn, en);
     int sbase = 0, segp = 0;
 
     for (int g = 0; g < steps; g += G) {
         const bool has1 = (g + G) < steps, has2 = (g + 2 * G) < steps;
-        const int sbase_next = sbase + group_ends(cb, eb);
+        const int sbase_next = sbase + ecur;
+        int enext = 0;
         if (has1) {
 #pragma unroll
             for (int i = 0; i < G; ++i) xn[i] = xval(cn[i]);
-            n0 = segs[sbase_next + lane];
-            n1 = segs[sbase_next + 64 + lane];
-            n2 = segs[sbase_next + 128 + lane];
-            n3 = segs[sbase_next + 192 + lane];
+            enext = group_ends(cn, en);
+            load_segs(sbase_next, enext, sn);
 #pragma unroll
             for (int i = 0; i < G; ++i) vn[i] = vp[(size_t)(g + G + i) * 64];
             if (has2) load_cols(g + 2 * G, cnn, enn);
         }
+        // this group's destinations -> LDS
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (j * 64 < ecur) segbuf[wv][j * 64 + lane] = sc[j];
+        wave_sync_lds();
 #pragma unroll
         for (int i = 0; i < G; ++i) {
             acc.add(vb[i], xb[i]);
-            const bool end = COL16 ? ((eb[i] >> lane) & 1ull) : (cb[i] < 0);
-            const unsigned long long m = COL16 ? eb[i] : __ballot(end);
+            bool end;
+            unsigned long long m;
+            if constexpr (COL16) {
+                m = eb[i];
+                end = (m >> lane) & 1ull;
+            } else {
+                end = cb[i] < 0;
+                m = __ballot(end);
+            }
             if (m) {
-                const int off = segp - sbase;  // segments of this group already ended
-                const int rel = off + mbcnt64s(m);
-                int d;
-                if (off + __popcll(m) <= 256) {
-                    const int r0 = off >> 6;
-                    const int wa = pick4(r0, s0, s1, s2, s3);
-                    const int wb = pick4(r0 + 1, s0, s1, s2, s3);
-                    const int da = __shfl(wa, rel & 63);
-                    const int db = __shfl(wb, rel & 63);
-                    d = ((rel >> 6) == r0) ? da : db;
-                } else {
-                    d = end ? segs[segp + mbcnt64s(m)] : 0;  // > 32 row ends per step: rare
-                }
                 if (end) {
+                    const int d = segbuf[wv][segp - sbase + mbcnt64s(m)];
                     const double v = acc.value();
                     if (d >= 0) {
                         store_y<YT>(a.y, d, v);
@@ -220,14 +240,19 @@ __global__ __launch_bounds__(256) void spmv_kernel(SpmvArgs a) {
             for (int i = 0; i < G; ++i) {
                 vb[i] = vn[i];
                 cb[i] = cn[i];
-                eb[i] = en[i];
                 xb[i] = xn[i];
                 cn[i] = cnn[i];
+                sc[i] = sn[i];
+            }
+#pragma unroll
+            for (int i = 0; i < GE; ++i) {
+                eb[i] = en[i];
                 en[i] = enn[i];
             }
-            s0 = n0, s1 = n1, s2 = n2, s3 = n3;
             sbase = sbase_next;
+            ecur = enext;
         }
+        wave_sync_lds();  // the next group overwrites segbuf
     }
 
     // split rows: sum slots in element order
@@ -335,14 +360,23 @@ __global__ __launch_bounds__(256) void finish_kernel(FinishArgs a) {
         for (int rb = r0; rb < r1; rb += 64) {
             const int r = rb + lane;
             const bool f = (r < r1) && a.rflag[r];
-            unsigned long long m = __ballot(f);
+            count += __popcll(__ballot(f));
+            // short rows: one lane each, sequential FMA in element order
+            const int len = f ? a.rp[r + 1] - a.rp[r] : 0;
+            const bool mine = f && len <= 128;
+            if (mine) {
+                double s = 0.0;
+                for (int j = a.rp[r]; j < a.rp[r + 1]; ++j) s = __builtin_fma(a.va[j], a.x64[a.ci[j]], s);
+                a.y[r] = (float)s;
+            }
             if (f) a.rflag[r] = 0;
+            // long rows: the whole wave, fixed xor-butterfly reduction
+            unsigned long long m = __ballot(f && !mine);
             while (m) {
                 const int j = __ffsll((long long)m) - 1;
                 const int row = rb + j;
                 const double s = row_fp64(a, row, lane);
                 if (lane == 0) a.y[row] = (float)s;
-                ++count;
                 m &= m - 1;
             }
         }
