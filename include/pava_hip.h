@@ -88,8 +88,14 @@ typedef struct pava_csr_matrix {
 int pava_read_coo_matrix(const char *path, pava_coo_matrix *out);
 /* COO -> CSR sorted by (row, col); duplicate entries are summed in file order. */
 int pava_convert_coo_to_csr(const pava_coo_matrix *coo, pava_csr_matrix *out);
-/* Convenience: read + convert. */
+/* Convenience: read + convert. A "<path>.csrbin" cache newer than the file is
+ * read instead of parsing (the fast input path); the body of a large file is
+ * parsed by all OpenMP threads. */
 int pava_read_csr_matrix(const char *path, pava_csr_matrix *out);
+/* Binary CSR cache: "PAVACSR1", int32 numRows, numCols, int64 nnz, then
+ * rowOffsets, cols (int32) and vals (f64), little-endian. */
+int pava_write_csr_binary(const char *path, const pava_csr_matrix *A);
+int pava_read_csr_binary(const char *path, pava_csr_matrix *out);
 /* Write a CSR as "%%MatrixMarket matrix coordinate real general", 1-based,
  * values printed with 17 significant digits (round-trips FP64 exactly). */
 int pava_write_matrix_market(const char *path, const pava_csr_matrix *A);
@@ -291,9 +297,16 @@ int conduct_cvr_hip(int numRows, int numCols, int nItems_raw, int *h_rowDelimite
                     double *h_val, double *h_vec, double *refOut, int alpha, char *matrixName,
                     int numIterations);
 
-/* benchFile for the CVR block: read, size filters (main.cpp:82-99), x = i%3,
- * referenceSpMV, conduct_cvr_hip, checkResults. Returns the error count
- * (>= 0), -1 if the matrix was skipped, or a PAVA_HIP_E* code. */
+/* The csr_hip block (same protocol as conduct_cvr_hip, report format
+ * "CSR_HIP", thread column = lanes per row); stands in for the MKL CSR block
+ * (main.cpp CSR block, utils/pava_formats.cpp:39-47). */
+int conduct_csr_hip(int numRows, int numCols, int nItems, int *h_rowDelimiters, int *h_cols, double *h_val,
+                    double *h_vec, double *refOut, int lanes_per_row, char *matrixName, int numIterations);
+
+/* benchFile with the GPU format registry: read, size filters (main.cpp:82-99),
+ * x = i%3, referenceSpMV, then each format block in turn (cvr_hip, csr_hip)
+ * with checkResults. Returns the total error count (>= 0), -1 if the matrix
+ * was skipped, or a PAVA_HIP_E* code. */
 int pava_bench_file(const char *path, int numIterations);
 
 const char *pava_hip_last_error(void);

@@ -81,7 +81,8 @@ EXPORTED = [
     "cvr_hip_batch_spmm", "cvr_hip_batch_graph", "cvr_hip_batch_replay", "cvr_hip_batch_info",
     "cvr_hip_batch_destroy", "csr_hip_create", "csr_hip_spmv", "csr_hip_destroy", "pava_csr_features",
     "pava_mlp_create", "pava_mlp_num_params", "pava_mlp_get_params", "pava_mlp_set_params", "pava_mlp_predict",
-    "pava_mlp_loss_grad", "pava_mlp_train", "pava_mlp_destroy", "cvr_hip_export_hot",
+    "pava_mlp_loss_grad", "pava_mlp_train", "pava_mlp_destroy", "cvr_hip_export_hot", "pava_write_csr_binary",
+    "pava_read_csr_binary", "conduct_csr_hip",
 ]
 
 
@@ -209,6 +210,17 @@ def read_mtx(path: str | os.PathLike) -> CSR:
     return _gen(lib().pava_read_csr_matrix, str(path).encode())
 
 
+def write_csr_binary(path: str | os.PathLike, A: CSR) -> None:
+    lib().pava_write_csr_binary.argtypes = [C.c_char_p, C.POINTER(_CSR)]
+    s = A.c_struct()
+    _check(lib().pava_write_csr_binary(str(path).encode(), C.byref(s)))
+
+
+def read_csr_binary(path: str | os.PathLike) -> CSR:
+    lib().pava_read_csr_binary.argtypes = [C.c_char_p, C.POINTER(_CSR)]
+    return _gen(lib().pava_read_csr_binary, str(path).encode())
+
+
 def write_mtx(path: str | os.PathLike, A: CSR) -> None:
     s = A.c_struct()
     _check(lib().pava_write_matrix_market(str(path).encode(), C.byref(s)))
@@ -273,7 +285,7 @@ class CvrHip:
 
     def __init__(self, A: CSR, steps_per_chunk: int = 256, precision: int = PREC_F64, col_index_bits: int = 32,
                  recompute: bool = False, kappa_threshold: float = 64.0, keep_fp64: bool = False, device: int = 0,
-                 x_hot_kb: int = 2048):
+                 x_hot_kb: int = 0):
         L = lib()
         o = _Opts()
         L.cvr_hip_default_opts(C.byref(o))

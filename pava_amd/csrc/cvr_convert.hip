@@ -383,11 +383,17 @@ __global__ void col_hist_kernel(const int *__restrict__ ci, long long nnz, int *
         atomicAdd(&cnt[ci[k]], 1);
 }
 
-__global__ void cnt_bucket_kernel(const int *__restrict__ cnt, int N, int *__restrict__ buckets) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
-    const int v = cnt[c];
-    if (v > 0) atomicAdd(&buckets[31 - __clz(v)], 1);
+// per-block LDS histogram, one global atomic per bucket per block
+__global__ __launch_bounds__(256) void cnt_bucket_kernel(const int *__restrict__ cnt, int N, int *__restrict__ buckets) {
+    __shared__ int h[32];
+    if (threadIdx.x < 32) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+        const int v = cnt[c];
+        if (v > 0) atomicAdd(&h[31 - __clz(v)], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 32 && h[threadIdx.x]) atomicAdd(&buckets[threadIdx.x], h[threadIdx.x]);
 }
 
 // per 1024-column block: number of columns with cnt >= t
@@ -440,7 +446,7 @@ hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, 
                               hipStream_t st) {
     // counts (cnt, buckets zeroed by the caller)
     hipLaunchKernelGGL(col_hist_kernel, dim3(2048), dim3(256), 0, st, ci, nnz, cnt);
-    hipLaunchKernelGGL(cnt_bucket_kernel, dim3((N + 255) / 256), dim3(256), 0, st, cnt, N, buckets);
+    hipLaunchKernelGGL(cnt_bucket_kernel, dim3(std::min((N + 255) / 256, 1024)), dim3(256), 0, st, cnt, N, buckets);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     int hb[32];

@@ -78,7 +78,7 @@ extern "C" void cvr_hip_default_opts(cvr_hip_opts *o) {
     o->kappa_threshold = 64.0;
     o->keep_fp64 = 0;
     o->device = 0;
-    o->x_hot_kb = 2048;
+    o->x_hot_kb = 0;
 }
 
 static int value_size(int prec) { return prec == CVR_HIP_PREC_F64 ? 8 : prec == CVR_HIP_PREC_F32 ? 4 : 2; }

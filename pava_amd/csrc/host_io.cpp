@@ -26,6 +26,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <sys/stat.h>
 #include <sys/time.h>
 #include <utility>
 #include <vector>
@@ -178,68 +179,146 @@ extern "C" int pava_read_coo_matrix(const char *path, pava_coo_matrix *out) {
         free(vals);
         return set_error(PAVA_MM_ENOMEM, "could not allocate COO arrays (nnz %lld)", cap);
     }
-    long long cnt = 0;
-    for (long long i = 0; i < L; ++i) {
-        // skip blank / comment lines between entries
-        for (;;) {
-            p = skip_ws(p);
-            if (*p == '%') {
-                const char *e = strchr(p, '\n');
-                p = e ? e + 1 : p + strlen(p);
+    // The body is parsed in parallel: the text is cut at line starts into one
+    // piece per thread, each piece parsed into its own arrays, and the pieces
+    // are concatenated in file order (so duplicates later sum in file order).
+    const char *body = p;
+    const size_t blen = strlen(body);
+    int T = 1;
+#ifdef _OPENMP
+    if (blen >= ((size_t)1 << 22)) T = std::max(1, std::min(omp_get_max_threads(), 64));
+#endif
+    std::vector<size_t> cut((size_t)T + 1, blen);
+    cut[0] = 0;
+    for (int t = 1; t < T; ++t) {
+        size_t q = std::max(cut[(size_t)t - 1], blen * (size_t)t / (size_t)T);
+        while (q < blen && body[q - 1] != '\n') ++q;
+        cut[(size_t)t] = q;
+    }
+    struct Part {
+        std::vector<int> r, c;
+        std::vector<double> v;
+        long long bad = -1;  // local index of the first malformed entry
+        long long oor = -1;  // local index of the first out-of-range entry
+        long long br = 0, bc = 0;
+    };
+    std::vector<Part> parts((size_t)T);
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < T; ++t) {
+        Part &P = parts[(size_t)t];
+        const char *q = body + cut[(size_t)t];
+        const char *qe = body + cut[(size_t)t + 1];
+        while (q < qe) {
+            q = skip_ws(q);
+            if (q >= qe) break;
+            if (*q == '%') {
+                while (q < qe && *q != '\n') ++q;
                 continue;
             }
-            break;
+            char *end;
+            const long long r = strtoll(q, &end, 10);
+            if (end == q) {
+                P.bad = (long long)P.r.size();
+                break;
+            }
+            q = end;
+            const long long c = strtoll(q, &end, 10);
+            if (end == q) {
+                P.bad = (long long)P.r.size();
+                break;
+            }
+            q = end;
+            double v = 1.0;
+            if (!b.pattern) {
+                v = strtod(q, &end);
+                if (end == q) {
+                    P.bad = (long long)P.r.size();
+                    break;
+                }
+                q = end;
+                if (b.complex_) {
+                    (void)strtod(q, &end);  // imaginary part ignored, as the reference does
+                    if (end == q) {
+                        P.bad = (long long)P.r.size();
+                        break;
+                    }
+                    q = end;
+                }
+            }
+            if ((r < 1 || r > M || c < 1 || c > N) && P.oor < 0) {
+                P.oor = (long long)P.r.size();
+                P.br = r;
+                P.bc = c;
+            }
+            P.r.push_back((int)(r - 1));
+            P.c.push_back((int)(c - 1));
+            P.v.push_back(v);
         }
-        if (*p == 0) {
+    }
+    // first error in file order, entries beyond L ignored (the reference reads L)
+    long long seen = 0;
+    int nparts_used = T;
+    for (int t = 0; t < T; ++t) {
+        const Part &P = parts[(size_t)t];
+        const long long n = (long long)P.r.size();
+        const long long firstbad = P.bad;
+        if (P.oor >= 0 && seen + P.oor < L && (firstbad < 0 || P.oor < firstbad)) {
             free(rows);
             free(cols);
             free(vals);
-            return set_error(PAVA_MM_EBODY, "truncated body: %lld of %lld entries", i, L);
+            return set_error(PAVA_MM_EBODY, "entry %lld (%lld, %lld) outside %lldx%lld", seen + P.oor + 1, P.br, P.bc,
+                             M, N);
         }
-        char *end;
-        long long r = strtoll(p, &end, 10);
-        if (end == p) goto bad;
-        p = end;
-        {
-            long long c = strtoll(p, &end, 10);
-            if (end == p) goto bad;
-            p = end;
-            double v = 1.0;
-            if (!b.pattern) {
-                v = strtod(p, &end);
-                if (end == p) goto bad;
-                p = end;
-                if (b.complex_) {
-                    (void)strtod(p, &end);  // imaginary part ignored, as the reference does
-                    if (end == p) goto bad;
-                    p = end;
-                }
-            }
-            if (r < 1 || r > M || c < 1 || c > N) {
-                free(rows);
-                free(cols);
-                free(vals);
-                return set_error(PAVA_MM_EBODY, "entry %lld (%lld, %lld) outside %lldx%lld", i + 1, r, c,
-                                 M, N);
-            }
-            rows[cnt] = (int)(r - 1);
-            cols[cnt] = (int)(c - 1);
-            vals[cnt] = v;
-            ++cnt;
-            if (mirror && r != c) {
-                rows[cnt] = (int)(c - 1);
-                cols[cnt] = (int)(r - 1);
-                vals[cnt] = b.skew ? -v : v;
-                ++cnt;
-            }
+        if (firstbad >= 0 && seen + firstbad < L) {
+            free(rows);
+            free(cols);
+            free(vals);
+            return set_error(PAVA_MM_EBODY, "malformed entry %lld", seen + firstbad + 1);
         }
-        continue;
-    bad:
+        seen += n;
+        if (seen >= L) {
+            nparts_used = t + 1;
+            break;
+        }
+    }
+    if (seen < L) {
         free(rows);
         free(cols);
         free(vals);
-        return set_error(PAVA_MM_EBODY, "malformed entry %lld", i + 1);
+        return set_error(PAVA_MM_EBODY, "truncated body: %lld of %lld entries", seen, L);
     }
+    // concatenate (with the mirrored halves) in file order
+    std::vector<long long> in_off((size_t)nparts_used + 1, 0), out_off((size_t)nparts_used + 1, 0);
+    for (int t = 0; t < nparts_used; ++t) {
+        const Part &P = parts[(size_t)t];
+        const long long take = std::min<long long>((long long)P.r.size(), L - in_off[(size_t)t]);
+        in_off[(size_t)t + 1] = in_off[(size_t)t] + take;
+        long long m = take;
+        if (mirror)
+            for (long long k = 0; k < take; ++k) m += (P.r[(size_t)k] != P.c[(size_t)k]);
+        out_off[(size_t)t + 1] = out_off[(size_t)t] + m;
+    }
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < nparts_used; ++t) {
+        const Part &P = parts[(size_t)t];
+        const long long take = in_off[(size_t)t + 1] - in_off[(size_t)t];
+        long long d = out_off[(size_t)t];
+        for (long long k = 0; k < take; ++k) {
+            const int r = P.r[(size_t)k], c = P.c[(size_t)k];
+            const double v = P.v[(size_t)k];
+            rows[d] = r;
+            cols[d] = c;
+            vals[d] = v;
+            ++d;
+            if (mirror && r != c) {
+                rows[d] = c;
+                cols[d] = r;
+                vals[d] = b.skew ? -v : v;
+                ++d;
+            }
+        }
+    }
+    const long long cnt = out_off[(size_t)nparts_used];
     out->nnz = (int)cnt;
     out->numRows = (int)M;
     out->numCols = (int)N;
@@ -362,7 +441,75 @@ extern "C" int pava_convert_coo_to_csr(const pava_coo_matrix *coo, pava_csr_matr
     return csr_from_buckets(M, N, start, bcol, bval, out);
 }
 
+// ---------------------------------------------------------- binary cache
+// "<file>.csrbin": 16-byte header ("PAVACSR1"), int32 numRows, numCols, int64
+// nnz, then rowOffsets (numRows+1 int32), cols (nnz int32), vals (nnz f64),
+// little-endian. Read back with one fread per array.
+static const char kCsrMagic[8] = {'P', 'A', 'V', 'A', 'C', 'S', 'R', '1'};
+
+extern "C" int pava_write_csr_binary(const char *path, const pava_csr_matrix *A) {
+    if (!path || !A) return set_error(PAVA_HIP_EINVAL, "null argument");
+    int rc = pava_csr_validate(A);
+    if (rc) return rc;
+    std::string tmp = std::string(path) + ".tmp";
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return set_error(PAVA_HIP_EIO, "cannot open %s for writing", tmp.c_str());
+    const int32_t dims[2] = {A->numRows, A->numCols};
+    const int64_t nnz = A->nnz;
+    bool ok = fwrite(kCsrMagic, 1, 8, f) == 8 && fwrite(dims, 4, 2, f) == 2 && fwrite(&nnz, 8, 1, f) == 1 &&
+              fwrite(A->rowOffsets, 4, (size_t)A->numRows + 1, f) == (size_t)A->numRows + 1 &&
+              fwrite(A->cols, 4, (size_t)nnz, f) == (size_t)nnz && fwrite(A->vals, 8, (size_t)nnz, f) == (size_t)nnz;
+    ok = (fclose(f) == 0) && ok;
+    if (!ok || rename(tmp.c_str(), path) != 0) {
+        remove(tmp.c_str());
+        return set_error(PAVA_HIP_EIO, "write failed: %s", path);
+    }
+    return 0;
+}
+
+extern "C" int pava_read_csr_binary(const char *path, pava_csr_matrix *out) {
+    if (!path || !out) return set_error(PAVA_HIP_EINVAL, "null argument");
+    memset(out, 0, sizeof *out);
+    FILE *f = fopen(path, "rb");
+    if (!f) return set_error(PAVA_MM_EOPEN, "cannot open %s", path);
+    char magic[8];
+    int32_t dims[2];
+    int64_t nnz = 0;
+    if (fread(magic, 1, 8, f) != 8 || memcmp(magic, kCsrMagic, 8) != 0 || fread(dims, 4, 2, f) != 2 ||
+        fread(&nnz, 8, 1, f) != 1 || dims[0] < 0 || dims[1] < 0 || nnz < 0 || nnz > INT32_MAX) {
+        fclose(f);
+        return set_error(PAVA_MM_EBANNER, "%s is not a CSR cache file", path);
+    }
+    int *rp = (int *)aligned_alloc64(4 * ((size_t)dims[0] + 1));
+    int *ci = (int *)aligned_alloc64(4 * (size_t)std::max<int64_t>(nnz, 1));
+    double *va = (double *)aligned_alloc64(8 * (size_t)std::max<int64_t>(nnz, 1));
+    bool ok = rp && ci && va && fread(rp, 4, (size_t)dims[0] + 1, f) == (size_t)dims[0] + 1 &&
+              fread(ci, 4, (size_t)nnz, f) == (size_t)nnz && fread(va, 8, (size_t)nnz, f) == (size_t)nnz;
+    fclose(f);
+    if (!ok) {
+        free(rp);
+        free(ci);
+        free(va);
+        return set_error(PAVA_MM_EBODY, "truncated CSR cache %s", path);
+    }
+    out->numRows = dims[0];
+    out->numCols = dims[1];
+    out->nnz = (int)nnz;
+    out->rowOffsets = rp;
+    out->cols = ci;
+    out->vals = va;
+    int rc = pava_csr_validate(out);
+    if (rc) pava_delete_csr_matrix(out);
+    return rc;
+}
+
 extern "C" int pava_read_csr_matrix(const char *path, pava_csr_matrix *out) {
+    if (!path || !out) return set_error(PAVA_HIP_EINVAL, "null argument");
+    // a cache next to the file, newer than it, is used instead of parsing
+    const std::string cache = std::string(path) + ".csrbin";
+    struct stat st_m, st_c;
+    if (stat(path, &st_m) == 0 && stat(cache.c_str(), &st_c) == 0 && st_c.st_mtime >= st_m.st_mtime)
+        if (pava_read_csr_binary(cache.c_str(), out) == 0) return 0;
     pava_coo_matrix coo;
     int rc = pava_read_coo_matrix(path, &coo);
     if (rc != 0) return rc;
