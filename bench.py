@@ -211,6 +211,8 @@ def main():
     ap.add_argument("--nmat", type=int, default=512, help="batch variant: matrices")
     ap.add_argument("--x-hot-kb", type=int, default=2048,
                     help="hot-x staging buffer (KiB; 0 = off): the most used columns read from a dense copy")
+    ap.add_argument("--x-lds-kb", type=int, default=-1,
+                    help="of the hot-x buffer, KiB held in LDS per workgroup (-1 = as much as fits, 0 = none)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run 3 plain device copies of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -254,7 +256,7 @@ def main():
           f"gen {t_gen:.1f}s", file=sys.stderr, flush=True)
 
     S = args.steps_per_chunk or 256
-    h = P.CvrHip(A, steps_per_chunk=S, precision=w["prec"], x_hot_kb=args.x_hot_kb)
+    h = P.CvrHip(A, steps_per_chunk=S, precision=w["prec"], x_hot_kb=args.x_hot_kb, x_lds_kb=args.x_lds_kb)
     info = h.info()
     dev = torch.device("cuda:0")
     dx = torch.from_numpy(x).to(dev)
@@ -298,14 +300,16 @@ def main():
         "config": {"workload": args.workload, "rows": A.numRows, "cols": A.numCols, "nnz": A.nnz,
                    "max_row_len": int(np.diff(A.rowOffsets).max()), "steps_per_chunk": S,
                    "chunks": info["nchunks"], "algorithmic_bytes": B, "x_hot_kb": args.x_hot_kb,
-                   "hot_columns": info["n_hot_cols"]},
+                   "hot_columns": info["n_hot_cols"], "lds_columns": info["n_lds_cols"]},
         "nnz_per_s": A.nnz / (ms_step * 1e-3),
         "gflops": 2.0 * A.nnz / (ms_step * 1e-3) / 1e9,
         "hbm_frac": value / HBM_PEAK_GBS,
         "convert_ms": round(info["convert_ms"], 3),
+        "hot_setup_ms": round(info["hot_setup_ms"], 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "spmv_kernel<double,double,double,false,false>",
+                     "kernel": "spmv_kernel<double,double,double,false,false,false,%s>"
+                     % ("true" if info["n_lds_cols"] else "false"),
                      "kernel_ms": round(ms_main, 5)},
     }
 

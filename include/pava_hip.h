@@ -169,8 +169,11 @@ typedef struct cvr_hip_opts {
     int keep_fp64;           /* keep the FP64 CSR resident (required by recompute) */
     int device;              /* HIP device ordinal */
     int x_hot_kb;            /* hot-x staging: the most used columns (each used >= 2x, at most
-                                x_hot_kb KiB of x) are renumbered into a dense copy of x refreshed
-                                before each SpMV; 0 = off (default 2048; 32-bit layouts only) */
+                                x_hot_kb KiB of x) are renumbered, most used first, into a dense
+                                copy of x refreshed before each SpMV; 0 = off (default; 32-bit
+                                layouts only) */
+    int x_lds_kb;            /* of the hot-x copy, the first x_lds_kb KiB are held in LDS by each
+                                workgroup of the SpMV; -1 = as much as fits (default), 0 = none */
 } cvr_hip_opts;
 
 typedef struct cvr_hip_info {
@@ -184,6 +187,8 @@ typedef struct cvr_hip_info {
     long long device_bytes;  /* device memory held by the handle */
     double convert_ms;       /* GPU time of the CSR->CVR conversion (hipEvent) */
     int n_hot_cols;          /* columns staged in the hot-x copy */
+    int n_lds_cols;          /* of those, columns the SpMV reads from LDS */
+    double hot_setup_ms;     /* wall time of the hot-column staging (counts, selection, remap) */
 } cvr_hip_info;
 
 typedef struct cvr_hip_handle cvr_hip_handle;

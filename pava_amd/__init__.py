@@ -49,7 +49,7 @@ class _CSR(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("steps_per_chunk", C.c_int), ("precision", C.c_int), ("col_index_bits", C.c_int),
                 ("recompute", C.c_int), ("kappa_threshold", C.c_double), ("keep_fp64", C.c_int),
-                ("device", C.c_int), ("x_hot_kb", C.c_int)]
+                ("device", C.c_int), ("x_hot_kb", C.c_int), ("x_lds_kb", C.c_int)]
 
 
 class _Info(C.Structure):
@@ -57,7 +57,8 @@ class _Info(C.Structure):
                 ("steps_per_chunk", C.c_int), ("precision", C.c_int), ("col_index_bits", C.c_int),
                 ("layout_elems", C.c_longlong), ("seg_capacity", C.c_longlong), ("max_slots", C.c_int),
                 ("has_empty_rows", C.c_int), ("n_fixup_rows", C.c_int), ("device_bytes", C.c_longlong),
-                ("convert_ms", C.c_double), ("n_hot_cols", C.c_int)]
+                ("convert_ms", C.c_double), ("n_hot_cols", C.c_int), ("n_lds_cols", C.c_int),
+                ("hot_setup_ms", C.c_double)]
 
 
 class _GenParams(C.Structure):
@@ -285,13 +286,13 @@ class CvrHip:
 
     def __init__(self, A: CSR, steps_per_chunk: int = 256, precision: int = PREC_F64, col_index_bits: int = 32,
                  recompute: bool = False, kappa_threshold: float = 64.0, keep_fp64: bool = False, device: int = 0,
-                 x_hot_kb: int = 0):
+                 x_hot_kb: int = 0, x_lds_kb: int = -1):
         L = lib()
         o = _Opts()
         L.cvr_hip_default_opts(C.byref(o))
         o.steps_per_chunk, o.precision, o.col_index_bits = steps_per_chunk, precision, col_index_bits
         o.recompute, o.kappa_threshold, o.keep_fp64, o.device = int(recompute), kappa_threshold, int(keep_fp64), device
-        o.x_hot_kb = x_hot_kb
+        o.x_hot_kb, o.x_lds_kb = x_hot_kb, x_lds_kb
         self._h = C.c_void_p()
         s = A.c_struct()
         _check(L.cvr_hip_create(C.byref(s), C.byref(o), C.byref(self._h)))

@@ -31,6 +31,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "cvr_kernels.h"
@@ -375,107 +377,98 @@ __global__ __launch_bounds__(256) void convert_kernel(ConvertArgs a) {
 }
 
 // ---------------------------------------------------------------- hot columns
-// Column use counts, a log2 histogram of them, an ordered selection of the
-// columns used at least `t` times, and the layout rewrite that renumbers those
-// columns into the dense hot-x buffer (bit 30 of the column marks them).
-__global__ void col_hist_kernel(const int *__restrict__ ci, long long nnz, int *__restrict__ cnt) {
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (long long)gridDim.x * blockDim.x)
-        atomicAdd(&cnt[ci[k]], 1);
-}
+// Column use counts (GPU), the most used columns in descending use order
+// (host: counts are copied back once), and the layout rewrite that renumbers
+// those columns into the dense hot-x buffer (bit 30 of a column marks them).
+// Hot number 0 is the most used column, so the first entries of the buffer
+// are the ones worth keeping in LDS.
+// Counts go through a direct-mapped LDS table first (a column claims a free
+// slot of its hash once per workgroup), so a heavily used column costs one
+// global atomic per workgroup instead of one per use; a column whose slot is
+// held by another column counts globally.
+constexpr int kHistSlots = 4096;
 
-// per-block LDS histogram, one global atomic per bucket per block
-__global__ __launch_bounds__(256) void cnt_bucket_kernel(const int *__restrict__ cnt, int N, int *__restrict__ buckets) {
-    __shared__ int h[32];
-    if (threadIdx.x < 32) h[threadIdx.x] = 0;
+__global__ __launch_bounds__(256) void col_hist_kernel(const int *__restrict__ ci, long long nnz, int *__restrict__ cnt) {
+    __shared__ int key[kHistSlots];
+    __shared__ int val[kHistSlots];
+    for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x) key[i] = -1, val[i] = 0;
     __syncthreads();
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
-        const int v = cnt[c];
-        if (v > 0) atomicAdd(&h[31 - __clz(v)], 1);
-    }
-    __syncthreads();
-    if (threadIdx.x < 32 && h[threadIdx.x]) atomicAdd(&buckets[threadIdx.x], h[threadIdx.x]);
-}
-
-// per 1024-column block: number of columns with cnt >= t
-__global__ __launch_bounds__(1024) void hot_count_kernel(const int *__restrict__ cnt, int N, int t, int *__restrict__ bcnt) {
-    __shared__ int s;
-    if (threadIdx.x == 0) s = 0;
-    __syncthreads();
-    const int c = blockIdx.x * 1024 + threadIdx.x;
-    const bool f = c < N && cnt[c] >= t;
-    const unsigned long long m = __ballot(f);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&s, __popcll(m));
-    __syncthreads();
-    if (threadIdx.x == 0) bcnt[blockIdx.x] = s;
-}
-
-// ordered ranks: colmap[c] = hot index (column order) or -1; hot_cols[idx] = c
-__global__ __launch_bounds__(1024) void hot_assign_kernel(const int *__restrict__ cnt, int N, int t,
-                                                          const int *__restrict__ boff, int *__restrict__ colmap,
-                                                          int *__restrict__ hot_cols) {
-    __shared__ int wsum[16];
-    const int c = blockIdx.x * 1024 + threadIdx.x;
-    const int w = threadIdx.x >> 6;
-    const bool f = c < N && cnt[c] >= t;
-    const unsigned long long m = __ballot(f);
-    if ((threadIdx.x & 63) == 0) wsum[w] = __popcll(m);
-    __syncthreads();
-    int off = boff[blockIdx.x];
-    for (int i = 0; i < w; ++i) off += wsum[i];
-    if (c < N) {
-        if (f) {
-            const int idx = off + mbcnt64(m);
-            colmap[c] = idx;
-            hot_cols[idx] = c;
-        } else {
-            colmap[c] = -1;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+         k += (long long)gridDim.x * blockDim.x) {
+        const int col = ci[k];
+        const int slot = (int)(((unsigned)col * 0x9E3779B1u) >> 20);  // 12-bit multiplicative hash
+        int held = key[slot];
+        if (held == -1) {
+            held = atomicCAS(&key[slot], -1, col);
+            if (held == -1) held = col;
         }
+        if (held == col)
+            atomicAdd(&val[slot], 1);
+        else
+            atomicAdd(&cnt[col], 1);
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x)
+        if (val[i]) atomicAdd(&cnt[key[i]], val[i]);
+}
+
+__global__ void hot_scatter_kernel(const int *__restrict__ hot_cols, int n, int *__restrict__ colmap) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) colmap[hot_cols[k]] = k;
 }
 
 __global__ void remap_hot_kernel(int *__restrict__ cols, long long L, const int *__restrict__ colmap) {
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < L; e += (long long)gridDim.x * blockDim.x) {
         const int v = cols[e];
-        const int m = colmap[v & 0x3fffffff];
+        const int m = colmap[v & kColMask];
         if (m >= 0) cols[e] = (v & kEndFlag) | kHotFlag | m;
     }
 }
 
-hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, int *cnt, int *buckets, int *bcnt,
-                              int *boff, int *colmap, int *hot_cols, int *cols, long long L, int *n_hot,
-                              hipStream_t st) {
-    // counts (cnt, buckets zeroed by the caller)
+hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, int *cnt, int *colmap, int *hot_cols,
+                              int *cols, long long L, int *n_hot, hipStream_t st) {
+    *n_hot = 0;
+    hipError_t e;
+    if ((e = hipMemsetAsync(cnt, 0, (size_t)N * 4, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(col_hist_kernel, dim3(2048), dim3(256), 0, st, ci, nnz, cnt);
-    hipLaunchKernelGGL(cnt_bucket_kernel, dim3(std::min((N + 255) / 256, 1024)), dim3(256), 0, st, cnt, N, buckets);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    int hb[32];
-    if ((e = hipMemcpyAsync(hb, buckets, sizeof hb, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
-    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-    // threshold t = 2^b: the largest hot set that fits, columns used at least twice
-    int b = 31, total = 0;
-    while (b >= 1 && total + hb[b] <= max_hot) total += hb[b--];
-    const int bstar = b + 1;
-    *n_hot = total;
-    if (total == 0 || bstar > 31) {
-        *n_hot = 0;
-        return hipSuccess;
-    }
-    const int t = 1 << bstar;
-    const int nb = (N + 1023) / 1024;
-    hipLaunchKernelGGL(hot_count_kernel, dim3(nb), dim3(1024), 0, st, cnt, N, t, bcnt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    std::vector<int> bc((size_t)nb), bo((size_t)nb);
-    if ((e = hipMemcpyAsync(bc.data(), bcnt, (size_t)nb * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    std::vector<int> h((size_t)N);
+    if ((e = hipMemcpyAsync(h.data(), cnt, (size_t)N * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-    int run = 0;
-    for (int i = 0; i < nb; ++i) bo[(size_t)i] = run, run += bc[(size_t)i];
-    *n_hot = run;
-    if ((e = hipMemcpyAsync(boff, bo.data(), (size_t)nb * 4, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(hot_assign_kernel, dim3(nb), dim3(1024), 0, st, cnt, N, t, boff, colmap, hot_cols);
+    // the max_hot most used columns (each used at least twice), ordered by
+    // count descending, then column ascending: a counting sort over counts
+    int maxc = 0;
+    for (int c = 0; c < N; ++c) maxc = std::max(maxc, h[(size_t)c]);
+    std::vector<long long> nwith((size_t)maxc + 2, 0);  // columns per count value
+    for (int c = 0; c < N; ++c) ++nwith[(size_t)h[(size_t)c]];
+    long long taken = 0;
+    int t = maxc + 1;  // threshold: every column with count > t is taken
+    while (t > 2 && taken + nwith[(size_t)t - 1] <= max_hot) taken += nwith[(size_t)--t];
+    // t: smallest count fully taken; columns with count t-1 fill what is left
+    const int part = t - 1;
+    const long long part_take = part >= 2 ? std::min<long long>(max_hot - taken, nwith[(size_t)part]) : 0;
+    const int n = (int)(taken + part_take);
+    if (n == 0) return hipSuccess;
+    std::vector<long long> pos((size_t)maxc + 2, 0);  // first output position per count
+    long long acc = 0;
+    for (int v = maxc; v >= std::max(part, 2); --v) pos[(size_t)v] = acc, acc += nwith[(size_t)v];
+    std::vector<int> hc((size_t)n);
+    long long part_used = 0;
+    for (int c = 0; c < N; ++c) {
+        const int v = h[(size_t)c];
+        if (v >= t || (v == part && v >= 2 && part_used < part_take)) {
+            if (v == part && v < t) ++part_used;
+            hc[(size_t)pos[(size_t)v]++] = c;
+        }
+    }
+    if ((e = hipMemcpyAsync(hot_cols, hc.data(), (size_t)n * 4, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(colmap, 0xff, (size_t)N * 4, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(hot_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, st, hot_cols, n, colmap);
     hipLaunchKernelGGL(remap_hot_kernel, dim3(2048), dim3(256), 0, st, cols, L, colmap);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    return hipStreamSynchronize(st);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    *n_hot = n;
+    return hipSuccess;
 }
 
 // Host-side launchers --------------------------------------------------------

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -42,10 +43,12 @@ struct cvr_hip_handle {
     void *d_hx = nullptr, *d_hy = nullptr, *d_hx64 = nullptr;
     long long device_bytes = 0;
     double convert_ms = 0.0;
+    double hot_setup_ms = 0.0;  // wall time of the hot-column staging
     // hot-x staging
     int *d_hot_cols = nullptr;
     void *d_xhot = nullptr;
     int n_hot = 0;
+    int n_lds = 0, lds_blocks = 0;  // hot columns held in LDS by the SpMV, and its grid
     std::vector<void *> allocs;
 };
 
@@ -79,6 +82,7 @@ extern "C" void cvr_hip_default_opts(cvr_hip_opts *o) {
     o->keep_fp64 = 0;
     o->device = 0;
     o->x_hot_kb = 0;
+    o->x_lds_kb = -1;
 }
 
 static int value_size(int prec) { return prec == CVR_HIP_PREC_F64 ? 8 : prec == CVR_HIP_PREC_F32 ? 4 : 2; }
@@ -102,6 +106,8 @@ static int validate_opts(const cvr_hip_opts *o) {
         return set_error(PAVA_HIP_EINVAL, "col_index_bits must be 32 or 16");
     if (o->col_index_bits == 16 && o->precision == CVR_HIP_PREC_F64)
         return set_error(PAVA_HIP_EINVAL, "16-bit columns are offered for FP32/BF16 storage only");
+    if (o->x_hot_kb < 0) return set_error(PAVA_HIP_EINVAL, "x_hot_kb must be >= 0, got %d", o->x_hot_kb);
+    if (o->x_lds_kb < -1) return set_error(PAVA_HIP_EINVAL, "x_lds_kb must be >= -1, got %d", o->x_lds_kb);
     return 0;
 }
 
@@ -209,33 +215,30 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     const int xsize = h->prec == CVR_HIP_PREC_F64 ? 8 : 4;
     const long long max_hot = (long long)o->x_hot_kb * 1024 / xsize;
     if (max_hot > 0 && !h->col16 && nnz > 0 && N < (1 << 30)) {
-        int *cnt = nullptr, *buckets = nullptr, *bcnt = nullptr, *boff = nullptr, *colmap = nullptr;
-        const int nb = (N + 1023) / 1024;
+        const auto t0 = std::chrono::steady_clock::now();
+        int *cnt = nullptr, *colmap = nullptr;
         hipError_t e = hipSuccess;
         int nh = 0;
-        if (hipMalloc(&cnt, (size_t)N * 4 + 4) != hipSuccess || hipMalloc(&buckets, 256) != hipSuccess ||
-            hipMalloc(&bcnt, (size_t)nb * 4 + 4) != hipSuccess || hipMalloc(&boff, (size_t)nb * 4 + 4) != hipSuccess ||
-            hipMalloc(&colmap, (size_t)N * 4 + 4) != hipSuccess ||
+        if (hipMalloc(&cnt, (size_t)N * 4 + 4) != hipSuccess || hipMalloc(&colmap, (size_t)N * 4 + 4) != hipSuccess ||
             dev_alloc(h, (void **)&h->d_hot_cols, (size_t)std::min<long long>(max_hot, N) * 4 + 4) != 0) {
             e = hipErrorOutOfMemory;
         } else {
-            e = hipMemsetAsync(cnt, 0, (size_t)N * 4, h->stream);
-            if (e == hipSuccess) e = hipMemsetAsync(buckets, 0, 256, h->stream);
-            if (e == hipSuccess)
-                e = cvr::launch_hot_columns(d_ci, nnz, N, (int)std::min<long long>(max_hot, N), cnt, buckets, bcnt, boff,
-                                            colmap, h->d_hot_cols, (int *)h->d_cols, h->layout_elems, &nh, h->stream);
+            e = cvr::launch_hot_columns(d_ci, nnz, N, (int)std::min<long long>(max_hot, N), cnt, colmap, h->d_hot_cols,
+                                        (int *)h->d_cols, h->layout_elems, &nh, h->stream);
         }
+        (void)hipStreamSynchronize(h->stream);
         (void)hipFree(cnt);
-        (void)hipFree(buckets);
-        (void)hipFree(bcnt);
-        (void)hipFree(boff);
         (void)hipFree(colmap);
         if (e != hipSuccess) return set_error(PAVA_HIP_EDEVICE, "hot-column staging failed: %s", hipGetErrorString(e));
         h->n_hot = nh;
         if (nh > 0) {
             int rc2 = dev_alloc(h, &h->d_xhot, (size_t)nh * xsize);
             if (rc2) return rc2;
+            const long long lds_bytes = o->x_lds_kb < 0 ? -1 : (long long)o->x_lds_kb * 1024;
+            HIP_TRY(cvr::plan_spmv_lds(h->prec, h->d_rflag != nullptr, nh, lds_bytes, h->nchunks, &h->n_lds,
+                                       &h->lds_blocks));
         }
+        h->hot_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return 0;
 }
@@ -359,6 +362,8 @@ extern "C" int cvr_hip_get_info(const cvr_hip_handle *h, cvr_hip_info *info) {
     info->device_bytes = h->device_bytes;
     info->convert_ms = h->convert_ms;
     info->n_hot_cols = h->n_hot;
+    info->n_lds_cols = h->n_lds;
+    info->hot_setup_ms = h->hot_setup_ms;
     return 0;
 }
 
@@ -386,6 +391,8 @@ static cvr::SpmvArgs spmv_args(const cvr_hip_handle *h, const void *d_x, void *d
     a.M = h->M;
     a.has_empty = h->has_empty;
     a.recompute = h->d_rflag != nullptr;
+    a.n_lds = h->n_lds;
+    a.lds_blocks = h->lds_blocks;
     return a;
 }
 

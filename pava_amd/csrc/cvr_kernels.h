@@ -9,6 +9,7 @@ namespace cvr {
 
 constexpr int kLanes = 64;          // one wavefront = one CVR "SIMD unit"
 constexpr int kWavesPerBlock = 4;   // 256-thread workgroups, one chunk per wave
+constexpr int kWavesLds = 16;       // LDS hot-x variant: 1024-thread workgroups, one per CU
 constexpr int kStepGroup = 8;       // steps per chunk are a multiple of this
 constexpr int kSplitCap = 72;       // split-table entries per chunk (groups + sentinel)
 constexpr int kSegExtra = 66;       // segment-list slack per chunk (steals + padding piece)
@@ -68,6 +69,8 @@ struct SpmvArgs {
     float kappa;         // recompute threshold on sum|a x| / |sum a x|
     long long layout_elems;
     int nchunks, S, M, has_empty, recompute;
+    int n_lds;       // hot columns [0, n_lds) read from an LDS copy (0: no LDS table)
+    int lds_blocks;  // grid of the LDS variant
 };
 
 // FP64 recompute of flagged rows, one wave per chunk (mixed precision only).
@@ -115,11 +118,14 @@ hipError_t launch_spmm_batch(const SpmmDesc *d_descs, const int2 *d_queue, int t
 hipError_t launch_empty_rows(const int *rp, int M, unsigned *bits, int *status, hipStream_t st);
 hipError_t launch_convert(const ConvertArgs &a, hipStream_t st);
 hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st);
+// LDS table size for a handle: n_lds (0 = no LDS variant) and its grid.
+// max_bytes < 0: as much as the workgroup can hold.
+hipError_t plan_spmv_lds(int prec, int recompute, int n_hot, long long max_bytes, int nchunks, int *n_lds,
+                         int *lds_blocks);
 hipError_t launch_fixup(const int *fix, const double *carry, void *y, int nchunks, int prec, hipStream_t st);
 hipError_t launch_finish(const FinishArgs &a, hipStream_t st);
-hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, int *cnt, int *buckets, int *bcnt,
-                              int *boff, int *colmap, int *hot_cols, int *cols, long long L, int *n_hot,
-                              hipStream_t st);
+hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, int *cnt, int *colmap, int *hot_cols,
+                              int *cols, long long L, int *n_hot, hipStream_t st);
 hipError_t launch_hot_gather(const void *x, const int *hot_cols, void *xhot, int n_hot, int xsize, hipStream_t st);
 
 }  // namespace cvr

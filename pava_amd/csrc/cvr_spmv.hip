@@ -9,9 +9,9 @@
 // lane picks its destination from the chunk's segment list at
 // (segments ended so far + its rank among this step's ending lanes) -- a row
 // of y (pava's record-driven store, cvr/cvr.h:1303-1372) or an LDS slot for a
-// segment of a split row. The segment list is read through a 128-entry
-// register window (two VGPRs per lane, ds_bpermute lookups) refilled 64
-// entries ahead. After the last step the wave sums the slots of each split row
+// segment of a split row. Each 8-step group's segment entries (exactly as
+// many as its ends) are prefetched with the previous group's loads and staged
+// in LDS. After the last step the wave sums the slots of each split row
 // in element order (pava's t_rets / final_2 merge, cvr/cvr.h:1823-1868):
 // rows wholly inside the chunk are written, the partial of a row shared with
 // a neighbouring chunk goes to a per-chunk carry, which the fixup kernel adds
@@ -97,203 +97,231 @@ __device__ __forceinline__ void store_y(void *y, int r, double v) {
     reinterpret_cast<YT *>(y)[r] = (YT)v;
 }
 
-template <typename VT, typename XT, typename YT, bool COMP, bool COL16, bool KAPPA>
+// LDSX: 16-wave workgroups (one per CU) first copy the head of the hot-x
+// buffer -- the most used columns, hot numbers [0, n_lds) -- into LDS, then
+// each wave walks chunks of the workgroup's contiguous chunk range; a gather
+// of such a column is an LDS read instead of a divergent global load.
+template <typename VT, typename XT, typename YT, bool COMP, bool COL16, bool KAPPA, bool LDSX>
 // 32-bit layouts: >= 4 waves per SIMD (128 VGPRs, no spills); the 16-bit
 // layouts keep their end masks in registers and are left unconstrained
-__global__ __launch_bounds__(256, COL16 ? 1 : 4) void spmv_kernel(SpmvArgs a) {
+__global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock, (COL16 && !LDSX) ? 1 : 4) void spmv_kernel(
+    SpmvArgs a) {
+    constexpr int WPB = LDSX ? kWavesLds : kWavesPerBlock;
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    const int c = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wv;
-    if (c >= a.nchunks) return;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     constexpr int G = kStepGroup;
 
-    __shared__ double slots[kWavesPerBlock][kSlotCap];
-    __shared__ double slots_abs[KAPPA ? kWavesPerBlock : 1][KAPPA ? kSlotCap : 1];
+    __shared__ double slots[WPB][kSlotCap];
+    __shared__ double slots_abs[KAPPA ? WPB : 1][KAPPA ? kSlotCap : 1];
+    __shared__ int segbuf[WPB][64 * kStepGroup];
+    extern __shared__ __align__(16) unsigned char lds_tab[];
+    const XT *__restrict__ tab = reinterpret_cast<const XT *>(lds_tab);
 
-    const long long C = 64LL * a.S;
-    const size_t base = (size_t)C * (size_t)c;
-    const int steps = (c == a.nchunks - 1) ? (int)((a.layout_elems - (long long)base) >> 6) : a.S;
-    const int zr = a.chunk_rows[c];
-    const int *__restrict__ segs = a.seg + zr + kSegExtra * c;
-    const XT *__restrict__ x = reinterpret_cast<const XT *>(a.x);
-    const VT *__restrict__ vp = reinterpret_cast<const VT *>(a.vals) + base + lane;
-    const int *__restrict__ cp32 = reinterpret_cast<const int *>(a.cols) + base + lane;
-    const unsigned short *__restrict__ cp16 = reinterpret_cast<const unsigned short *>(a.cols) + base + lane;
-    const uint64_t *__restrict__ emp = a.end_mask + (base >> 6);
-    const int cbase = COL16 ? a.col_base[c] : 0;
+    auto run_chunk = [&](const int c) {
+        const long long C = 64LL * a.S;
+        const size_t base = (size_t)C * (size_t)c;
+        const int steps = (c == a.nchunks - 1) ? (int)((a.layout_elems - (long long)base) >> 6) : a.S;
+        const int zr = a.chunk_rows[c];
+        const int *__restrict__ segs = a.seg + zr + kSegExtra * c;
+        const XT *__restrict__ x = reinterpret_cast<const XT *>(a.x);
+        const VT *__restrict__ vp = reinterpret_cast<const VT *>(a.vals) + base + lane;
+        const int *__restrict__ cp32 = reinterpret_cast<const int *>(a.cols) + base + lane;
+        const unsigned short *__restrict__ cp16 = reinterpret_cast<const unsigned short *>(a.cols) + base + lane;
+        const uint64_t *__restrict__ emp = a.end_mask + (base >> 6);
+        const int cbase = COL16 ? a.col_base[c] : 0;
 
-    // Pipeline, one 8-step group deep: while group g is processed, the x
-    // gathers and the segment-list entries of group g+1 (exactly as many as
-    // its row ends, <= 512, issued together with the gathers), the values of
-    // group g+1 and the columns of group g+2 are in flight. A group's segment
-    // entries are copied to LDS when its processing starts, so each ending
-    // lane finds its destination with one LDS read and never waits on memory.
-    __shared__ int segbuf[kWavesPerBlock][64 * kStepGroup];
-    const XT *__restrict__ xh = reinterpret_cast<const XT *>(a.xhot);
-    // x element of a stored column: hot columns read the dense hot-x copy
-    // (an explicit 64-bit address select: a select of two pointers would be
-    // lowered to an indexed private array)
-    const uint64_t ax = reinterpret_cast<uint64_t>(x), ah = reinterpret_cast<uint64_t>(xh);
-    auto xval = [&](int cv) -> XT {
-        if constexpr (COL16) {
-            return x[cbase + cv];
-        } else {
-            const uint64_t b = (cv & kHotFlag) ? ah : ax;
-            return *reinterpret_cast<const XT *>(b + (uint64_t)(uint32_t)(cv & kColMask) * sizeof(XT));
-        }
-    };
-    constexpr int GE = COL16 ? G : 1;  // end masks are stored only for 16-bit layouts
-    auto load_cols = [&](int g0, int (&cc)[G], uint64_t (&em)[GE]) {
-        if constexpr (COL16) {
-#pragma unroll
-            for (int i = 0; i < G; ++i) cc[i] = (int)cp16[(size_t)(g0 + i) * 64];
-#pragma unroll
-            for (int i = 0; i < G; ++i) em[i] = emp[g0 + i];
-        } else {
-#pragma unroll
-            for (int i = 0; i < G; ++i) cc[i] = cp32[(size_t)(g0 + i) * 64];
-        }
-    };
-    auto group_ends = [&](const int (&cc)[G], const uint64_t (&em)[GE]) -> int {
-        int e = 0;
-        if constexpr (COL16) {
-#pragma unroll
-            for (int i = 0; i < G; ++i) e += __popcll(em[i]);
-        } else {
-#pragma unroll
-            for (int i = 0; i < G; ++i) e += __popcll(__ballot(cc[i] < 0));
-        }
-        return e;
-    };
-    auto load_segs = [&](int sb, int ne, int (&r)[G]) {
-#pragma unroll
-        for (int j = 0; j < G; ++j)
-            if (j * 64 < ne) r[j] = segs[sb + j * 64 + lane];
-    };
-
-    Acc<VT, XT, COMP, KAPPA> acc;
-    VT vb[G], vn[G];
-    int cb[G], cn[G], cnn[G];
-    uint64_t eb[GE], en[GE], enn[GE];
-    XT xb[G], xn[G];
-    int sc[G], sn[G];
-
-    // prologue: group 0 fully issued, columns of group 1 in flight
-    load_cols(0, cb, eb);
-#pragma unroll
-    for (int i = 0; i < G; ++i) vb[i] = vp[(size_t)i * 64];
-#pragma unroll
-    for (int i = 0; i < G; ++i) xb[i] = xval(cb[i]);
-    int ecur = group_ends(cb, eb);
-    load_segs(0, ecur, sc);
-    if (G < steps) load_cols(G, cn, en);
-    int sbase = 0, segp = 0;
-
-    for (int g = 0; g < steps; g += G) {
-        const bool has1 = (g + G) < steps, has2 = (g + 2 * G) < steps;
-        const int sbase_next = sbase + ecur;
-        int enext = 0;
-        if (has1) {
-#pragma unroll
-            for (int i = 0; i < G; ++i) xn[i] = xval(cn[i]);
-            enext = group_ends(cn, en);
-            load_segs(sbase_next, enext, sn);
-#pragma unroll
-            for (int i = 0; i < G; ++i) vn[i] = vp[(size_t)(g + G + i) * 64];
-            if (has2) load_cols(g + 2 * G, cnn, enn);
-        }
-        // this group's destinations -> LDS
-#pragma unroll
-        for (int j = 0; j < G; ++j)
-            if (j * 64 < ecur) segbuf[wv][j * 64 + lane] = sc[j];
-        wave_sync_lds();
-#pragma unroll
-        for (int i = 0; i < G; ++i) {
-            acc.add(vb[i], xb[i]);
-            bool end;
-            unsigned long long m;
+        // Pipeline, one 8-step group deep: while group g is processed, the x
+        // gathers and the segment-list entries of group g+1 (exactly as many as
+        // its row ends, <= 512, issued together with the gathers), the values of
+        // group g+1 and the columns of group g+2 are in flight. A group's segment
+        // entries are copied to LDS when its processing starts, so each ending
+        // lane finds its destination with one LDS read and never waits on memory.
+        // The layout streams (values, columns, segment entries) are read once:
+        // non-temporal loads keep them from evicting x lines from the caches.
+        const XT *__restrict__ xh = reinterpret_cast<const XT *>(a.xhot);
+        // x element of a stored column: hot columns read the dense hot-x copy
+        // (an explicit 64-bit address select: a select of two pointers would be
+        // lowered to an indexed private array)
+        const uint64_t ax = reinterpret_cast<uint64_t>(x), ah = reinterpret_cast<uint64_t>(xh);
+        auto xval = [&](int cv) -> XT {
             if constexpr (COL16) {
-                m = eb[i];
-                end = (m >> lane) & 1ull;
+                return x[cbase + cv];
             } else {
-                end = cb[i] < 0;
-                m = __ballot(end);
-            }
-            if (m) {
-                if (end) {
-                    const int d = segbuf[wv][segp - sbase + mbcnt64s(m)];
-                    const double v = acc.value();
-                    if (d >= 0) {
-                        store_y<YT>(a.y, d, v);
-                        if (KAPPA && acc.absval() > (double)a.kappa * __builtin_fabs(v)) a.rflag[d] = 1;
-                    } else {
-                        slots[wv][~d] = v;
-                        if (KAPPA) slots_abs[wv][~d] = acc.absval();
-                    }
-                    acc.reset();
+                const uint32_t idx = (uint32_t)(cv & kColMask);
+                if constexpr (LDSX) {
+                    if ((cv & kHotFlag) && idx < (uint32_t)a.n_lds) return tab[idx];
                 }
-                segp += __popcll(m);
+                const uint64_t b = (cv & kHotFlag) ? ah : ax;
+                return *reinterpret_cast<const XT *>(b + (uint64_t)idx * sizeof(XT));
             }
-        }
-        if (has1) {
+        };
+        constexpr int GE = COL16 ? G : 1;  // end masks are stored only for 16-bit layouts
+        auto load_cols = [&](int g0, int (&cc)[G], uint64_t (&em)[GE]) {
+            if constexpr (COL16) {
+#pragma unroll
+                for (int i = 0; i < G; ++i) cc[i] = (int)cp16[(size_t)(g0 + i) * 64];
+#pragma unroll
+                for (int i = 0; i < G; ++i) em[i] = emp[g0 + i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < G; ++i) cc[i] = __builtin_nontemporal_load(&cp32[(size_t)(g0 + i) * 64]);
+            }
+        };
+        auto group_ends = [&](const int (&cc)[G], const uint64_t (&em)[GE]) -> int {
+            int e = 0;
+            if constexpr (COL16) {
+#pragma unroll
+                for (int i = 0; i < G; ++i) e += __popcll(em[i]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < G; ++i) e += __popcll(__ballot(cc[i] < 0));
+            }
+            return e;
+        };
+        auto load_segs = [&](int sb, int ne, int (&r)[G]) {
+#pragma unroll
+            for (int j = 0; j < G; ++j)
+                if (j * 64 < ne) r[j] = __builtin_nontemporal_load(&segs[sb + j * 64 + lane]);
+        };
+
+        Acc<VT, XT, COMP, KAPPA> acc;
+        VT vb[G], vn[G];
+        int cb[G], cn[G], cnn[G];
+        uint64_t eb[GE], en[GE], enn[GE];
+        XT xb[G], xn[G];
+        int sc[G], sn[G];
+
+        // prologue: group 0 fully issued, columns of group 1 in flight
+        load_cols(0, cb, eb);
+#pragma unroll
+        for (int i = 0; i < G; ++i) vb[i] = __builtin_nontemporal_load(&vp[(size_t)i * 64]);
+#pragma unroll
+        for (int i = 0; i < G; ++i) xb[i] = xval(cb[i]);
+        int ecur = group_ends(cb, eb);
+        load_segs(0, ecur, sc);
+        if (G < steps) load_cols(G, cn, en);
+        int sbase = 0, segp = 0;
+
+        for (int g = 0; g < steps; g += G) {
+            const bool has1 = (g + G) < steps, has2 = (g + 2 * G) < steps;
+            const int sbase_next = sbase + ecur;
+            int enext = 0;
+            if (has1) {
+#pragma unroll
+                for (int i = 0; i < G; ++i) xn[i] = xval(cn[i]);
+                enext = group_ends(cn, en);
+                load_segs(sbase_next, enext, sn);
+#pragma unroll
+                for (int i = 0; i < G; ++i) vn[i] = __builtin_nontemporal_load(&vp[(size_t)(g + G + i) * 64]);
+                if (has2) load_cols(g + 2 * G, cnn, enn);
+            }
+            // this group's destinations -> LDS
+#pragma unroll
+            for (int j = 0; j < G; ++j)
+                if (j * 64 < ecur) segbuf[wv][j * 64 + lane] = sc[j];
+            wave_sync_lds();
 #pragma unroll
             for (int i = 0; i < G; ++i) {
-                vb[i] = vn[i];
-                cb[i] = cn[i];
-                xb[i] = xn[i];
-                cn[i] = cnn[i];
-                sc[i] = sn[i];
+                acc.add(vb[i], xb[i]);
+                bool end;
+                unsigned long long m;
+                if constexpr (COL16) {
+                    m = eb[i];
+                    end = (m >> lane) & 1ull;
+                } else {
+                    end = cb[i] < 0;
+                    m = __ballot(end);
+                }
+                if (m) {
+                    if (end) {
+                        const int d = segbuf[wv][segp - sbase + mbcnt64s(m)];
+                        const double v = acc.value();
+                        if (d >= 0) {
+                            store_y<YT>(a.y, d, v);
+                            if (KAPPA && acc.absval() > (double)a.kappa * __builtin_fabs(v)) a.rflag[d] = 1;
+                        } else {
+                            slots[wv][~d] = v;
+                            if (KAPPA) slots_abs[wv][~d] = acc.absval();
+                        }
+                        acc.reset();
+                    }
+                    segp += __popcll(m);
+                }
             }
+            if (has1) {
 #pragma unroll
-            for (int i = 0; i < GE; ++i) {
-                eb[i] = en[i];
-                en[i] = enn[i];
+                for (int i = 0; i < G; ++i) {
+                    vb[i] = vn[i];
+                    cb[i] = cn[i];
+                    xb[i] = xn[i];
+                    cn[i] = cnn[i];
+                    sc[i] = sn[i];
+                }
+#pragma unroll
+                for (int i = 0; i < GE; ++i) {
+                    eb[i] = en[i];
+                    en[i] = enn[i];
+                }
+                sbase = sbase_next;
+                ecur = enext;
             }
-            sbase = sbase_next;
-            ecur = enext;
+            wave_sync_lds();  // the next group overwrites segbuf
         }
-        wave_sync_lds();  // the next group overwrites segbuf
-    }
 
-    // split rows: sum slots in element order
-    wave_sync_lds();
-    const int ns = a.split_n[c];
-    const int2 *tab = a.split_tab + (size_t)c * kSplitCap;
-    for (int k = lane; k < ns; k += 64) {
-        const int2 t = tab[k];
-        const int se = tab[k + 1].y;
-        double s = slots[wv][t.y];
-        for (int j = t.y + 1; j < se; ++j) s += slots[wv][j];
-        double sa = 0.0;
-        if (KAPPA)
-            for (int j = t.y; j < se; ++j) sa += slots_abs[wv][j];
-        if (t.x >= 0) {
-            store_y<YT>(a.y, t.x, s);
-            if (KAPPA && sa > (double)a.kappa * __builtin_fabs(s)) a.rflag[t.x] = 1;
-        } else if (t.x == kCodeHead) {
-            a.carry[2 * c] = s;
-            if (KAPPA) a.carry_abs[2 * c] = sa;
-        } else if (t.x == kCodeTail) {
-            a.carry[2 * c + 1] = s;
-            if (KAPPA) a.carry_abs[2 * c + 1] = sa;
-        }
-    }
-
-    // empty rows: any wave may zero any empty row, so the bitmap words are
-    // split evenly over the waves (not by the chunks' row ranges, which are
-    // very uneven on power-law matrices)
-    if (a.has_empty) {
-        const int nw = (a.M + 31) >> 5;
-        const int w0 = (int)((long long)nw * c / a.nchunks), w1 = (int)((long long)nw * (c + 1) / a.nchunks);
-        for (int w = w0 + lane; w < w1; w += 64) {
-            unsigned bits = a.empty_bits[w];
-            const int rb = w << 5;
-            while (bits) {
-                const int j = __ffs(bits) - 1;
-                store_y<YT>(a.y, rb + j, 0.0);
-                bits &= bits - 1;
+        // split rows: sum slots in element order
+        wave_sync_lds();
+        const int ns = a.split_n[c];
+        const int2 *tab = a.split_tab + (size_t)c * kSplitCap;
+        for (int k = lane; k < ns; k += 64) {
+            const int2 t = tab[k];
+            const int se = tab[k + 1].y;
+            double s = slots[wv][t.y];
+            for (int j = t.y + 1; j < se; ++j) s += slots[wv][j];
+            double sa = 0.0;
+            if (KAPPA)
+                for (int j = t.y; j < se; ++j) sa += slots_abs[wv][j];
+            if (t.x >= 0) {
+                store_y<YT>(a.y, t.x, s);
+                if (KAPPA && sa > (double)a.kappa * __builtin_fabs(s)) a.rflag[t.x] = 1;
+            } else if (t.x == kCodeHead) {
+                a.carry[2 * c] = s;
+                if (KAPPA) a.carry_abs[2 * c] = sa;
+            } else if (t.x == kCodeTail) {
+                a.carry[2 * c + 1] = s;
+                if (KAPPA) a.carry_abs[2 * c + 1] = sa;
             }
         }
+
+        // empty rows: any wave may zero any empty row, so the bitmap words are
+        // split evenly over the waves (not by the chunks' row ranges, which are
+        // very uneven on power-law matrices)
+        if (a.has_empty) {
+            const int nw = (a.M + 31) >> 5;
+            const int w0 = (int)((long long)nw * c / a.nchunks), w1 = (int)((long long)nw * (c + 1) / a.nchunks);
+            for (int w = w0 + lane; w < w1; w += 64) {
+                unsigned bits = a.empty_bits[w];
+                const int rb = w << 5;
+                while (bits) {
+                    const int j = __ffs(bits) - 1;
+                    store_y<YT>(a.y, rb + j, 0.0);
+                    bits &= bits - 1;
+                }
+            }
+        }
+        wave_sync_lds();  // the next chunk reuses this wave's slots
+    };
+
+    if constexpr (LDSX) {
+        XT *t = reinterpret_cast<XT *>(lds_tab);
+        const XT *__restrict__ src = reinterpret_cast<const XT *>(a.xhot);
+        for (int i = threadIdx.x; i < a.n_lds; i += blockDim.x) t[i] = src[i];
+        __syncthreads();
+        const int lb = xcd_remap(blockIdx.x, gridDim.x);
+        const int c_end = (int)((long long)a.nchunks * (lb + 1) / gridDim.x);
+        for (int c = (int)((long long)a.nchunks * lb / gridDim.x) + wv; c < c_end; c += WPB) run_chunk(c);
+    } else {
+        const int c = xcd_remap(blockIdx.x, gridDim.x) * WPB + wv;
+        if (c < a.nchunks) run_chunk(c);
     }
 }
 
@@ -403,12 +431,31 @@ hipError_t launch_hot_gather(const void *x, const int *hot_cols, void *xhot, int
 }
 
 // --------------------------------------------------------------- launchers
-template <typename VT, bool COL16>
-static void launch_mixed(const SpmvArgs &a, dim3 grid, dim3 block, hipStream_t st) {
-    if (a.recompute)
-        hipLaunchKernelGGL((spmv_kernel<VT, float, float, true, COL16, true>), grid, block, 0, st, a);
+template <typename VT, typename XT, typename YT, bool COMP, bool KAPPA>
+static void launch_one(const SpmvArgs &a, dim3 grid, hipStream_t st) {
+    if (a.n_lds > 0)
+        hipLaunchKernelGGL((spmv_kernel<VT, XT, YT, COMP, false, KAPPA, true>), dim3(a.lds_blocks),
+                           dim3(64 * kWavesLds), (size_t)a.n_lds * sizeof(XT), st, a);
     else
-        hipLaunchKernelGGL((spmv_kernel<VT, float, float, true, COL16, false>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((spmv_kernel<VT, XT, YT, COMP, false, KAPPA, false>), grid, dim3(64 * kWavesPerBlock), 0,
+                           st, a);
+}
+
+template <typename VT, bool COL16>
+static void launch_mixed(const SpmvArgs &a, dim3 grid, hipStream_t st) {
+    if constexpr (COL16) {
+        if (a.recompute)
+            hipLaunchKernelGGL((spmv_kernel<VT, float, float, true, true, true, false>), grid,
+                               dim3(64 * kWavesPerBlock), 0, st, a);
+        else
+            hipLaunchKernelGGL((spmv_kernel<VT, float, float, true, true, false, false>), grid,
+                               dim3(64 * kWavesPerBlock), 0, st, a);
+    } else {
+        if (a.recompute)
+            launch_one<VT, float, float, true, true>(a, grid, st);
+        else
+            launch_one<VT, float, float, true, false>(a, grid, st);
+    }
 }
 
 hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st) {
@@ -421,21 +468,66 @@ hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st) {
         }
         return hipGetLastError();
     }
-    dim3 grid((a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
+    if (a.n_lds > 0 && (col16 || a.lds_blocks <= 0)) return hipErrorInvalidValue;
+    dim3 grid((a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock);
     if (prec == 0) {
-        hipLaunchKernelGGL((spmv_kernel<double, double, double, false, false, false>), grid, block, 0, st, a);
+        launch_one<double, double, double, false, false>(a, grid, st);
     } else if (prec == 1) {
         if (col16)
-            launch_mixed<float, true>(a, grid, block, st);
+            launch_mixed<float, true>(a, grid, st);
         else
-            launch_mixed<float, false>(a, grid, block, st);
+            launch_mixed<float, false>(a, grid, st);
     } else {
         if (col16)
-            launch_mixed<unsigned short, true>(a, grid, block, st);
+            launch_mixed<unsigned short, true>(a, grid, st);
         else
-            launch_mixed<unsigned short, false>(a, grid, block, st);
+            launch_mixed<unsigned short, false>(a, grid, st);
     }
     return hipGetLastError();
+}
+
+template <typename VT, typename XT, bool COMP, bool KAPPA>
+static const void *lds_kernel() {
+    return reinterpret_cast<const void *>(&spmv_kernel<VT, XT, XT, COMP, false, KAPPA, true>);
+}
+
+hipError_t plan_spmv_lds(int prec, int recompute, int n_hot, long long max_bytes, int nchunks, int *n_lds,
+                         int *lds_blocks) {
+    *n_lds = 0;
+    *lds_blocks = 0;
+    if (n_hot <= 0 || max_bytes == 0 || nchunks <= 0) return hipSuccess;
+    const void *fn = prec == 0   ? lds_kernel<double, double, false, false>()
+                     : prec == 1 ? (recompute ? lds_kernel<float, float, true, true>()
+                                              : lds_kernel<float, float, true, false>())
+                                 : (recompute ? lds_kernel<unsigned short, float, true, true>()
+                                              : lds_kernel<unsigned short, float, true, false>());
+    const int xsize = prec == 0 ? 8 : 4;
+    hipError_t e;
+    int dev = 0, max_lds = 0, ncu = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    hipFuncAttributes fa{};
+    if ((e = hipFuncGetAttributes(&fa, fn)) != hipSuccess) return e;
+    long long bytes = (long long)max_lds - (long long)fa.sharedSizeBytes;
+    if (max_bytes > 0 && max_bytes < bytes) bytes = max_bytes;
+    if ((long long)n_hot * xsize < bytes) bytes = (long long)n_hot * xsize;
+    const int n = (int)(bytes / xsize);
+    // when sized automatically, a table this small does not repay its load
+    if (n < (max_bytes < 0 ? 1024 : 1)) return hipSuccess;
+    if ((e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, n * xsize)) != hipSuccess) return e;
+    int per_cu = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * kWavesLds, (size_t)n * xsize)) !=
+        hipSuccess)
+        return e;
+    if (per_cu < 1) return hipSuccess;
+    // at least one workgroup per CU while there are chunks for it
+    long long want = (nchunks + kWavesLds - 1) / kWavesLds;
+    if (want < ncu) want = nchunks < ncu ? nchunks : ncu;
+    const long long cap = (long long)per_cu * ncu;
+    *lds_blocks = (int)(want < cap ? want : cap);
+    *n_lds = n;
+    return hipSuccess;
 }
 
 hipError_t launch_finish(const FinishArgs &a, hipStream_t st) {

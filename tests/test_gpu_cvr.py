@@ -63,15 +63,21 @@ def test_gpu_layout_equals_emulation(P, O, mats, S):
         assert not reconstruct(A, gpu, S)
 
 
-@pytest.mark.parametrize("hot_kb", [1, 64, 2048])
-def test_hot_x_staging_decodes_and_is_bit_identical(P, O, mats, hot_kb):
+@pytest.mark.parametrize("hot_kb,lds_kb", [(1, 0), (64, 1), (2048, -1), (2048, 0)])
+def test_hot_x_staging_decodes_and_is_bit_identical(P, O, mats, hot_kb, lds_kb):
     for name, A in mats.items():
+        _say(f"hot-x {name} hot_kb={hot_kb} lds_kb={lds_kb}")
         S = 64
         x = P.fill_uniform(A.numCols, 11)
-        with P.CvrHip(A, steps_per_chunk=S, x_hot_kb=hot_kb) as h:
+        with P.CvrHip(A, steps_per_chunk=S, x_hot_kb=hot_kb, x_lds_kb=lds_kb) as h:
             lay = h.export_layout()
             hot = h.hot_cols()
             y = h.spmv_host(x)
+            n_lds = h.info()["n_lds_cols"]
+        if lds_kb == 0 or len(hot) == 0:
+            assert n_lds == 0
+        elif lds_kb > 0:
+            assert n_lds == min(len(hot), lds_kb * 1024 // 8)
         emu = O.cvr64_build(A, S, 0)
         cols = lay["cols"].astype(np.int64)
         is_hot = (cols & 0x40000000) != 0
@@ -81,10 +87,38 @@ def test_hot_x_staging_decodes_and_is_bit_identical(P, O, mats, hot_kb):
         assert np.array_equal(cols < 0, emu["cols"] < 0), name
         assert len(hot) <= hot_kb * 1024 // 8
         if len(hot):
-            assert np.all(np.diff(hot) > 0)  # hot numbers follow column order
             counts = np.bincount(A.cols, minlength=A.numCols)
+            # hot numbers follow use count (descending), ties by column
+            key = np.stack([-counts[hot], hot])
+            assert np.all((np.diff(key[0]) > 0) | ((np.diff(key[0]) == 0) & (np.diff(key[1]) > 0)))
             assert counts[hot].min() >= 2
+            # the hot set is the top of the ranking: nothing left out beats its last entry
+            rest = np.setdiff1d(np.nonzero(counts >= 2)[0], hot)
+            if len(rest):
+                last = (counts[hot[-1]], hot[-1])
+                assert not np.any((counts[rest] > last[0]) | ((counts[rest] == last[0]) & (rest < last[1])))
         assert np.array_equal(y, O.cvr64_spmv(A, emu, x)), name
+
+
+@pytest.mark.parametrize("prec,recompute", [(0, False), (1, False), (1, True), (2, False)])
+def test_lds_hot_table_bit_identical(P, O, prec, recompute):
+    """The LDS hot-x variant (16-wave workgroups walking chunk ranges) gives
+    the same y, bit for bit, as the plain kernel on a power-law matrix."""
+    A = P.rmat(16, 16, seed=3)
+    x = P.fill_uniform(A.numCols, 9)
+    _say(f"lds-hot prec={prec} recompute={recompute} nnz={A.nnz}")
+    kw = dict(steps_per_chunk=64, precision=prec, recompute=recompute, keep_fp64=recompute)
+    with P.CvrHip(A, x_hot_kb=256, **kw) as h, P.CvrHip(A, x_hot_kb=0, **kw) as h0:
+        assert h.info()["n_lds_cols"] >= 1024
+        y, y0 = h.spmv_host(x), h0.spmv_host(x)
+        if recompute:
+            assert h.last_recompute_count() == h0.last_recompute_count()
+        # the chunk queues rearm themselves: repeated launches cover every chunk
+        for _ in range(3):
+            assert np.array_equal(h.spmv_host(x), y)
+    assert np.array_equal(y, y0)
+    if prec == 0:
+        assert np.array_equal(y, O.cvr64_spmv(A, O.cvr64_build(A, 64, 0), x))
 
 
 @pytest.mark.parametrize("S", [8, 64, 256])
