@@ -348,11 +348,10 @@ __global__ void zero_kernel(void *y, int n) {
 }
 
 // Mixed precision, after the SpMV: one wave per chunk (1) closes the chunk's
-// cross-chunk row (as fixup_kernel) and flags it when its cancellation ratio
-// exceeds kappa, then (2) recomputes every flagged row of the chunk's row range
-// [chunk_rows[c], chunk_rows[c+1]) from the resident FP64 CSR and FP64 x:
-// lanes stride the row with FMA, a fixed xor-butterfly adds the lane partials
-// (deterministic), the flag is cleared for the next SpMV.
+// cross-chunk row (as fixup_kernel) and recomputes it when its cancellation
+// ratio exceeds kappa, then (2) recomputes every flagged row of an even 1/nchunks
+// share of the rows from the resident FP64 CSR and FP64 x, and clears the flags
+// for the next SpMV. Every reduction has a fixed order (deterministic).
 __device__ __forceinline__ double wave_sum_f64(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
@@ -364,7 +363,11 @@ __device__ __forceinline__ double row_fp64(const FinishArgs &a, int row, int lan
     return wave_sum_f64(s);
 }
 
+constexpr int kFlagWindow = 1024;  // rows whose flags one wave lists at a time
+constexpr int kLongRow = 128;      // longer flagged rows are recomputed by the whole wave
+
 __global__ __launch_bounds__(256) void finish_kernel(FinishArgs a) {
+    __shared__ int flag_list[kWavesPerBlock][kFlagWindow];
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     if (c >= a.nchunks) return;
@@ -383,30 +386,63 @@ __global__ __launch_bounds__(256) void finish_kernel(FinishArgs a) {
         if (lane == 0) a.y[frow] = (float)s;
     }
     if (a.x64) {
-        // flagged rows: any wave may recompute any row; rows are split evenly
+        // flagged rows: any wave may recompute any row; rows are split evenly.
+        // Each window's flagged rows are first listed in LDS, then recomputed
+        // side by side: short rows one per lane (sequential FMA in element
+        // order, loads issued 8 elements at a time), long rows by the whole
+        // wave (fixed xor-butterfly reduction).
+        const int wv = threadIdx.x >> 6;
+        int *flist = flag_list[wv];
         const int r0 = (int)((long long)a.M * c / a.nchunks), r1 = (int)((long long)a.M * (c + 1) / a.nchunks);
-        for (int rb = r0; rb < r1; rb += 64) {
-            const int r = rb + lane;
-            const bool f = (r < r1) && a.rflag[r];
-            count += __popcll(__ballot(f));
-            // short rows: one lane each, sequential FMA in element order
-            const int len = f ? a.rp[r + 1] - a.rp[r] : 0;
-            const bool mine = f && len <= 128;
-            if (mine) {
+        for (int wb = r0; wb < r1; wb += kFlagWindow) {
+            const int we = wb + kFlagWindow < r1 ? wb + kFlagWindow : r1;
+            int nf = 0;
+            for (int rb = wb; rb < we; rb += 64) {
+                const int r = rb + lane;
+                const bool f = (r < we) && a.rflag[r];
+                const unsigned long long m = __ballot(f);
+                if (f) {
+                    flist[nf + mbcnt64s(m)] = r;
+                    a.rflag[r] = 0;
+                }
+                nf += __popcll(m);
+            }
+            count += nf;
+            wave_sync_lds();
+            for (int k = lane; k < nf; k += 64) {
+                const int r = flist[k];
+                const int b = a.rp[r], e = a.rp[r + 1];
+                if (e - b > kLongRow) continue;
                 double s = 0.0;
-                for (int j = a.rp[r]; j < a.rp[r + 1]; ++j) s = __builtin_fma(a.va[j], a.x64[a.ci[j]], s);
+                for (int j0 = b; j0 < e; j0 += 8) {
+                    int cc[8];
+                    double vv[8], xx[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const bool in = j0 + u < e;
+                        cc[u] = in ? a.ci[j0 + u] : 0;
+                        vv[u] = in ? a.va[j0 + u] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) xx[u] = a.x64[cc[u]];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (j0 + u < e) s = __builtin_fma(vv[u], xx[u], s);
+                }
                 a.y[r] = (float)s;
             }
-            if (f) a.rflag[r] = 0;
-            // long rows: the whole wave, fixed xor-butterfly reduction
-            unsigned long long m = __ballot(f && !mine);
-            while (m) {
-                const int j = __ffsll((long long)m) - 1;
-                const int row = rb + j;
-                const double s = row_fp64(a, row, lane);
-                if (lane == 0) a.y[row] = (float)s;
-                m &= m - 1;
+            for (int k0 = 0; k0 < nf; k0 += 64) {
+                const int k = k0 + lane;
+                const bool lg = k < nf && a.rp[flist[k] + 1] - a.rp[flist[k]] > kLongRow;
+                unsigned long long m = __ballot(lg);
+                while (m) {
+                    const int row = flist[k0 + __ffsll((long long)m) - 1];
+                    const double s = row_fp64(a, row, lane);
+                    if (lane == 0) a.y[row] = (float)s;
+                    m &= m - 1;
+                }
             }
+            wave_sync_lds();  // the next window overwrites the list
         }
     }
     if (lane == 0) a.rc_count[c] = count;

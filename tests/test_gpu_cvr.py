@@ -249,6 +249,14 @@ def test_fp32_recompute_meets_1e5_on_cancelling_rows(P, O):
     _say("kappa max", kappa.max(), "rows > 64:", int((kappa > 64).sum()), "recomputed", n1,
          "max rel plain", rel_plain.max(), "with recompute", rel_rc.max())
     assert rel_rc.max() <= 1e-5
+    # rows certainly flagged and recomputed one per lane (not rows spanning
+    # chunks) follow the serial FMA order: the FP32 rounding of the oracle
+    emu = O.cvr64_build(A, 64, 1)
+    fr = emu["fix"][0::2]
+    fixrows = set(fr[fr >= 0].tolist())
+    sure = np.array([r for r in np.nonzero(kappa > 1024)[0] if r not in fixrows])
+    assert sure.size > 0
+    assert np.array_equal(y_rc[sure], yref[sure].astype(np.float32).astype(np.float64))
     # rows with kappa well below the threshold are left to the FP32 path and its bound
     low = kappa < 32
     assert np.all(np.abs(y_rc[low] - yref[low]) <= 2.0 ** -22 * sabs[low] + 2.0 ** -23 * np.abs(yref[low]))
