@@ -3,8 +3,8 @@
 * ``CONFIGS`` -- the candidate space: format (cvr_hip / csr_hip), value
   precision (FP64; FP32 with FP64 recompute of rows whose cancellation ratio
   exceeds 64, which keeps every row within ~1e-5 of FP64; the same with 16-bit
-  chunk-relative columns) and tile (CVR steps per chunk S, or CSR lanes per
-  row L).
+  chunk-relative columns), tile (CVR steps per chunk S, or CSR lanes per row L)
+  and x staging (hot-x copy of the most used columns, its head held in LDS).
 * ``autotune`` -- exhaustive search: builds each candidate on the GPU, times
   it with hipEvents, returns per-config times (inf where a candidate does not
   apply, e.g. a chunk spanning more than 65536 columns for 16-bit columns).
@@ -32,16 +32,19 @@ class Config:
     fmt: str        # "cvr" | "csr"
     prec: str       # "f64" | "f32rc" | "f32rc16"
     tile: int       # S for cvr, L for csr
+    xs: int = 0     # x staging: hot-x copy size in KiB (0 = off)
 
     @property
     def name(self) -> str:
-        return f"{self.fmt}-{self.prec}-{self.tile}"
+        return f"{self.fmt}-{self.prec}-{self.tile}" + (f"-x{self.xs}" if self.xs else "")
 
 
 CONFIGS = ([Config("cvr", "f64", s) for s in (32, 128, 512)]
            + [Config("cvr", "f32rc", s) for s in (32, 128, 512)]
            + [Config("cvr", "f32rc16", s) for s in (32, 128, 512)]
-           + [Config("csr", "f64", l) for l in (1, 4, 16, 64)])
+           + [Config("csr", "f64", l) for l in (1, 4, 16, 64)]
+           + [Config("cvr", "f64", s, 2048) for s in (128, 512)]
+           + [Config("cvr", "f32rc", 128, 2048)])
 DEFAULT = Config("cvr", "f64", 128)  # the fallback without a model
 
 
@@ -72,9 +75,9 @@ def build(A: CSR, cfg: Config):
     if cfg.fmt == "csr":
         return CsrHip(A, lanes_per_row=cfg.tile, precision=PREC_F64)
     if cfg.prec == "f64":
-        return CvrHip(A, steps_per_chunk=cfg.tile)
+        return CvrHip(A, steps_per_chunk=cfg.tile, x_hot_kb=cfg.xs)
     return CvrHip(A, steps_per_chunk=cfg.tile, precision=PREC_F32, recompute=True, kappa_threshold=64.0,
-                  col_index_bits=16 if cfg.prec == "f32rc16" else 32)
+                  col_index_bits=16 if cfg.prec == "f32rc16" else 32, x_hot_kb=cfg.xs)
 
 
 def autotune(A: CSR, x: np.ndarray, iters: int = 5, configs=CONFIGS) -> np.ndarray:
@@ -207,12 +210,12 @@ def slowdowns(times: np.ndarray, choice: np.ndarray) -> np.ndarray:
     return t[np.arange(t.shape[0]), choice] / best
 
 
-def gen_params(rng: np.random.Generator, max_rows: int = 1_000_000) -> dict:
+def gen_params(rng: np.random.Generator, max_rows: int = 1_000_000, min_rows: int = 10_000) -> dict:
     """One random member of the selector's synthetic family: rows log-uniform in
-    [1e4, max_rows], row lengths constant / Poisson / lognormal (sigma
+    [min_rows, max_rows], row lengths constant / Poisson / lognormal (sigma
     0.5-2) / power law (alpha 1.8-2.5), columns banded / block-diagonal (block
     32-512) / uniform."""
-    rows = int(math.exp(rng.uniform(math.log(1e4), math.log(max_rows))))
+    rows = int(math.exp(rng.uniform(math.log(min_rows), math.log(max_rows))))
     dist = int(rng.integers(0, 4))
     mean = float(math.exp(rng.uniform(math.log(4), math.log(64))))
     shape = 1.0

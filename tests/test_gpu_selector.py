@@ -42,7 +42,7 @@ def test_mlp_training_reduces_loss():
     W = rng.normal(size=(16, 13))
     T = np.tanh(X @ W / 4).astype(np.float32)
     M = np.ones_like(T)
-    m = S.Mlp(seed=1)
+    m = S.Mlp(n_out=13, seed=1)
     hist = m.train(X, T, M, epochs=60, batch=64, lr=3e-3)
     _say("loss", hist[0], "->", hist[-1])
     assert hist[-1] < 0.25 * hist[0]
@@ -75,3 +75,27 @@ def test_learned_selector_beats_fixed_default():
          "loss", hist[0], "->", hist[-1])
     assert np.all(np.isfinite(sd_model))
     assert gm(sd_model) <= gm(sd_default)
+
+
+def test_trained_selector_gain_on_committed_dataset():
+    """The MLP trained on the GPU on the committed autotune dataset (2,000
+    matrices) picks configs whose measured times on the 200 held-out matrices
+    beat the fixed default and come close to the exhaustive best."""
+    from pathlib import Path
+    from pava_amd import selector as S
+    from pava_amd import selector_data as D
+    d = np.load(Path(S.__file__).parent / "data" / "selector_dataset.npz", allow_pickle=False)
+    idx, F, T = d["index"], d["features"], d["ms"]
+    tr = idx < D.N_TRAIN
+    ho = (idx >= D.N_TRAIN) & (idx < D.N_TRAIN + D.N_HELD)
+    sel, hist = S.Selector.fit(F[tr], T[tr], epochs=200, seed=0)
+    choice = sel.choose(F[ho], valid=np.isfinite(T[ho]))
+    sel.mlp.close()
+    gm = lambda v: float(np.exp(np.mean(np.log(v))))
+    sd_model = S.slowdowns(T[ho], choice)
+    sd_default = S.slowdowns(T[ho], np.full(int(ho.sum()), S.CONFIGS.index(S.DEFAULT)))
+    _say("held-out geo-mean slowdown vs exhaustive best: model", gm(sd_model), "default", gm(sd_default),
+         "loss", hist[0], "->", hist[-1])
+    assert hist[-1] < 0.5 * hist[0]
+    assert gm(sd_model) < 0.6 * gm(sd_default)
+    assert gm(sd_model) < 1.25

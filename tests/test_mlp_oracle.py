@@ -58,5 +58,28 @@ def test_features_closed_forms(P):
 def test_config_space():
     from pava_amd import selector as S
     names = [c.name for c in S.CONFIGS]
-    assert len(names) == len(set(names)) == 13
+    assert len(names) == len(set(names)) == 16
     assert S.DEFAULT in S.CONFIGS
+    # every axis the autotuner searches is present: format, precision, tile, x staging
+    assert {c.fmt for c in S.CONFIGS} == {"cvr", "csr"}
+    assert {c.prec for c in S.CONFIGS} == {"f64", "f32rc", "f32rc16"}
+    assert {c.xs for c in S.CONFIGS} == {0, 2048}
+    assert len({c.tile for c in S.CONFIGS if c.fmt == "cvr"}) >= 3
+
+
+def test_committed_selector_dataset_is_consistent():
+    """The autotune dataset matches the config space and the seeded family:
+    every index once, features recomputed on the CPU for a few matrices agree."""
+    from pathlib import Path
+    from pava_amd import selector as S
+    from pava_amd import selector_data as D
+    d = np.load(Path(S.__file__).parent / "data" / "selector_dataset.npz", allow_pickle=False)
+    assert [str(n) for n in d["configs"]] == [c.name for c in S.CONFIGS]
+    n = D.N_TRAIN + D.N_HELD + D.N_LARGE
+    assert np.array_equal(d["index"], np.arange(n))
+    assert d["features"].shape == (n, S.N_FEATURES) and d["ms"].shape == (n, len(S.CONFIGS))
+    assert np.all(np.isfinite(d["ms"]).sum(1) >= 12) and np.all(d["ms"][np.isfinite(d["ms"])] > 0)
+    for i in (0, 7, 2001):
+        A = D.matrix(i)
+        assert A.numRows == d["rows"][i] and A.nnz == d["nnz"][i]
+        assert np.allclose(S.features(A), d["features"][i], rtol=1e-5, atol=1e-6)
