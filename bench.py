@@ -108,7 +108,7 @@ def cpu_baseline(O, A, x, budget_s: float = 10.0):
     }
 
 
-def run_precision(P, A, x, steps: int, S: int = 256) -> dict:
+def run_precision(P, A, x, steps: int, S: int = 0) -> dict:
     """FP64 vs FP32 / BF16 storage (FP32 compensated accumulation), with and
     without FP64 recompute of rows whose cancellation ratio exceeds 64, and with
     16-bit chunk-relative columns; per-row error vs the FP64 reference SpMV."""
@@ -173,7 +173,7 @@ def run_batch(P, k: int, nmat: int, steps: int, S: int = 64, check: int = 4) -> 
         A.vals = A.vals.astype(np.float32).astype(np.float64)
         mats.append(A)
         hs.append(P.CvrHip(A, steps_per_chunk=S, precision=P.PREC_F32))
-        h64.append(P.CvrHip(A, steps_per_chunk=256))
+        h64.append(P.CvrHip(A))
         if i % 64 == 63:
             print(f"[bench] batch: {i + 1}/{nmat} matrices built ({time.perf_counter() - t0:.0f}s)", file=sys.stderr,
                   flush=True)
@@ -236,7 +236,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="rmat22_fp64", choices=sorted(WORKLOADS))
-    ap.add_argument("--steps-per-chunk", type=int, default=0, help="CVR chunk steps (0 = default)")
+    ap.add_argument("--steps-per-chunk", type=int, default=0, help="CVR chunk steps (0 = automatic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--variant", choices=["none", "precision", "batch"], default="none",
@@ -276,7 +276,7 @@ def main():
     if args.variant == "precision":
         A = make_matrix(P, w)
         x = P.fill_uniform(A.numCols, 0)
-        res = run_precision(P, A, x, args.steps, args.steps_per_chunk or 256)
+        res = run_precision(P, A, x, args.steps, args.steps_per_chunk)
         print(json.dumps({"variant": "precision", "workload": args.workload, "rows": A.numRows, "nnz": A.nnz,
                           "results": res}), flush=True)
         return
@@ -287,9 +287,10 @@ def main():
     print(f"[bench] {args.workload}: rows={A.numRows} nnz={A.nnz} max_row={int(np.diff(A.rowOffsets).max())} "
           f"gen {t_gen:.1f}s", file=sys.stderr, flush=True)
 
-    S = args.steps_per_chunk or 256
-    h = P.CvrHip(A, steps_per_chunk=S, precision=w["prec"], x_hot_kb=args.x_hot_kb, x_lds_kb=args.x_lds_kb)
+    h = P.CvrHip(A, steps_per_chunk=args.steps_per_chunk, precision=w["prec"], x_hot_kb=args.x_hot_kb,
+                 x_lds_kb=args.x_lds_kb)
     info = h.info()
+    S = info["steps_per_chunk"]
     dev = torch.device("cuda:0")
     dx = torch.from_numpy(x).to(dev)
     dy = torch.empty(A.numRows, dtype=torch.float64, device=dev)

@@ -74,7 +74,7 @@ static int dev_alloc(cvr_hip_handle *h, void **p, size_t bytes) {
 
 extern "C" void cvr_hip_default_opts(cvr_hip_opts *o) {
     memset(o, 0, sizeof *o);
-    o->steps_per_chunk = 256;
+    o->steps_per_chunk = 0;  // automatic
     o->precision = CVR_HIP_PREC_F64;
     o->col_index_bits = 32;
     o->recompute = 0;
@@ -98,7 +98,8 @@ extern "C" int cvr_hip_destroy(cvr_hip_handle *h) {
 }
 
 static int validate_opts(const cvr_hip_opts *o) {
-    if (o->steps_per_chunk < 8 || o->steps_per_chunk > 4096 || o->steps_per_chunk % cvr::kStepGroup)
+    if (o->steps_per_chunk != 0 &&
+        (o->steps_per_chunk < 8 || o->steps_per_chunk > 4096 || o->steps_per_chunk % cvr::kStepGroup))
         return set_error(PAVA_HIP_EINVAL, "steps_per_chunk must be a multiple of 8 in [8, 4096], got %d",
                          o->steps_per_chunk);
     if (o->precision < 0 || o->precision > 2) return set_error(PAVA_HIP_EINVAL, "bad precision %d", o->precision);
@@ -243,14 +244,28 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     return 0;
 }
 
-static int begin_handle(const cvr_hip_opts *opts, cvr_hip_opts *o, cvr_hip_handle **out, cvr_hip_handle **hp) {
+// Automatic chunk size: about two chunks per resident wave slot of the device
+// (16 waves per CU), at most 256 steps. One chunk per wave runs start to end,
+// so a chunk count just past a whole round of slots leaves a tail as long as
+// a chunk; two rounds keep the tail short and the chip full.
+static int auto_steps(long long nnz, int device) {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
+        ncu = 256;
+    const long long per = 64LL * 2 * 16 * ncu;  // elements per chunk-step over two rounds
+    long long s = (nnz + per - 1) / per;
+    s = (s + cvr::kStepGroup - 1) / cvr::kStepGroup * cvr::kStepGroup;
+    return (int)std::min<long long>(std::max<long long>(s, cvr::kStepGroup), 256);
+}
+
+static int begin_handle(const cvr_hip_opts *opts, long long nnz, cvr_hip_opts *o, cvr_hip_handle **out,
+                        cvr_hip_handle **hp) {
     if (!out) return set_error(PAVA_HIP_EINVAL, "null output handle");
     *out = nullptr;
     if (opts)
         *o = *opts;
     else
         cvr_hip_default_opts(o);
-    if (o->steps_per_chunk == 0) o->steps_per_chunk = 256;
     if (o->kappa_threshold <= 0) o->kappa_threshold = 64.0;
     int rc = validate_opts(o);
     if (rc) return rc;
@@ -258,6 +273,7 @@ static int begin_handle(const cvr_hip_opts *opts, cvr_hip_opts *o, cvr_hip_handl
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(PAVA_HIP_EDEVICE, "no HIP device");
     if (o->device < 0 || o->device >= ndev) return set_error(PAVA_HIP_EINVAL, "device %d out of range", o->device);
     HIP_TRY(hipSetDevice(o->device));
+    if (o->steps_per_chunk == 0) o->steps_per_chunk = auto_steps(nnz, o->device);
     cvr_hip_handle *h = new cvr_hip_handle();
     h->device = o->device;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -273,7 +289,7 @@ extern "C" int cvr_hip_create(const pava_csr_matrix *A, const cvr_hip_opts *opts
     if (rc) return rc;
     cvr_hip_opts o;
     cvr_hip_handle *h = nullptr;
-    if ((rc = begin_handle(opts, &o, out, &h))) return rc;
+    if ((rc = begin_handle(opts, A->nnz, &o, out, &h))) return rc;
     const int M = A->numRows, nnz = A->nnz;
     int *d_rp = nullptr, *d_ci = nullptr;
     double *d_va = nullptr;
@@ -321,7 +337,7 @@ extern "C" int cvr_hip_create_device(int numRows, int numCols, int nnz, const in
     if (!d_rowOffsets || (nnz > 0 && (!d_cols || !d_vals))) return set_error(PAVA_HIP_EINVAL, "null device CSR");
     cvr_hip_opts o;
     cvr_hip_handle *h = nullptr;
-    int rc = begin_handle(opts, &o, out, &h);
+    int rc = begin_handle(opts, nnz, &o, out, &h);
     if (rc) return rc;
     if (o.recompute || o.keep_fp64) {
         // keep a private copy of the FP64 CSR for the recompute pass

@@ -102,7 +102,7 @@ def test_hot_x_staging_decodes_and_is_bit_identical(P, O, mats, hot_kb, lds_kb):
 
 @pytest.mark.parametrize("prec,recompute", [(0, False), (1, False), (1, True), (2, False)])
 def test_lds_hot_table_bit_identical(P, O, prec, recompute):
-    """The LDS hot-x variant (16-wave workgroups walking chunk ranges) gives
+    """The LDS hot-x variant (one workgroup per CU walking a chunk range) gives
     the same y, bit for bit, as the plain kernel on a power-law matrix."""
     A = P.rmat(16, 16, seed=3)
     x = P.fill_uniform(A.numCols, 9)
@@ -113,7 +113,7 @@ def test_lds_hot_table_bit_identical(P, O, prec, recompute):
         y, y0 = h.spmv_host(x), h0.spmv_host(x)
         if recompute:
             assert h.last_recompute_count() == h0.last_recompute_count()
-        # the chunk queues rearm themselves: repeated launches cover every chunk
+        # repeated launches give the same y
         for _ in range(3):
             assert np.array_equal(h.spmv_host(x), y)
     assert np.array_equal(y, y0)
@@ -206,6 +206,22 @@ def test_low_precision_bit_exact_vs_emulation(P, O, mats, prec):
         assert np.array_equal(gpu["vals"], emu["vals"]), name
         ye = O.cvr64_spmv(A, emu, x.astype(np.float32)).astype(np.float64)
         assert np.array_equal(y, ye), name
+
+
+def test_automatic_chunk_size(P, O):
+    """steps_per_chunk = 0: about two chunks per resident wave slot (16 per CU),
+    a multiple of 8 in [8, 256]; the layout is the emulation's at that size."""
+    import torch
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    for A in (P.stencil27(64, 0.01, 0), P.laplacian2d(40, 40)):
+        with P.CvrHip(A) as h:
+            S = h.info()["steps_per_chunk"]
+            x = P.fill_uniform(A.numCols, 1)
+            y = h.spmv_host(x)
+        steps = -(-A.nnz // (64 * 2 * 16 * ncu))  # ceil: two chunks per wave slot
+        want = min(max(-(-steps // 8) * 8, 8), 256)
+        assert S == want, (S, want)
+        assert np.array_equal(y, O.cvr64_spmv(A, O.cvr64_build(A, S, 0), x))
 
 
 def test_invalid_options_rejected(P):
