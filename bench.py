@@ -83,6 +83,19 @@ def exact_rows(A, x, rows) -> np.ndarray:
     return np.array(out)
 
 
+def measured_traffic(workload: str, kernel: str, S: int, x_hot_kb: int):
+    """HBM bytes per launch of the SpMV kernel from the committed rocprofv3
+    counter passes (FETCH_SIZE + WRITE_SIZE, calibrated against a plain
+    copy), when this run matches the measured configuration; else None."""
+    f = ROOT / "profiles" / "traffic_per_launch.json"
+    if not f.exists():
+        return None
+    m = json.loads(f.read_text()).get(workload)
+    if not m or m["kernel"] != kernel or m["steps_per_chunk"] != S or m["x_hot_kb"] != x_hot_kb:
+        return None
+    return m["traffic_bytes"]
+
+
 def cpu_baseline(O, A, x, budget_s: float = 10.0):
     """Oracle CSR SpMV (FP64, OpenMP over rows) on the host: a bounded sample
     of repeated full SpMVs until ~budget_s of wall time."""
@@ -314,6 +327,7 @@ def main():
 
     B = algorithmic_bytes(A.numRows, A.numCols, A.nnz)
     value = B / (ms_step * 1e-3) / 1e9
+    kernel = "spmv_kernel<double,double,double,false,false,false,%s>" % ("true" if info["n_lds_cols"] else "false")
     achieved = B / (ms_main * 1e-3) / 1e9
 
     result = {
@@ -340,11 +354,13 @@ def main():
         "convert_ms": round(info["convert_ms"], 3),
         "hot_setup_ms": round(info["hot_setup_ms"], 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "spmv_kernel<double,double,double,false,false,false,%s>"
-                     % ("true" if info["n_lds_cols"] else "false"),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
                      "kernel_ms": round(ms_main, 5)},
     }
+    traffic = measured_traffic(args.workload, kernel, S, args.x_hot_kb)
+    if traffic is not None:
+        result["roofline"]["traffic"] = traffic
+        result["roofline"]["traffic_GB_s_at_this_kernel_time"] = round(traffic / (ms_main * 1e-3) / 1e9, 1)
 
     if not args.no_check:
         # FP64 reference: the driver's verification SpMV (pava's referenceSpMV,
