@@ -271,7 +271,8 @@ def reference_spmv(A: CSR, x: np.ndarray) -> np.ndarray:
     """The driver's verification SpMV (main.cpp:318), not the hot path."""
     y = np.empty(A.numRows)
     s = A.c_struct()
-    lib().pava_reference_spmv(C.byref(s), _ptr(np.ascontiguousarray(x, np.float64)), _ptr(y))
+    xx = np.ascontiguousarray(x, np.float64)  # kept alive across the call
+    lib().pava_reference_spmv(C.byref(s), _ptr(xx), _ptr(y))
     return y
 
 

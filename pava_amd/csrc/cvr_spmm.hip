@@ -2,19 +2,22 @@
 // for gfx950, single matrix or a persistent batched launch.
 //
 // Same CVR-64 layout and schedule as the SpMV (cvr_spmv.hip); each nonzero
-// scales one row of X (k contiguous FP32 values, fetched with 16-byte vector
-// loads) into the lane's k accumulators. X and Y are FP32, row-major
+// scales one row of X (k contiguous FP32 values) into k accumulators. For
+// k <= 16 the accumulators are held 4 per thread by the KP/4 consecutive
+// threads that serve a CVR lane (a row of X is one coalesced request, a chunk
+// runs on KP/4 waves); for k > 16 one lane holds all 32 and loads its row with
+// 8 16-byte loads. X and Y are FP32, row-major
 // (X: numCols x k, Y: numRows x k). Accumulation is FP64: the product of two
 // FP32 numbers is exact in FP64 and the FP64 FMA issues at the same VALU rate
 // as the (non-packed) FP32 FMA on CDNA4, so every output is the FP64 sum of
 // exact products rounded once to FP32 -- no recompute pass is needed to stay
 // within 1e-5 of an FP64 evaluation on the same inputs.
 //
-// Batched path: many matrices' chunks form one work queue; a persistent grid
-// (sized to the resident capacity of the GPU) pops chunk indices with one
-// atomic per chunk until the queue is empty. The queue counter is reset by the
-// fixup kernel that follows, so the pair can be captured in a hipGraph and
-// replayed with no memset node.
+// Batched path: many matrices' chunks form one work queue, cut into one range
+// per XCD; a persistent grid (sized to the resident capacity of the GPU) pops
+// chunk indices with one atomic per chunk until the queue is empty. The queue
+// counters are reset by the fixup kernel that follows, so the pair can be
+// captured in a hipGraph and replayed with no memset node.
 
 #include <hip/hip_runtime.h>
 
@@ -32,6 +35,8 @@ __device__ __forceinline__ void wave_sync_lds_m() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+constexpr int kBatchQueues = 8;  // queue ranges of the batched launch (one per XCD); counters in use
+
 template <typename VT>
 __device__ __forceinline__ double val_f64(VT v);
 template <>
@@ -41,6 +46,243 @@ __device__ __forceinline__ double val_f64<float>(float v) { return (double)v; }
 template <>
 __device__ __forceinline__ double val_f64<unsigned short>(unsigned short v) {
     return (double)__uint_as_float((unsigned)v << 16);
+}
+
+// One chunk of one matrix, processed by the KP/4 waves of a workgroup. Each
+// CVR lane is served by KP/4 consecutive threads, thread `part` owning columns
+// [4 part, 4 part + 4) of the right-hand sides: the X row of a nonzero is read
+// as KP/4 consecutive 16-byte pieces by consecutive threads (one request per
+// row), and the 4 accumulators of a thread are FP64. Every wave reads the
+// whole step's columns (the end flags of all 64 CVR lanes give each ending
+// segment its rank in the chunk's segment list) and a private copy of each
+// 8-step group's segment entries, staged in LDS one group ahead as in the
+// SpMV. `slots` (max_slots x KP doubles) is shared by the workgroup.
+template <typename VT, int KP>
+__device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const int lane, double *slots,
+                           int *segbuf) {
+    constexpr int TPL = KP / 4;   // threads per CVR lane
+    constexpr int VL = 64 / TPL;  // CVR lanes per wave
+    constexpr int G = kStepGroup;
+    const int part = lane % TPL;
+    const int cl = wv * VL + lane / TPL;  // this thread's CVR lane
+    const int j0 = 4 * part;
+    const long long C = 64LL * a.S;
+    const size_t base = (size_t)C * (size_t)c;
+    const int steps = (c == a.nchunks - 1) ? (int)((a.layout_elems - (long long)base) >> 6) : a.S;
+    const int zr = a.chunk_rows[c];
+    const int *__restrict__ segs = a.seg + zr + kSegExtra * c;
+    const VT *__restrict__ vp = reinterpret_cast<const VT *>(a.vals) + base + cl;
+    const int *__restrict__ cpa = a.cols + base + lane;  // the step's 64 columns, one per thread
+    const int k = a.k;
+    const bool full = (k == KP);
+
+    auto load_cols = [&](int g0, int (&cc)[G]) {
+#pragma unroll
+        for (int i = 0; i < G; ++i) cc[i] = __builtin_nontemporal_load(&cpa[(size_t)(g0 + i) * 64]);
+    };
+    // this thread's 4 values of the X row of stored column cv
+    auto xpart = [&](int cv) -> float4 {
+        int col = cv & kColMask;
+        if (cv & kHotFlag) col = a.hot_cols[col];
+        const float *p = a.X + (size_t)col * k + j0;
+        if (full) return *reinterpret_cast<const float4 *>(p);
+        float4 q;
+        q.x = j0 < k ? p[0] : 0.f;
+        q.y = j0 + 1 < k ? p[1] : 0.f;
+        q.z = j0 + 2 < k ? p[2] : 0.f;
+        q.w = j0 + 3 < k ? p[3] : 0.f;
+        return q;
+    };
+    auto group_ends = [&](const int (&cc)[G]) -> int {
+        int e = 0;
+#pragma unroll
+        for (int i = 0; i < G; ++i) e += __popcll(__ballot(cc[i] < 0));
+        return e;
+    };
+    auto load_segs = [&](int sb, int ne, int (&r)[G]) {
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (j * 64 < ne) r[j] = __builtin_nontemporal_load(&segs[sb + j * 64 + lane]);
+    };
+
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    VT vb[G], vn[G];
+    int cb[G], cn[G], cnn[G];
+    float4 xb[G], xn[G];
+    int sc[G], sn[G];
+
+    // prologue: group 0 issued, columns of group 1 in flight
+    load_cols(0, cb);
+#pragma unroll
+    for (int i = 0; i < G; ++i) vb[i] = __builtin_nontemporal_load(&vp[(size_t)i * 64]);
+#pragma unroll
+    for (int i = 0; i < G; ++i) xb[i] = xpart(__shfl(cb[i], cl));
+    int ecur = group_ends(cb);
+    load_segs(0, ecur, sc);
+    if (G < steps) load_cols(G, cn);
+    int sbase = 0, segp = 0;
+
+    for (int g = 0; g < steps; g += G) {
+        const bool has1 = (g + G) < steps, has2 = (g + 2 * G) < steps;
+        const int sbase_next = sbase + ecur;
+        int enext = 0;
+        if (has1) {
+#pragma unroll
+            for (int i = 0; i < G; ++i) xn[i] = xpart(__shfl(cn[i], cl));
+            enext = group_ends(cn);
+            load_segs(sbase_next, enext, sn);
+#pragma unroll
+            for (int i = 0; i < G; ++i) vn[i] = __builtin_nontemporal_load(&vp[(size_t)(g + G + i) * 64]);
+            if (has2) load_cols(g + 2 * G, cnn);
+        }
+        // this group's destinations -> this wave's LDS copy
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (j * 64 < ecur) segbuf[j * 64 + lane] = sc[j];
+        wave_sync_lds_m();
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+            const double v = val_f64<VT>(vb[i]);
+            acc[0] = __builtin_fma(v, (double)xb[i].x, acc[0]);
+            acc[1] = __builtin_fma(v, (double)xb[i].y, acc[1]);
+            acc[2] = __builtin_fma(v, (double)xb[i].z, acc[2]);
+            acc[3] = __builtin_fma(v, (double)xb[i].w, acc[3]);
+            const unsigned long long m = __ballot(cb[i] < 0);
+            if (m) {
+                if ((m >> cl) & 1ull) {
+                    const int rank = __popcll(m & ((1ull << cl) - 1ull));
+                    const int d = segbuf[segp - sbase + rank];
+                    if (d >= 0) {
+                        float *yr = a.Y + (size_t)d * k + j0;
+                        if (full) {
+                            *reinterpret_cast<float4 *>(yr) =
+                                make_float4((float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                if (j0 + j < k) yr[j] = (float)acc[j];
+                        }
+                    } else {
+                        double *sl = slots + (size_t)(~d) * KP + j0;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) sl[j] = acc[j];
+                    }
+                    acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
+                }
+                segp += __popcll(m);
+            }
+        }
+        if (has1) {
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+                vb[i] = vn[i];
+                cb[i] = cn[i];
+                xb[i] = xn[i];
+                cn[i] = cnn[i];
+                sc[i] = sn[i];
+            }
+            sbase = sbase_next;
+            ecur = enext;
+        }
+        wave_sync_lds_m();  // the next group overwrites segbuf
+    }
+
+    // split rows: (group, column) pairs over the workgroup, slots summed in element order
+    __syncthreads();
+    const int tid = threadIdx.x, nthreads = blockDim.x;
+    const int ns = a.split_n[c];
+    const int2 *tab = a.split_tab + (size_t)c * kSplitCap;
+    for (int t = tid; t < ns * KP; t += nthreads) {
+        const int gi = t / KP, j = t % KP;
+        if (j >= k) continue;
+        const int2 e = tab[gi];
+        const int se = tab[gi + 1].y;
+        double s = slots[(size_t)e.y * KP + j];
+        for (int q = e.y + 1; q < se; ++q) s += slots[(size_t)q * KP + j];
+        if (e.x >= 0)
+            a.Y[(size_t)e.x * k + j] = (float)s;
+        else if (e.x == kCodeHead)
+            a.carry[(size_t)(2 * c) * k + j] = s;
+        else if (e.x == kCodeTail)
+            a.carry[(size_t)(2 * c + 1) * k + j] = s;
+    }
+    // empty rows of the chunk's row range
+    if (a.has_empty) {
+        const int r0 = zr, r1 = a.chunk_rows[c + 1];
+        if (r1 > r0) {
+            const int wlo = r0 >> 5, whi = (r1 - 1) >> 5;
+            for (int w = wlo + tid; w <= whi; w += nthreads) {
+                unsigned bits = a.empty_bits[w];
+                const int rb = w << 5;
+                if (rb < r0) bits &= ~0u << (r0 - rb);
+                if (rb + 32 > r1) bits &= (r1 - rb >= 32) ? ~0u : ((1u << (r1 - rb)) - 1u);
+                while (bits) {
+                    const int jj = __ffs(bits) - 1;
+                    float *yr = a.Y + (size_t)(rb + jj) * k;
+                    for (int j = 0; j < k; ++j) yr[j] = 0.f;
+                    bits &= bits - 1;
+                }
+            }
+        }
+    }
+    // the slots are reused by the workgroup's next chunk (batched path)
+    __syncthreads();
+}
+
+// Threads per workgroup: KP/4 waves (one chunk) for thread groups, one wave
+// for the lane-per-nonzero scheme of KP = 32.
+template <int KP>
+constexpr int spmm_threads() { return KP >= 32 ? 64 : 16 * KP; }
+
+template <typename VT, int KP>
+__device__ __forceinline__ void spmm_any_chunk(const SpmmDesc &a, int c, double *smem, int *segbuf) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if constexpr (KP >= 32)
+        spmm_chunk_lane<VT, KP>(a, c, lane, smem);
+    else
+        spmm_chunk<VT, KP>(a, c, wv, lane, smem, segbuf + wv * 64 * kStepGroup);
+}
+
+template <typename VT, int KP>
+__global__ __launch_bounds__(spmm_threads<KP>()) void spmm_kernel(SpmmDesc a) {
+    extern __shared__ double smem[];
+    __shared__ int segbuf[KP >= 32 ? 1 : KP / 4][64 * kStepGroup];
+    const int nb = gridDim.x;
+    // XCD-aware: each XCD gets a contiguous chunk range
+    const int q = nb >> 3, r = nb & 7, x = blockIdx.x & 7, i = blockIdx.x >> 3;
+    const int c = (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+    if (c >= a.nchunks) return;
+    spmm_any_chunk<VT, KP>(a, c, smem, &segbuf[0][0]);
+}
+
+// Persistent batched SpMM: workgroups pop (matrix, chunk) work items until the
+// queue is exhausted. The queue (matrices in order, chunks in order) is cut
+// into kBatchQueues contiguous ranges, one per XCD (workgroups reach the XCDs
+// round-robin): a workgroup drains its own XCD's range first, so the chunks of
+// one matrix, and the rows of X they share, stay in one L2; then it helps
+// with the other ranges.
+template <typename VT, int KP>
+__global__ __launch_bounds__(spmm_threads<KP>()) void spmm_batch_kernel(const SpmmDesc *__restrict__ descs,
+                                                                        const int2 *__restrict__ queue, int total,
+                                                                        int *__restrict__ counter) {
+    extern __shared__ double smem[];
+    __shared__ int segbuf[KP >= 32 ? 1 : KP / 4][64 * kStepGroup];
+    __shared__ int item;
+    const int nq = (int)gridDim.x < kBatchQueues ? (int)gridDim.x : kBatchQueues;
+    const int own = (int)blockIdx.x % nq;
+    for (int r = 0; r < nq; ++r) {
+        const int q = (own + r) % nq;
+        const int q0 = (int)((long long)total * q / nq), q1 = (int)((long long)total * (q + 1) / nq);
+        for (;;) {
+            if (threadIdx.x == 0) item = q0 + atomicAdd(&counter[q], 1);
+            __syncthreads();
+            const int idx = item;
+            __syncthreads();  // every thread has read `item` before it is overwritten
+            if (idx >= q1) break;
+            const int2 w = queue[idx];
+            spmm_any_chunk<VT, KP>(descs[w.x], w.y, smem, &segbuf[0][0]);
+        }
+    }
 }
 
 // Loads row `col` of X (k <= KP floats) into xv[0..KP).
@@ -59,10 +301,13 @@ __device__ __forceinline__ void load_xrow(const float *__restrict__ X, int k, in
     }
 }
 
-// One chunk of one matrix. `slots` is this wave's LDS area of
-// max_slots * KP doubles.
+// Wide blocks (KP = 32): one chunk of one matrix on one wave, each lane
+// loading the whole X row of its nonzero (KP/4 16-byte loads, two steps in
+// flight) into its own KP accumulators. Measured faster than thread groups at
+// this width: every lane keeps 16 independent loads in flight. `slots` is the
+// wave's LDS area of max_slots * KP doubles.
 template <typename VT, int KP>
-__device__ void spmm_chunk(const SpmmDesc &a, int c, int lane, double *slots) {
+__device__ void spmm_chunk_lane(const SpmmDesc &a, int c, int lane, double *slots) {
     const long long C = 64LL * a.S;
     const size_t base = (size_t)C * (size_t)c;
     const int steps = (c == a.nchunks - 1) ? (int)((a.layout_elems - (long long)base) >> 6) : a.S;
@@ -195,35 +440,6 @@ __device__ void spmm_chunk(const SpmmDesc &a, int c, int lane, double *slots) {
     wave_sync_lds_m();
 }
 
-template <typename VT, int KP>
-__global__ __launch_bounds__(64) void spmm_kernel(SpmmDesc a) {
-    extern __shared__ double smem[];
-    const int lane = threadIdx.x & 63;
-    const int nb = gridDim.x;
-    // XCD-aware: each XCD gets a contiguous chunk range
-    const int q = nb >> 3, r = nb & 7, x = blockIdx.x & 7, i = blockIdx.x >> 3;
-    const int c = (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-    if (c >= a.nchunks) return;
-    spmm_chunk<VT, KP>(a, c, lane, smem);
-}
-
-// Persistent batched SpMM: waves pop (matrix, chunk) work items until the
-// queue is exhausted.
-template <typename VT, int KP>
-__global__ __launch_bounds__(64) void spmm_batch_kernel(const SpmmDesc *__restrict__ descs,
-                                                        const int2 *__restrict__ queue, int total,
-                                                        int *__restrict__ counter) {
-    extern __shared__ double smem[];
-    const int lane = threadIdx.x & 63;
-    for (;;) {
-        int idx = 0;
-        if (lane == 0) idx = atomicAdd(counter, 1);
-        idx = __shfl(idx, 0);
-        if (idx >= total) break;
-        const int2 w = queue[idx];
-        spmm_chunk<VT, KP>(descs[w.x], w.y, lane, smem);
-    }
-}
 
 // Cross-chunk rows of one matrix, all k columns; chunk order.
 __device__ __forceinline__ void spmm_fix_chunk(const SpmmDesc &a, int c) {
@@ -247,7 +463,7 @@ __global__ void spmm_fixup_kernel(SpmmDesc a) {
 __global__ void spmm_batch_fixup_kernel(const SpmmDesc *__restrict__ descs, const int2 *__restrict__ queue, int total,
                                         int *__restrict__ counter) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) *counter = 0;  // the queue kernel has finished (stream order): rearm it
+    if (t < kBatchQueues) counter[t] = 0;  // the queue kernel has finished (stream order): rearm it
     if (t < total) {
         const int2 w = queue[t];
         spmm_fix_chunk(descs[w.x], w.y);
@@ -257,7 +473,7 @@ __global__ void spmm_batch_fixup_kernel(const SpmmDesc *__restrict__ descs, cons
 // ---------------------------------------------------------------- launchers
 template <typename VT>
 static hipError_t launch_spmm_vt(const SpmmDesc &a, int kp, size_t lds, hipStream_t st) {
-    dim3 grid(a.nchunks), block(64);
+    dim3 grid(a.nchunks), block(kp >= 32 ? 64 : 16 * kp);
     switch (kp) {
     case 8: hipLaunchKernelGGL((spmm_kernel<VT, 8>), grid, block, lds, st, a); break;
     case 16: hipLaunchKernelGGL((spmm_kernel<VT, 16>), grid, block, lds, st, a); break;
@@ -302,7 +518,7 @@ hipError_t spmm_batch_grid(int prec, int k, size_t lds, int *grid) {
         f = kp == 8 ? batch_kernel_ptr<unsigned short, 8>()
                     : kp == 16 ? batch_kernel_ptr<unsigned short, 16>() : batch_kernel_ptr<unsigned short, 32>();
     int per_cu = 0, dev = 0, ncu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64, lds);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kp >= 32 ? 64 : 16 * kp, lds);
     if (e != hipSuccess) return e;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
@@ -314,7 +530,7 @@ hipError_t launch_spmm_batch(const SpmmDesc *d_descs, const int2 *d_queue, int t
                              size_t lds, int grid, hipStream_t st) {
     if (total == 0) return hipSuccess;
     const int kp = spmm_kp(k);
-    dim3 g(grid), b(64);
+    dim3 g(grid), b(kp >= 32 ? 64 : 16 * kp);
 #define PAVA_BATCH(VT)                                                                                          \
     switch (kp) {                                                                                              \
     case 8: hipLaunchKernelGGL((spmm_batch_kernel<VT, 8>), g, b, lds, st, d_descs, d_queue, total, d_counter); break;  \

@@ -138,3 +138,15 @@ def test_omp_equals_serial(P, O):
     A = P.rmat(12, 8, seed=5)
     x = P.fill_uniform(A.numCols, 1)
     assert np.array_equal(O.spmv_omp(A, x, 4), O.spmv_fma(A, x))
+
+
+def test_driver_reference_spmv_equals_oracle_on_strided_and_f32_x(P, O):
+    """The driver's reference SpMV (pava's referenceSpMV) equals the oracle bit
+    for bit, also when x arrives as a strided column or in FP32."""
+    A = P.synthetic(3000, P.RL_POISSON, 12.0, col_struct=P.COL_MIXED, half_band=100, band_frac=0.7, seed=9)
+    X = np.random.default_rng(1).uniform(-1, 1, (A.numCols, 4))
+    for j in range(4):
+        col = X[:, j]  # strided view
+        assert np.array_equal(P.reference_spmv(A, col), O.spmv_fma(A, np.ascontiguousarray(col)))
+    x32 = X[:, 0].astype(np.float32)
+    assert np.array_equal(P.reference_spmv(A, x32), O.spmv_fma(A, x32.astype(np.float64)))
