@@ -9,7 +9,8 @@ namespace cvr {
 
 constexpr int kLanes = 64;          // one wavefront = one CVR "SIMD unit"
 constexpr int kWavesPerBlock = 4;   // 256-thread workgroups, one chunk per wave
-constexpr int kWavesLds = 16;       // LDS hot-x variant: 1024-thread workgroups, one per CU
+constexpr int kWavesLds = 8;        // LDS hot-x variant: 512-thread workgroups, one per CU (the
+                                    // smaller static LDS leaves room for a larger table)
 constexpr int kStepGroup = 8;       // steps per chunk are a multiple of this
 constexpr int kSplitCap = 72;       // split-table entries per chunk (groups + sentinel)
 constexpr int kSegExtra = 66;       // segment-list slack per chunk (steals + padding piece)

@@ -97,7 +97,7 @@ __device__ __forceinline__ void store_y(void *y, int r, double v) {
     reinterpret_cast<YT *>(y)[r] = (YT)v;
 }
 
-// LDSX: 16-wave workgroups (one per CU) first copy the head of the hot-x
+// LDSX: kWavesLds-wave workgroups (one per CU) first copy the head of the hot-x
 // buffer -- the most used columns, hot numbers [0, n_lds) -- into LDS, then
 // each wave walks chunks of the workgroup's contiguous chunk range; a gather
 // of such a column is an LDS read instead of a divergent global load.
