@@ -244,6 +244,23 @@ int csr_hip_create(const pava_csr_matrix *A, int lanes_per_row, int precision, i
 int csr_hip_spmv(csr_hip_handle *h, const void *d_x, void *d_y, void *stream);
 int csr_hip_destroy(csr_hip_handle *h);
 
+/* "csr5_hip": CSR5 (csr5/detail/avx512/format_avx512.h, csr5_spmv_avx512.h;
+ * anonymouslibHandle::asCSR5 / spmv, csr5/anonymouslib_avx512.h:107-221)
+ * re-derived for 64-lane wavefronts: tiles of 64 x sigma nonzeros stored
+ * transposed, per-lane row-start bit flags, lane-order segmented sums and a
+ * tile-order calibrate step. FP64. sigma in {4, 8, 16, 32}, 0 -> 16.
+ * csr5_hip_export_layout copies the device arrays out (vals/cols: ntiles *
+ * 64 * sigma; flags/yoff: ntiles * 64; tile_base: ntiles; nerows: non-empty
+ * rows; fix: 2 * ntiles; cont: ntiles). */
+typedef struct csr5_hip_handle csr5_hip_handle;
+int csr5_hip_create(const pava_csr_matrix *A, int sigma, int device, csr5_hip_handle **out);
+int csr5_hip_spmv(csr5_hip_handle *h, const double *d_x, double *d_y, void *stream);
+int csr5_hip_spmv_host(csr5_hip_handle *h, const double *x, double *y);
+int csr5_hip_get_info(const csr5_hip_handle *h, int *ntiles, int *sigma, double *convert_ms);
+int csr5_hip_export_layout(const csr5_hip_handle *h, double *vals, int *cols, unsigned *flags, int *yoff,
+                           int *tile_base, int *nerows, int *fix, unsigned char *cont);
+int csr5_hip_destroy(csr5_hip_handle *h);
+
 /* ------------------------------------------------------------- selector
  * No counterpart in pava (it benchmarks every format instead of choosing).
  * 16 cheap CSR features (log sizes, row-length statistics, bandwidth, gather
@@ -310,8 +327,15 @@ int conduct_cvr_hip(int numRows, int numCols, int nItems_raw, int *h_rowDelimite
 int conduct_csr_hip(int numRows, int numCols, int nItems, int *h_rowDelimiters, int *h_cols, double *h_val,
                     double *h_vec, double *refOut, int lanes_per_row, char *matrixName, int numIterations);
 
+/* conduct_csr5 (csr5/csr5.h:72-179) on the GPU, same arguments (alpha is a
+ * double there and ignored here as there): conversion (timed, sigma 16), one
+ * verification SpMV into y (written completely), numIterations timed SpMVs,
+ * the report line with format "CSR5_HIP" and thread column = tiles. */
+int conduct_csr5_hip(int m, int n, int nnz, int *csrRowPtr, int *csrColIdx, double *csrVal, double *x,
+                     double *y, double alpha, char *matrixName, int numIterations);
+
 /* benchFile with the GPU format registry: read, size filters (main.cpp:82-99),
- * x = i%3, referenceSpMV, then each format block in turn (cvr_hip, csr_hip)
+ * x = i%3, referenceSpMV, then each format block in turn (cvr_hip, csr5_hip, csr_hip)
  * with checkResults. Returns the total error count (>= 0), -1 if the matrix
  * was skipped, or a PAVA_HIP_E* code. */
 int pava_bench_file(const char *path, int numIterations);

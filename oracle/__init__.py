@@ -37,6 +37,10 @@ def lib() -> C.CDLL:
         L.orc_cvr64_build.argtypes = [i, i, i, P, P, P, i, i] + [P] * 9
         L.orc_cvr64_spmv.argtypes = [i, i, i, i] + [P] * 10
         L.orc_cvr64_spmv.restype = None
+        L.orc_csr5_ntiles.argtypes = [i, i]
+        L.orc_csr5_build.argtypes = [i, i, P, P, P, i] + [P] * 8
+        L.orc_csr5_spmv.argtypes = [i, i, i] + [P] * 10
+        L.orc_csr5_spmv.restype = None
         _lib = L
     return _lib
 
@@ -146,3 +150,31 @@ def error_stats(y, y_ref, scale=None) -> dict:
         nrm = np.where(sc > 0, d / np.where(sc > 0, sc, 1.0), d)
         out["max_scaled"] = float(nrm.max()) if nrm.size else 0.0
     return out
+
+
+def csr5_build(A, sigma: int = 16) -> dict:
+    """CPU emulation of the GPU CSR5-64 layout (same arrays as Csr5Hip.export_layout)."""
+    rp, ci, va = _csr(A)
+    nt = lib().orc_csr5_ntiles(A.nnz, sigma)
+    L = nt * 64 * sigma
+    out = {"vals": np.zeros(max(L, 1)), "cols": np.zeros(max(L, 1), np.int32),
+           "flags": np.zeros(max(nt * 64, 1), np.uint32), "yoff": np.zeros(max(nt * 64, 1), np.int32),
+           "tile_base": np.zeros(max(nt, 1), np.int32), "nerows": np.zeros(max(A.numRows, 1), np.int32),
+           "fix": np.zeros(max(2 * nt, 2), np.int32), "cont": np.zeros(max(nt, 1), np.uint8)}
+    rc = lib().orc_csr5_build(A.numRows, A.nnz, _p(rp), _p(ci), _p(va), sigma,
+                              *[_p(out[k]) for k in ["vals", "cols", "flags", "yoff", "tile_base", "nerows", "fix",
+                                                     "cont"]])
+    if rc != 0:
+        raise RuntimeError(f"orc_csr5_build failed: {rc}")
+    out.update(ntiles=nt, sigma=sigma, n_nonempty=int(np.count_nonzero(np.diff(rp))))
+    return out
+
+
+def csr5_spmv(A, lay: dict, x):
+    """y over an emulated CSR5-64 layout, in the GPU kernel's summation order."""
+    xx = np.ascontiguousarray(x, np.float64)
+    y = np.empty(A.numRows)
+    lib().orc_csr5_spmv(A.numRows, A.nnz, lay["sigma"], *[_p(lay[k]) for k in
+                                                          ["vals", "cols", "flags", "yoff", "tile_base", "nerows",
+                                                           "fix", "cont"]], _p(xx), _p(y))
+    return y

@@ -414,3 +414,146 @@ void orc_cvr64_spmv(int m, int nnz, int S, int prec, const void *vals, const int
 #undef STORE_Y
     free(carry);
 }
+
+/* ---------------------------------------------------------------- CSR5-64 */
+int orc_csr5_ntiles(int nnz, int sigma) {
+    const long long T = 64LL * sigma;
+    return nnz > 0 ? (int)((nnz + T - 1) / T) : 0;
+}
+
+/* first row r with rp[r] > e (rows 0..m-1), i.e. the row holding element e is r-1 */
+static int row_of(const int *rp, int m, long long e) {
+    int lo = 0, hi = m;  /* answer in [1, m] for 0 <= e < nnz */
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((long long)rp[mid + 1] > e) hi = mid; else lo = mid + 1;
+    }
+    return lo;  /* row containing element e */
+}
+
+int orc_csr5_build(int m, int nnz, const int *rp, const int *ci, const double *va, int sigma, double *vals_t,
+                   int *cols_t, uint32_t *flags, int *yoff, int *tile_base, int *nerows, int *fix,
+                   unsigned char *cont) {
+    if (sigma < 1 || sigma > 32) return -1;
+    const int nt = orc_csr5_ntiles(nnz, sigma);
+    const long long T = 64LL * sigma;
+    /* element e starts a non-empty row: mark by walking the rows */
+    unsigned char *start = (unsigned char *)calloc((size_t)(nt * T + 1), 1);
+    int *rowstart = (int *)malloc(sizeof(int) * (size_t)(nt * T + 1));
+    if (!start || !rowstart) { free(start); free(rowstart); return -2; }
+    int ne = 0;
+    for (int r = 0; r < m; ++r)
+        if (rp[r + 1] > rp[r]) { start[rp[r]] = 1; rowstart[rp[r]] = r; nerows[ne++] = r; }
+    int base = 0;
+    for (int t = 0; t < nt; ++t) {
+        tile_base[t] = base;
+        int acc = 0;
+        for (int l = 0; l < 64; ++l) {
+            uint32_t f = 0;
+            yoff[t * 64 + l] = acc;
+            for (int i = 0; i < sigma; ++i) {
+                const long long e = t * T + (long long)l * sigma + i;
+                const long long dst = t * T + (long long)i * 64 + l;
+                if (e < nnz) {
+                    vals_t[dst] = va[e];
+                    cols_t[dst] = ci[e];
+                    if (start[e]) { f |= 1u << i; ++acc; }
+                } else {
+                    vals_t[dst] = 0.0;
+                    cols_t[dst] = 0;
+                }
+            }
+            flags[t * 64 + l] = f;
+        }
+        base += acc;
+        /* the row open at the tile start, when it began earlier and ends here */
+        const long long t0e = t * T, t1e = (t + 1) * T < nnz ? (t + 1) * T : nnz;
+        const int rf = row_of(rp, m, t0e);
+        fix[2 * t] = -1;
+        fix[2 * t + 1] = -1;
+        if (rp[rf] < t0e && rp[rf + 1] <= t1e) {
+            fix[2 * t] = rf;
+            fix[2 * t + 1] = (int)(rp[rf] / T);
+        }
+        const int rl = row_of(rp, m, t1e - 1);
+        cont[t] = (unsigned char)(rp[rl + 1] > t1e);
+    }
+    free(start);
+    free(rowstart);
+    return 0;
+}
+
+void orc_csr5_spmv(int m, int nnz, int sigma, const double *vals_t, const int *cols_t, const uint32_t *flags,
+                   const int *yoff, const int *tile_base, const int *nerows, const int *fix,
+                   const unsigned char *cont, const double *x, double *y) {
+    const int nt = orc_csr5_ntiles(nnz, sigma);
+    const long long T = 64LL * sigma;
+    double *head = (double *)calloc((size_t)(nt > 0 ? nt : 1), sizeof(double));
+    double *tail = (double *)calloc((size_t)(nt > 0 ? nt : 1), sizeof(double));
+    for (int r = 0; r < m; ++r) y[r] = 0.0;
+    for (int t = 0; t < nt; ++t) {
+        double first[64], open[64];
+        int has[64], owner_row[64];
+        for (int l = 0; l < 64; ++l) {
+            const uint32_t f = flags[t * 64 + l];
+            int k = tile_base[t] + yoff[t * 64 + l];
+            double acc = 0.0;
+            int nseg = 0; /* flags seen */
+            int row = -1;
+            for (int i = 0; i < sigma; ++i) {
+                const long long e = t * T + (long long)l * sigma + i;
+                if (e >= nnz) break;
+                if ((f >> i) & 1u) {
+                    if (nseg == 0) first[l] = acc;
+                    else y[row] = acc; /* a row wholly inside the lane */
+                    row = nerows[k++];
+                    ++nseg;
+                    acc = 0.0;
+                }
+                const long long src = t * T + (long long)i * 64 + l;
+                acc = fma(vals_t[src], x[cols_t[src]], acc);
+            }
+            if (nseg == 0) first[l] = acc;
+            has[l] = nseg > 0;
+            open[l] = nseg > 0 ? acc : 0.0;
+            owner_row[l] = row;
+        }
+        /* rows crossing lanes, in lane order */
+        for (int l = 0; l < 64; ++l) {
+            if (!has[l]) continue;
+            double s = open[l];
+            int j = l + 1;
+            while (j < 64 && !has[j]) s += first[j++];
+            if (j < 64) {
+                s += first[j];
+                y[owner_row[l]] = s;
+            } else if (cont[t]) {
+                tail[t] = s;
+            } else {
+                y[owner_row[l]] = s;
+            }
+        }
+        /* the row open at the tile start */
+        if (!(flags[t * 64] & 1u)) {
+            double h = first[0];
+            if (!has[0]) {
+                int j = 1;
+                while (j < 64 && !has[j]) h += first[j++];
+                if (j < 64) h += first[j];
+            }
+            head[t] = h;
+        }
+    }
+    /* rows crossing tiles, in tile order */
+    for (int t = 0; t < nt; ++t) {
+        const int r = fix[2 * t];
+        if (r < 0) continue;
+        const int t0 = fix[2 * t + 1];
+        double s = tail[t0];
+        for (int u = t0 + 1; u < t; ++u) s += head[u];
+        s += head[t];
+        y[r] = s;
+    }
+    free(head);
+    free(tail);
+}

@@ -41,6 +41,32 @@ void orc_cvr64_spmv(int m, int nnz, int S, int prec, const void *vals, const int
                     const int *seg, const int *split_n, const int *split_tab, const int *fix, const int *rp,
                     const void *x, void *y);
 
+/* ---------------------------------------------------------------- CSR5-64
+ * CPU emulation of the GPU csr5_hip format (CSR5 re-derived for 64 lanes:
+ * csr5/detail/avx512/format_avx512.h, csr5_spmv_avx512.h) and of its
+ * summation order. Tile t holds elements [t*64*sigma, (t+1)*64*sigma); lane l
+ * owns elements t*64*sigma + l*sigma + i (i < sigma), stored transposed at
+ * t*64*sigma + i*64 + l; elements past nnz are padding (value 0, column 0).
+ *   flags[t*64+l]  bit i: element i of the lane starts a non-empty row
+ *   yoff[t*64+l]   row starts in lanes < l of the tile
+ *   tile_base[t]   non-empty rows starting before the tile (index into nerows)
+ *   nerows[]       the non-empty rows in order
+ *   fix[2t]        row whose last element is in tile t and which began in an
+ *                  earlier tile (else -1); fix[2t+1] the tile it began in
+ *   cont[t]        1 if the row open at the end of tile t continues past it */
+int orc_csr5_ntiles(int nnz, int sigma);
+int orc_csr5_build(int m, int nnz, const int *rp, const int *ci, const double *va, int sigma, double *vals_t,
+                   int *cols_t, uint32_t *flags, int *yoff, int *tile_base, int *nerows, int *fix,
+                   unsigned char *cont);
+/* y = A x over a built CSR5-64 layout, in the GPU kernel's summation order:
+ * segments summed by fma from 0 in element order, a row crossing lanes as
+ * (its owner lane's open segment) + (whole following lanes) + (leading
+ * segment of the lane where it ends), in lane order, a row crossing tiles as
+ * tail + whole tiles + head in tile order; rows with no element get 0. */
+void orc_csr5_spmv(int m, int nnz, int sigma, const double *vals_t, const int *cols_t, const uint32_t *flags,
+                   const int *yoff, const int *tile_base, const int *nerows, const int *fix,
+                   const unsigned char *cont, const double *x, double *y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,7 +83,8 @@ EXPORTED = [
     "cvr_hip_batch_destroy", "csr_hip_create", "csr_hip_spmv", "csr_hip_destroy", "pava_csr_features",
     "pava_mlp_create", "pava_mlp_num_params", "pava_mlp_get_params", "pava_mlp_set_params", "pava_mlp_predict",
     "pava_mlp_loss_grad", "pava_mlp_train", "pava_mlp_destroy", "cvr_hip_export_hot", "pava_write_csr_binary",
-    "pava_read_csr_binary", "conduct_csr_hip",
+    "pava_read_csr_binary", "conduct_csr_hip", "csr5_hip_create", "csr5_hip_spmv", "csr5_hip_spmv_host",
+    "csr5_hip_get_info", "csr5_hip_export_layout", "csr5_hip_destroy", "conduct_csr5_hip",
 ]
 
 
@@ -435,6 +436,82 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Csr5Hip:
+    """The csr5_hip format: CSR5 tiles of 64 x sigma nonzeros, FP64."""
+
+    def __init__(self, A: CSR, sigma: int = 16, device: int = 0):
+        L = lib()
+        P = C.c_void_p
+        L.csr5_hip_create.argtypes = [C.POINTER(_CSR), C.c_int, C.c_int, C.POINTER(P)]
+        L.csr5_hip_spmv.argtypes = [P] * 4
+        L.csr5_hip_spmv_host.argtypes = [P] * 3
+        L.csr5_hip_get_info.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.csr5_hip_export_layout.argtypes = [P] * 9
+        L.csr5_hip_destroy.argtypes = [P]
+        self._h = P()
+        s = A.c_struct()
+        _check(L.csr5_hip_create(C.byref(s), sigma, device, C.byref(self._h)))
+        self.shape = (A.numRows, A.numCols)
+        self.n_nonempty = int(np.count_nonzero(np.diff(A.rowOffsets)))
+
+    def info(self) -> dict:
+        nt, sg, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().csr5_hip_get_info(self._h, C.byref(nt), C.byref(sg), C.byref(ms)))
+        return {"ntiles": nt.value, "sigma": sg.value, "convert_ms": ms.value}
+
+    def spmv(self, x_ptr: int, y_ptr: int, stream: int = 0) -> None:
+        """y = A x on device pointers, asynchronous on `stream`."""
+        _check(lib().csr5_hip_spmv(self._h, x_ptr, y_ptr, stream))
+
+    def spmv_host(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(self.shape[0])
+        _check(lib().csr5_hip_spmv_host(self._h, _ptr(x), _ptr(y)))
+        return y
+
+    def export_layout(self) -> dict:
+        inf = self.info()
+        nt, sg = inf["ntiles"], inf["sigma"]
+        L = nt * 64 * sg
+        out = {"vals": np.zeros(max(L, 1)), "cols": np.zeros(max(L, 1), np.int32),
+               "flags": np.zeros(max(nt * 64, 1), np.uint32), "yoff": np.zeros(max(nt * 64, 1), np.int32),
+               "tile_base": np.zeros(max(nt, 1), np.int32), "nerows": np.zeros(max(self.n_nonempty, 1), np.int32),
+               "fix": np.zeros(max(2 * nt, 2), np.int32), "cont": np.zeros(max(nt, 1), np.uint8)}
+        _check(lib().csr5_hip_export_layout(self._h, *[_ptr(out[k]) for k in ["vals", "cols", "flags", "yoff",
+                                                                               "tile_base", "nerows", "fix",
+                                                                               "cont"]]))
+        out.update(inf)
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            lib().csr5_hip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def conduct_csr5_hip(A: CSR, x: np.ndarray, name: str = "matrix", iterations: int = 10) -> np.ndarray:
+    """Drop-in conduct_csr5: converts, verifies into y, times, prints the report line."""
+    L = lib()
+    L.conduct_csr5_hip.argtypes = [C.c_int] * 3 + [C.c_void_p] * 5 + [C.c_double, C.c_char_p, C.c_int]
+    y = np.empty(A.numRows)
+    x = np.ascontiguousarray(x, np.float64)
+    _check(L.conduct_csr5_hip(A.numRows, A.numCols, A.nnz, _ptr(A.rowOffsets), _ptr(A.cols), _ptr(A.vals),
+                              _ptr(x), _ptr(y), 1.0, name.encode(), iterations))
+    return y
 
 
 class CsrHip:

@@ -149,7 +149,13 @@ extern "C" int pava_bench_file(const char *path, int numIterations) {
     rc = conduct_cvr_hip(A.numRows, A.numCols, A.nnz, A.rowOffsets, A.cols, A.vals, x.data(), y.data(), 1, nm.data(),
                          numIterations);
     int errors = rc ? rc : pava_check_results(A.numRows, y.data(), y_ref.data());
-    // the registry's second GPU format, as pava runs its format blocks in turn
+    // the registry's other GPU formats, as pava runs its format blocks in turn
+    printf("**************\n    CSR5_HIP   \n**************\n");
+    pava_init_vectors(A.numRows, 0, nullptr, y.data(), nullptr);
+    int rc5 = conduct_csr5_hip(A.numRows, A.numCols, A.nnz, A.rowOffsets, A.cols, A.vals, x.data(), y.data(), 1.0,
+                               nm.data(), numIterations);
+    const int err5 = rc5 ? rc5 : pava_check_results(A.numRows, y.data(), y_ref.data());
+    if (errors >= 0 && err5 != 0) errors = err5 < 0 ? err5 : errors + err5;
     printf("**************\n    CSR_HIP    \n**************\n");
     pava_init_vectors(A.numRows, 0, nullptr, y.data(), nullptr);
     int rc2 = conduct_csr_hip(A.numRows, A.numCols, A.nnz, A.rowOffsets, A.cols, A.vals, x.data(), y.data(), 8,
