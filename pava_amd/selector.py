@@ -1,10 +1,11 @@
 """Autotuner and learned selector for the SpMV path (no pava counterpart).
 
-* ``CONFIGS`` -- the candidate space: format (cvr_hip / csr_hip), value
+* ``CONFIGS`` -- the candidate space: format (cvr_hip / csr5_hip / csr_hip), value
   precision (FP64; FP32 with FP64 recompute of rows whose cancellation ratio
   exceeds 64, which keeps every row within ~1e-5 of FP64; the same with 16-bit
-  chunk-relative columns), tile (CVR steps per chunk S, or CSR lanes per row L)
-  and x staging (hot-x copy of the most used columns, its head held in LDS).
+  chunk-relative columns), tile (CVR steps per chunk S, CSR5 elements per lane
+  sigma, or CSR lanes per row L) and x staging (hot-x copy of the most used
+  columns, its head held in LDS).
 * ``autotune`` -- exhaustive search: builds each candidate on the GPU, times
   it with hipEvents, returns per-config times (inf where a candidate does not
   apply, e.g. a chunk spanning more than 65536 columns for 16-bit columns).
@@ -22,16 +23,16 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (CSR, CvrHip, CsrHip, PREC_F32, PREC_F64, PavaError, _check, lib)
+from . import (CSR, Csr5Hip, CvrHip, CsrHip, PREC_F32, PREC_F64, PavaError, _check, lib)
 
 N_FEATURES = 16
 
 
 @dataclass(frozen=True)
 class Config:
-    fmt: str        # "cvr" | "csr"
+    fmt: str        # "cvr" | "csr5" | "csr"
     prec: str       # "f64" | "f32rc" | "f32rc16"
-    tile: int       # S for cvr, L for csr
+    tile: int       # S for cvr (0: the library's automatic size), sigma for csr5, L for csr
     xs: int = 0     # x staging: hot-x copy size in KiB (0 = off)
 
     @property
@@ -44,8 +45,10 @@ CONFIGS = ([Config("cvr", "f64", s) for s in (32, 128, 512)]
            + [Config("cvr", "f32rc16", s) for s in (32, 128, 512)]
            + [Config("csr", "f64", l) for l in (1, 4, 16, 64)]
            + [Config("cvr", "f64", s, 2048) for s in (128, 512)]
-           + [Config("cvr", "f32rc", 128, 2048)])
-DEFAULT = Config("cvr", "f64", 128)  # the fallback without a model
+           + [Config("cvr", "f32rc", 128, 2048)]
+           + [Config("csr5", "f64", s) for s in (8, 32)]
+           + [Config("cvr", "f64", 0)])
+DEFAULT = Config("cvr", "f64", 0)  # the fallback without a model: the library's default
 
 
 def features(A: CSR) -> np.ndarray:
@@ -74,6 +77,8 @@ def _time(fn, iters: int):
 def build(A: CSR, cfg: Config):
     if cfg.fmt == "csr":
         return CsrHip(A, lanes_per_row=cfg.tile, precision=PREC_F64)
+    if cfg.fmt == "csr5":
+        return Csr5Hip(A, sigma=cfg.tile)
     if cfg.prec == "f64":
         return CvrHip(A, steps_per_chunk=cfg.tile, x_hot_kb=cfg.xs)
     return CvrHip(A, steps_per_chunk=cfg.tile, precision=PREC_F32, recompute=True, kappa_threshold=64.0,
@@ -97,7 +102,7 @@ def autotune(A: CSR, x: np.ndarray, iters: int = 5, configs=CONFIGS) -> np.ndarr
         except PavaError:
             continue
         try:
-            if cfg.fmt == "csr":
+            if cfg.fmt in ("csr", "csr5"):
                 fn = lambda: h.spmv(x64.data_ptr(), y64.data_ptr(), st)
             elif cfg.prec == "f64":
                 fn = lambda: h.spmv(x64.data_ptr(), y64.data_ptr(), st)

@@ -104,7 +104,7 @@ def _chosen_error(i: int, cfg) -> float:
     try:
         dev = torch.device("cuda:0")
         x64 = torch.from_numpy(x).to(dev)
-        if cfg.fmt == "csr" or cfg.prec == "f64":
+        if cfg.fmt in ("csr", "csr5") or cfg.prec == "f64":
             y = torch.empty(A.numRows, dtype=torch.float64, device=dev)
             h.spmv(x64.data_ptr(), y.data_ptr(), 0)
         else:
@@ -134,7 +134,7 @@ def evaluate(data: Path, epochs: int = 400, seed: int = 0, check_errors: int = 2
               "train_loss_first_last": [float(hist[0]), float(hist[-1])],
               "train_s": round(time.perf_counter() - t0, 2)}
     dflt = S.CONFIGS.index(S.DEFAULT)
-    plain = S.CONFIGS.index(S.Config("cvr", "f64", 128))
+    plain = dflt  # plain FP64 CVR-HIP: the library's default configuration
     for name, m in sets.items():
         if not m.any():
             continue

@@ -80,7 +80,8 @@ def test_learned_selector_beats_fixed_default():
 def test_trained_selector_gain_on_committed_dataset():
     """The MLP trained on the GPU on the committed autotune dataset (2,000
     matrices) picks configs whose measured times on the 200 held-out matrices
-    beat the fixed default and come close to the exhaustive best."""
+    beat the library's default configuration and come close to the exhaustive
+    best."""
     from pathlib import Path
     from pava_amd import selector as S
     from pava_amd import selector_data as D
@@ -97,5 +98,7 @@ def test_trained_selector_gain_on_committed_dataset():
     _say("held-out geo-mean slowdown vs exhaustive best: model", gm(sd_model), "default", gm(sd_default),
          "loss", hist[0], "->", hist[-1])
     assert hist[-1] < 0.5 * hist[0]
-    assert gm(sd_model) < 0.6 * gm(sd_default)
-    assert gm(sd_model) < 1.25
+    # the default is the library's own (automatic chunk size); the model must
+    # recover most of what separates it from the exhaustive best
+    assert gm(sd_model) - 1 < 0.4 * (gm(sd_default) - 1)
+    assert gm(sd_model) < 1.1

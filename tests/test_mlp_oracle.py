@@ -58,10 +58,10 @@ def test_features_closed_forms(P):
 def test_config_space():
     from pava_amd import selector as S
     names = [c.name for c in S.CONFIGS]
-    assert len(names) == len(set(names)) == 16
+    assert len(names) == len(set(names)) == 19
     assert S.DEFAULT in S.CONFIGS
     # every axis the autotuner searches is present: format, precision, tile, x staging
-    assert {c.fmt for c in S.CONFIGS} == {"cvr", "csr"}
+    assert {c.fmt for c in S.CONFIGS} == {"cvr", "csr5", "csr"}
     assert {c.prec for c in S.CONFIGS} == {"f64", "f32rc", "f32rc16"}
     assert {c.xs for c in S.CONFIGS} == {0, 2048}
     assert len({c.tile for c in S.CONFIGS if c.fmt == "cvr"}) >= 3
