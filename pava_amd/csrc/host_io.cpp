@@ -170,15 +170,6 @@ extern "C" int pava_read_coo_matrix(const char *path, pava_coo_matrix *out) {
     bool mirror = b.symmetric || b.skew || b.hermitian;
     long long cap = mirror ? 2 * L : L;
     if (cap > INT32_MAX) return set_error(PAVA_MM_ESIZE, "nnz beyond int32");
-    int *rows = (int *)aligned_alloc64(sizeof(int) * (size_t)cap);
-    int *cols = (int *)aligned_alloc64(sizeof(int) * (size_t)cap);
-    double *vals = (double *)aligned_alloc64(sizeof(double) * (size_t)cap);
-    if (!rows || !cols || !vals) {
-        free(rows);
-        free(cols);
-        free(vals);
-        return set_error(PAVA_MM_ENOMEM, "could not allocate COO arrays (nnz %lld)", cap);
-    }
     // The body is parsed in parallel: the text is cut at line starts into one
     // piece per thread, each piece parsed into its own arrays, and the pieces
     // are concatenated in file order (so duplicates later sum in file order).
@@ -263,16 +254,10 @@ extern "C" int pava_read_coo_matrix(const char *path, pava_coo_matrix *out) {
         const long long n = (long long)P.r.size();
         const long long firstbad = P.bad;
         if (P.oor >= 0 && seen + P.oor < L && (firstbad < 0 || P.oor < firstbad)) {
-            free(rows);
-            free(cols);
-            free(vals);
             return set_error(PAVA_MM_EBODY, "entry %lld (%lld, %lld) outside %lldx%lld", seen + P.oor + 1, P.br, P.bc,
                              M, N);
         }
         if (firstbad >= 0 && seen + firstbad < L) {
-            free(rows);
-            free(cols);
-            free(vals);
             return set_error(PAVA_MM_EBODY, "malformed entry %lld", seen + firstbad + 1);
         }
         seen += n;
@@ -282,9 +267,6 @@ extern "C" int pava_read_coo_matrix(const char *path, pava_coo_matrix *out) {
         }
     }
     if (seen < L) {
-        free(rows);
-        free(cols);
-        free(vals);
         return set_error(PAVA_MM_EBODY, "truncated body: %lld of %lld entries", seen, L);
     }
     // concatenate (with the mirrored halves) in file order
@@ -297,6 +279,18 @@ extern "C" int pava_read_coo_matrix(const char *path, pava_coo_matrix *out) {
         if (mirror)
             for (long long k = 0; k < take; ++k) m += (P.r[(size_t)k] != P.c[(size_t)k]);
         out_off[(size_t)t + 1] = out_off[(size_t)t] + m;
+    }
+    // the COO arrays are sized by the entries actually read, not by the size
+    // line, so a header that overstates nnz cannot force a huge allocation
+    const long long cnt = out_off[(size_t)nparts_used];
+    int *rows = (int *)aligned_alloc64(sizeof(int) * (size_t)std::max<long long>(cnt, 1));
+    int *cols = (int *)aligned_alloc64(sizeof(int) * (size_t)std::max<long long>(cnt, 1));
+    double *vals = (double *)aligned_alloc64(sizeof(double) * (size_t)std::max<long long>(cnt, 1));
+    if (!rows || !cols || !vals) {
+        free(rows);
+        free(cols);
+        free(vals);
+        return set_error(PAVA_MM_ENOMEM, "could not allocate COO arrays (nnz %lld)", cnt);
     }
 #pragma omp parallel for num_threads(T) schedule(static, 1)
     for (int t = 0; t < nparts_used; ++t) {
@@ -318,7 +312,6 @@ extern "C" int pava_read_coo_matrix(const char *path, pava_coo_matrix *out) {
             }
         }
     }
-    const long long cnt = out_off[(size_t)nparts_used];
     out->nnz = (int)cnt;
     out->numRows = (int)M;
     out->numCols = (int)N;
