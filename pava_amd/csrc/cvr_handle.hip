@@ -150,8 +150,8 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     if ((rc = dev_alloc(h, (void **)&h->d_carry, (size_t)h->nchunks * 16 + 16))) return rc;
     if (o->recompute && h->prec != CVR_HIP_PREC_F64) {
         if ((rc = dev_alloc(h, (void **)&h->d_carry_abs, (size_t)h->nchunks * 16 + 16))) return rc;
-        if ((rc = dev_alloc(h, (void **)&h->d_rflag, (size_t)M + 64))) return rc;
-        if ((rc = dev_alloc(h, (void **)&h->d_rc_count, (size_t)h->nchunks * 4 + 4))) return rc;
+        if ((rc = dev_alloc(h, (void **)&h->d_rflag, cvr::rflag_bytes(M)))) return rc;
+        if ((rc = dev_alloc(h, (void **)&h->d_rc_count, cvr::rc_count_len(M, h->nchunks) * 4 + 4))) return rc;
     }
     if ((rc = dev_alloc(h, (void **)&h->d_empty, (size_t)((M + 31) / 32) * 4))) return rc;
     if ((rc = dev_alloc(h, (void **)&h->d_status, 64))) return rc;
@@ -163,9 +163,9 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     HIP_TRY(hipMemsetAsync(h->d_seg, 0, (size_t)h->seg_cap * 4, h->stream));
     HIP_TRY(hipMemsetAsync(h->d_carry, 0, (size_t)h->nchunks * 16 + 16, h->stream));
     if (h->d_rflag) {
-        HIP_TRY(hipMemsetAsync(h->d_rflag, 0, (size_t)M + 64, h->stream));
+        HIP_TRY(hipMemsetAsync(h->d_rflag, 0, cvr::rflag_bytes(M), h->stream));
         HIP_TRY(hipMemsetAsync(h->d_carry_abs, 0, (size_t)h->nchunks * 16 + 16, h->stream));
-        HIP_TRY(hipMemsetAsync(h->d_rc_count, 0, (size_t)h->nchunks * 4 + 4, h->stream));
+        HIP_TRY(hipMemsetAsync(h->d_rc_count, 0, cvr::rc_count_len(M, h->nchunks) * 4 + 4, h->stream));
     }
 
     hipEvent_t e0, e1;
@@ -684,7 +684,7 @@ extern "C" int cvr_hip_last_recompute_count(cvr_hip_handle *h, int *count) {
     if (!h->d_rc_count || h->nchunks == 0) return 0;
     HIP_TRY(hipSetDevice(h->device));
     HIP_TRY(hipDeviceSynchronize());
-    std::vector<int> v((size_t)h->nchunks);
+    std::vector<int> v(cvr::rc_count_len(h->M, h->nchunks));
     HIP_TRY(hipMemcpy(v.data(), h->d_rc_count, v.size() * 4, hipMemcpyDeviceToHost));
     long long t = 0;
     for (int c : v) t += c;

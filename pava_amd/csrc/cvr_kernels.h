@@ -74,7 +74,19 @@ struct SpmvArgs {
     int lds_blocks;  // grid of the LDS variant
 };
 
-// FP64 recompute of flagged rows, one wave per chunk (mixed precision only).
+// FP64 recompute after a mixed-precision SpMV: one wave per 64 chunks for the
+// cross-chunk rows, then one wave per block of kFinishRows rows for the rows
+// flagged by the SpMV.
+constexpr int kFinishRows = 1024;
+__host__ __device__ inline int finish_fix_waves(int nchunks) { return (nchunks + 63) / 64; }
+__host__ __device__ inline int finish_row_blocks(int M) { return (M + kFinishRows - 1) / kFinishRows; }
+// flag buffer bytes: whole blocks (read 16 flags per lane) plus slack
+inline size_t rflag_bytes(int M) { return (size_t)finish_row_blocks(M) * kFinishRows + 64; }
+// recompute counters: one per wave of the recompute pass
+inline size_t rc_count_len(int M, int nchunks) {
+    return (size_t)finish_fix_waves(nchunks) + (size_t)finish_row_blocks(M);
+}
+
 struct FinishArgs {
     const int *fix;
     const double *carry;
@@ -86,7 +98,7 @@ struct FinishArgs {
     const double *x64;
     unsigned char *rflag;
     float *y;
-    int *rc_count;       // per chunk: rows recomputed by the last SpMV
+    int *rc_count;       // rows recomputed by the last SpMV, per wave (rc_count_len entries)
     float kappa;
     int nchunks, M;
 };

@@ -347,105 +347,195 @@ __global__ void zero_kernel(void *y, int n) {
     if (i < n) reinterpret_cast<YT *>(y)[i] = (YT)0;
 }
 
-// Mixed precision, after the SpMV: one wave per chunk (1) closes the chunk's
-// cross-chunk row (as fixup_kernel) and recomputes it when its cancellation
-// ratio exceeds kappa, then (2) recomputes every flagged row of an even 1/nchunks
-// share of the rows from the resident FP64 CSR and FP64 x, and clears the flags
-// for the next SpMV. Every reduction has a fixed order (deterministic).
+// Mixed precision, after the SpMV. Waves [0, ceil(nchunks / 64)) take 64
+// chunks each, one per lane: each lane closes its chunk's cross-chunk row (as
+// fixup_kernel), and the rows whose cancellation ratio exceeds kappa are
+// recomputed in FP64. The following waves take kFinishRows rows each, recompute
+// the rows the SpMV flagged from the resident FP64 CSR and FP64 x, and clear
+// the flags for the next SpMV. Every reduction has a fixed order
+// (deterministic).
 __device__ __forceinline__ double wave_sum_f64(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
 
+// Long rows: lane-strided FMA chains (loads issued 8 elements per lane at a
+// time), then a fixed xor-butterfly reduction.
 __device__ __forceinline__ double row_fp64(const FinishArgs &a, int row, int lane) {
+    const int b = a.rp[row], e = a.rp[row + 1];
     double s = 0.0;
-    for (int j = a.rp[row] + lane; j < a.rp[row + 1]; j += 64) s = __builtin_fma(a.va[j], a.x64[a.ci[j]], s);
+    for (int j0 = b + lane; j0 < e; j0 += 8 * 64) {
+        int cc[8];
+        double vv[8], xx[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = j0 + u * 64;
+            cc[u] = j < e ? a.ci[j] : 0;
+            vv[u] = j < e ? a.va[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xx[u] = a.x64[cc[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (j0 + u * 64 < e) s = __builtin_fma(vv[u], xx[u], s);
+    }
     return wave_sum_f64(s);
 }
 
-constexpr int kFlagWindow = 1024;  // rows whose flags one wave lists at a time
-constexpr int kLongRow = 128;      // longer flagged rows are recomputed by the whole wave
+constexpr int kLongRow = 128;  // longer flagged rows are recomputed by the whole wave
+constexpr int kStage = 512;    // elements of short rows staged in LDS per pass (8 per lane)
 
-__global__ __launch_bounds__(256) void finish_kernel(FinishArgs a) {
-    __shared__ int flag_list[kWavesPerBlock][kFlagWindow];
+struct FinishLds {
+    int flist[kFinishRows];  // flagged rows of the wave's block
+    int soff[64];            // staged offset of each lane's row (exclusive prefix)
+    int sb[64];              // first element of each lane's row
+    double sv[kStage], sx[kStage];
+};
+
+// y[rows[i]] for i < n <= 64, in FP64. Rows of at most kLongRow elements: lane
+// i owns rows[i]; the values and x of all of them are loaded side by side
+// (kStage elements per pass) into LDS, then each owning lane runs its row's
+// FMA chain in element order -- the sequential FMA sum of the row, rounded
+// once to FP32. Longer rows: the whole wave, one after another (row_fp64).
+__device__ void recompute_rows(const FinishArgs &a, const int *rows, int n, int lane, FinishLds &L) {
+    int row = -1, b = 0, len = 0;
+    if (lane < n) {
+        row = rows[lane];
+        b = a.rp[row];
+        len = a.rp[row + 1] - b;
+    }
+    const bool lng = len > kLongRow;
+    const int sl = lng ? 0 : len;
+    int inc = sl;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(inc, d);
+        if (lane >= d) inc += t;
+    }
+    const int off = inc - sl;
+    const int T = __shfl(inc, 63);
+    L.soff[lane] = off;
+    L.sb[lane] = b;
+    wave_sync_lds();
+    double s = 0.0;
+    for (int p0 = 0; p0 < T; p0 += kStage) {
+        int cc[8];
+        double vv[8], xx[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = p0 + lane + 64 * u;
+            cc[u] = 0;
+            vv[u] = 0.0;
+            if (e < T) {
+                // owner: the last lane whose staged range starts at or before e
+                int k = 0;
+#pragma unroll
+                for (int st = 32; st > 0; st >>= 1)
+                    if (L.soff[k + st] <= e) k += st;
+                const int j = L.sb[k] + (e - L.soff[k]);
+                cc[u] = a.ci[j];
+                vv[u] = a.va[j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xx[u] = a.x64[cc[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = p0 + lane + 64 * u;
+            if (e < T) {
+                L.sv[e - p0] = vv[u];
+                L.sx[e - p0] = xx[u];
+            }
+        }
+        wave_sync_lds();
+        const int lo = off > p0 ? off : p0;
+        const int hi = off + sl < p0 + kStage ? off + sl : p0 + kStage;
+        int e = lo;
+        for (; e + 4 <= hi; e += 4) {  // LDS reads of 4 elements issued together
+            double v[4], w[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = L.sv[e + u - p0], w[u] = L.sx[e + u - p0];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) s = __builtin_fma(v[u], w[u], s);
+        }
+        for (; e < hi; ++e) s = __builtin_fma(L.sv[e - p0], L.sx[e - p0], s);
+        wave_sync_lds();  // the next pass overwrites the stage
+    }
+    if (lane < n && !lng) a.y[row] = (float)s;
+    unsigned long long m = __ballot(lng);
+    while (m) {
+        const int k = __ffsll((long long)m) - 1;
+        const int r = __shfl(row, k);
+        const double v = row_fp64(a, r, lane);
+        if (lane == 0) a.y[r] = (float)v;
+        m &= m - 1;
+    }
+}
+
+__global__ __launch_bounds__(64 * kWavesPerBlock) void finish_kernel(FinishArgs a) {
+    __shared__ FinishLds lds[kWavesPerBlock];
     const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    if (c >= a.nchunks) return;
-    int count = 0;
-    const int frow = a.fix[2 * c];
-    if (frow >= 0) {
-        const int c0 = a.fix[2 * c + 1];
-        double s = a.carry[2 * c0 + 1], sa = a.carry_abs[2 * c0 + 1];
-        for (int k = c0 + 1; k < c; ++k) s += a.carry[2 * k + 1], sa += a.carry_abs[2 * k + 1];
-        s += a.carry[2 * c];
-        sa += a.carry_abs[2 * c];
-        if (a.x64 && sa > (double)a.kappa * __builtin_fabs(s)) {
-            s = row_fp64(a, frow, lane);
-            ++count;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int gw = blockIdx.x * kWavesPerBlock + wv;
+    FinishLds &L = lds[wv];
+    const int nfw = finish_fix_waves(a.nchunks);
+    if (gw < nfw) {
+        const int c = 64 * gw + lane;
+        int frow = -1;
+        double s = 0.0, sa = 0.0;
+        if (c < a.nchunks) frow = a.fix[2 * c];
+        if (frow >= 0) {
+            const int c0 = a.fix[2 * c + 1];
+            s = a.carry[2 * c0 + 1], sa = a.carry_abs[2 * c0 + 1];
+            for (int k = c0 + 1; k < c; ++k) s += a.carry[2 * k + 1], sa += a.carry_abs[2 * k + 1];
+            s += a.carry[2 * c];
+            sa += a.carry_abs[2 * c];
         }
-        if (lane == 0) a.y[frow] = (float)s;
+        const bool need = frow >= 0 && a.x64 && sa > (double)a.kappa * __builtin_fabs(s);
+        if (frow >= 0 && !need) a.y[frow] = (float)s;
+        const unsigned long long m = __ballot(need);
+        if (need) L.flist[mbcnt64s(m)] = frow;
+        const int nf = __popcll(m);
+        wave_sync_lds();
+        if (nf) recompute_rows(a, L.flist, nf, lane, L);
+        if (lane == 0) a.rc_count[gw] = nf;
+        return;
     }
-    if (a.x64) {
-        // flagged rows: any wave may recompute any row; rows are split evenly.
-        // Each window's flagged rows are first listed in LDS, then recomputed
-        // side by side: short rows one per lane (sequential FMA in element
-        // order, loads issued 8 elements at a time), long rows by the whole
-        // wave (fixed xor-butterfly reduction).
-        const int wv = threadIdx.x >> 6;
-        int *flist = flag_list[wv];
-        const int r0 = (int)((long long)a.M * c / a.nchunks), r1 = (int)((long long)a.M * (c + 1) / a.nchunks);
-        for (int wb = r0; wb < r1; wb += kFlagWindow) {
-            const int we = wb + kFlagWindow < r1 ? wb + kFlagWindow : r1;
-            int nf = 0;
-            for (int rb = wb; rb < we; rb += 64) {
-                const int r = rb + lane;
-                const bool f = (r < we) && a.rflag[r];
-                const unsigned long long m = __ballot(f);
-                if (f) {
-                    flist[nf + mbcnt64s(m)] = r;
-                    a.rflag[r] = 0;
-                }
-                nf += __popcll(m);
-            }
-            count += nf;
-            wave_sync_lds();
-            for (int k = lane; k < nf; k += 64) {
-                const int r = flist[k];
-                const int b = a.rp[r], e = a.rp[r + 1];
-                if (e - b > kLongRow) continue;
-                double s = 0.0;
-                for (int j0 = b; j0 < e; j0 += 8) {
-                    int cc[8];
-                    double vv[8], xx[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const bool in = j0 + u < e;
-                        cc[u] = in ? a.ci[j0 + u] : 0;
-                        vv[u] = in ? a.va[j0 + u] : 0.0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) xx[u] = a.x64[cc[u]];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                        if (j0 + u < e) s = __builtin_fma(vv[u], xx[u], s);
-                }
-                a.y[r] = (float)s;
-            }
-            for (int k0 = 0; k0 < nf; k0 += 64) {
-                const int k = k0 + lane;
-                const bool lg = k < nf && a.rp[flist[k] + 1] - a.rp[flist[k]] > kLongRow;
-                unsigned long long m = __ballot(lg);
-                while (m) {
-                    const int row = flist[k0 + __ffsll((long long)m) - 1];
-                    const double s = row_fp64(a, row, lane);
-                    if (lane == 0) a.y[row] = (float)s;
-                    m &= m - 1;
-                }
-            }
-            wave_sync_lds();  // the next window overwrites the list
-        }
+    const int rw = gw - nfw;
+    if (rw >= finish_row_blocks(a.M)) return;
+    // flags of rows [r0, r0 + kFinishRows): 16 per lane in two loads (the flag
+    // buffer is padded to a whole number of blocks)
+    const int rb = rw * kFinishRows + 16 * lane;
+    uint64_t f0 = 0, f1 = 0;
+    if (rb < a.M) {
+        f0 = *reinterpret_cast<const uint64_t *>(a.rflag + rb);
+        f1 = *reinterpret_cast<const uint64_t *>(a.rflag + rb + 8);
     }
-    if (lane == 0) a.rc_count[c] = count;
+    unsigned bits = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        bits |= ((f0 >> (8 * i)) & 0xffull) ? (1u << i) : 0u;
+        bits |= ((f1 >> (8 * i)) & 0xffull) ? (1u << (8 + i)) : 0u;
+    }
+    if (f0) *reinterpret_cast<uint64_t *>(a.rflag + rb) = 0ull;
+    if (f1) *reinterpret_cast<uint64_t *>(a.rflag + rb + 8) = 0ull;
+    const int cnt = __popc(bits);
+    int inc = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(inc, d);
+        if (lane >= d) inc += t;
+    }
+    const int nf = __shfl(inc, 63);
+    int pos = inc - cnt;
+    while (bits) {
+        L.flist[pos++] = rb + __ffs(bits) - 1;
+        bits &= bits - 1;
+    }
+    wave_sync_lds();
+    if (a.x64)
+        for (int g = 0; g < nf; g += 64) recompute_rows(a, L.flist + g, nf - g < 64 ? nf - g : 64, lane, L);
+    if (lane == 0) a.rc_count[gw] = a.x64 ? nf : 0;
 }
 
 // xhot[k] = x[hot_cols[k]] (before each SpMV on layouts with hot columns)
@@ -568,7 +658,8 @@ hipError_t plan_spmv_lds(int prec, int recompute, int n_hot, long long max_bytes
 
 hipError_t launch_finish(const FinishArgs &a, hipStream_t st) {
     if (a.nchunks == 0) return hipSuccess;
-    dim3 grid((a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
+    const long long waves = (long long)finish_fix_waves(a.nchunks) + finish_row_blocks(a.M);
+    dim3 grid((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock)), block(64 * kWavesPerBlock);
     hipLaunchKernelGGL(finish_kernel, grid, block, 0, st, a);
     return hipGetLastError();
 }

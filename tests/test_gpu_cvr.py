@@ -265,17 +265,41 @@ def test_fp32_recompute_meets_1e5_on_cancelling_rows(P, O):
     _say("kappa max", kappa.max(), "rows > 64:", int((kappa > 64).sum()), "recomputed", n1,
          "max rel plain", rel_plain.max(), "with recompute", rel_rc.max())
     assert rel_rc.max() <= 1e-5
-    # rows certainly flagged and recomputed one per lane (not rows spanning
-    # chunks) follow the serial FMA order: the FP32 rounding of the oracle
-    emu = O.cvr64_build(A, 64, 1)
-    fr = emu["fix"][0::2]
-    fixrows = set(fr[fr >= 0].tolist())
-    sure = np.array([r for r in np.nonzero(kappa > 1024)[0] if r not in fixrows])
+    # rows certainly flagged (rows spanning chunks included) are recomputed in
+    # the serial FMA order: the FP32 rounding of the oracle
+    sure = np.nonzero(kappa > 1024)[0]
     assert sure.size > 0
     assert np.array_equal(y_rc[sure], yref[sure].astype(np.float32).astype(np.float64))
     # rows with kappa well below the threshold are left to the FP32 path and its bound
     low = kappa < 32
     assert np.all(np.abs(y_rc[low] - yref[low]) <= 2.0 ** -22 * sabs[low] + 2.0 ** -23 * np.abs(yref[low]))
+
+
+@pytest.mark.parametrize("S", [8, 64])
+def test_fp32_recompute_every_row_flagged(P, O, S):
+    """Every row cancels (values +-1 alternating, x = 1 + tiny): all rows are
+    flagged, including rows spanning chunks and rows longer than one lane's
+    share. Rows of at most 128 elements are the serial FMA sum rounded to FP32
+    exactly; longer rows (whole-wave recompute) are within FP64 rounding of it."""
+    rng = np.random.default_rng(11)
+    lens = np.concatenate([rng.integers(2, 40, 3000), rng.integers(100, 160, 300), [700, 2000]])
+    rng.shuffle(lens)
+    M = lens.size
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    n = 4096
+    cols = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in lens]).astype(np.int32)
+    vals = np.concatenate([np.where(np.arange(l) % 2 == 0, 1.0, -1.0) * (1 + 0.01 * rng.random(l)) for l in lens])
+    A = P.CSR(M, n, rp, cols, vals)
+    x = 1.0 + 1e-3 * rng.random(n)
+    yref = O.spmv_fma(A, x)
+    kappa = O.row_abs(A, x) / np.maximum(np.abs(yref), 1e-300)
+    y, cnt = _mixed_run(P, A, x, 1, True, kappa=1.0, S=S)
+    _say(f"all-flagged S={S}: rows {M}, recomputed {cnt}, kappa min {kappa.min():.1f}")
+    assert cnt == M
+    short = lens <= 128
+    assert np.array_equal(y[short], yref[short].astype(np.float32).astype(np.float64))
+    rel = np.abs(y[~short] - yref[~short]) / np.abs(yref[~short])
+    assert rel.max() <= 2.0 ** -24 + 1e-12 * kappa[~short].max()
 
 
 def test_fp32_recompute_random_x_and_rmat(P, O):
