@@ -282,8 +282,9 @@ def main():
                     help="precision: FP64/FP32/BF16 storage on the workload's matrix; batch: 512-matrix SpMM")
     ap.add_argument("--k", type=int, default=8, help="batch variant: right-hand sides")
     ap.add_argument("--nmat", type=int, default=512, help="batch variant: matrices")
-    ap.add_argument("--x-hot-kb", type=int, default=2048,
-                    help="hot-x staging buffer (KiB; 0 = off): the most used columns read from a dense copy")
+    ap.add_argument("--x-hot-kb", type=int, default=-1,
+                    help="hot-x staging buffer (KiB; 0 = off; -1 = automatic: 2048 KiB when column use is skewed): "
+                         "the most used columns read from a dense copy")
     ap.add_argument("--x-lds-kb", type=int, default=-1,
                     help="of the hot-x buffer, KiB held in LDS per workgroup (-1 = as much as fits, 0 = none)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -383,7 +384,10 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
                      "kernel_ms": round(ms_main, 5)},
     }
-    traffic = measured_traffic(args.workload, kernel, S, args.x_hot_kb)
+    # the staging actually in effect (the automatic mode stages 2048 KiB or nothing)
+    hot_kb = 0 if info["n_hot_cols"] == 0 else (2048 if args.x_hot_kb < 0 else args.x_hot_kb)
+    result["config"]["x_hot_kb_effective"] = hot_kb
+    traffic = measured_traffic(args.workload, kernel, S, hot_kb)
     if traffic is not None:
         result["roofline"]["traffic"] = traffic
         result["roofline"]["traffic_GB_s_at_this_kernel_time"] = round(traffic / (ms_main * 1e-3) / 1e9, 1)

@@ -425,8 +425,8 @@ __global__ void remap_hot_kernel(int *__restrict__ cols, long long L, const int 
     }
 }
 
-hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, int *cnt, int *colmap, int *hot_cols,
-                              int *cols, long long L, int *n_hot, hipStream_t st) {
+hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, double min_skew, int *cnt, int *colmap,
+                              int *hot_cols, int *cols, long long L, int *n_hot, hipStream_t st) {
     *n_hot = 0;
     hipError_t e;
     if ((e = hipMemsetAsync(cnt, 0, (size_t)N * 4, st)) != hipSuccess) return e;
@@ -449,6 +449,20 @@ hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, 
     const long long part_take = part >= 2 ? std::min<long long>(max_hot - taken, nwith[(size_t)part]) : 0;
     const int n = (int)(taken + part_take);
     if (n == 0) return hipSuccess;
+    if (min_skew > 0) {
+        // skew of the column use: uses of the most used eighth of the used
+        // columns (at most the selection) against an even spread
+        const long long used = (long long)N - nwith[0];
+        const long long K = std::max<long long>(1, std::min<long long>(n, used / 8));
+        long long left = K;
+        double top_uses = 0;
+        for (int v = maxc; v >= 1 && left > 0; --v) {
+            const long long k = std::min<long long>(left, nwith[(size_t)v]);
+            top_uses += (double)v * (double)k;
+            left -= k;
+        }
+        if (top_uses * (double)used < min_skew * (double)K * (double)nnz) return hipSuccess;
+    }
     std::vector<long long> pos((size_t)maxc + 2, 0);  // first output position per count
     long long acc = 0;
     for (int v = maxc; v >= std::max(part, 2); --v) pos[(size_t)v] = acc, acc += nwith[(size_t)v];

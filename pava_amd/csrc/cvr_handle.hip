@@ -87,6 +87,13 @@ extern "C" void cvr_hip_default_opts(cvr_hip_opts *o) {
 
 static int value_size(int prec) { return prec == CVR_HIP_PREC_F64 ? 8 : prec == CVR_HIP_PREC_F32 ? 4 : 2; }
 
+// Automatic hot-x staging (x_hot_kb = -1): a copy of at most 2 MiB (half an
+// XCD's L2), made only when the most used eighth of the used columns is used
+// at least twice as often as an average used column (power-law column use;
+// not stencils or FEM meshes).
+constexpr int kAutoHotKb = 2048;
+constexpr double kAutoHotSkew = 2.0;
+
 extern "C" int cvr_hip_destroy(cvr_hip_handle *h) {
     if (!h) return 0;
     hipSetDevice(h->device);
@@ -107,7 +114,7 @@ static int validate_opts(const cvr_hip_opts *o) {
         return set_error(PAVA_HIP_EINVAL, "col_index_bits must be 32 or 16");
     if (o->col_index_bits == 16 && o->precision == CVR_HIP_PREC_F64)
         return set_error(PAVA_HIP_EINVAL, "16-bit columns are offered for FP32/BF16 storage only");
-    if (o->x_hot_kb < 0) return set_error(PAVA_HIP_EINVAL, "x_hot_kb must be >= 0, got %d", o->x_hot_kb);
+    if (o->x_hot_kb < -1) return set_error(PAVA_HIP_EINVAL, "x_hot_kb must be >= -1, got %d", o->x_hot_kb);
     if (o->x_lds_kb < -1) return set_error(PAVA_HIP_EINVAL, "x_lds_kb must be >= -1, got %d", o->x_lds_kb);
     return 0;
 }
@@ -214,7 +221,9 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
 
     // hot-x staging: renumber the most used columns into a dense copy of x
     const int xsize = h->prec == CVR_HIP_PREC_F64 ? 8 : 4;
-    const long long max_hot = (long long)o->x_hot_kb * 1024 / xsize;
+    // x_hot_kb < 0: automatic (kAutoHotKb budget, staged only on skewed column use)
+    const bool auto_hot = o->x_hot_kb < 0;
+    const long long max_hot = (long long)(auto_hot ? kAutoHotKb : o->x_hot_kb) * 1024 / xsize;
     if (max_hot > 0 && !h->col16 && nnz > 0 && N < (1 << 30)) {
         const auto t0 = std::chrono::steady_clock::now();
         int *cnt = nullptr, *colmap = nullptr;
@@ -224,8 +233,8 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
             dev_alloc(h, (void **)&h->d_hot_cols, (size_t)std::min<long long>(max_hot, N) * 4 + 4) != 0) {
             e = hipErrorOutOfMemory;
         } else {
-            e = cvr::launch_hot_columns(d_ci, nnz, N, (int)std::min<long long>(max_hot, N), cnt, colmap, h->d_hot_cols,
-                                        (int *)h->d_cols, h->layout_elems, &nh, h->stream);
+            e = cvr::launch_hot_columns(d_ci, nnz, N, (int)std::min<long long>(max_hot, N), auto_hot ? kAutoHotSkew : 0.0,
+                                        cnt, colmap, h->d_hot_cols, (int *)h->d_cols, h->layout_elems, &nh, h->stream);
         }
         (void)hipStreamSynchronize(h->stream);
         (void)hipFree(cnt);

@@ -137,8 +137,11 @@ hipError_t plan_spmv_lds(int prec, int recompute, int n_hot, long long max_bytes
                          int *lds_blocks);
 hipError_t launch_fixup(const int *fix, const double *carry, void *y, int nchunks, int prec, hipStream_t st);
 hipError_t launch_finish(const FinishArgs &a, hipStream_t st);
-hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, int *cnt, int *colmap, int *hot_cols,
-                              int *cols, long long L, int *n_hot, hipStream_t st);
+// Hot-x staging. min_skew > 0: stage nothing unless the most used eighth of
+// the used columns is used at least min_skew times as often as an average
+// used column.
+hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, double min_skew, int *cnt, int *colmap,
+                              int *hot_cols, int *cols, long long L, int *n_hot, hipStream_t st);
 hipError_t launch_hot_gather(const void *x, const int *hot_cols, void *xhot, int n_hot, int xsize, hipStream_t st);
 
 }  // namespace cvr

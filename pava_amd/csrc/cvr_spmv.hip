@@ -92,6 +92,16 @@ struct Acc<VT, float, true, KAPPA> {
     __device__ __forceinline__ void reset() { s = 0.f, c = 0.f, ab = 0.f; }
 };
 
+// Global-memory load from an integer address (a plain pointer cast would be
+// a flat access, which the compiler's wait-count tracking treats as possibly
+// LDS as well).
+template <typename T>
+__device__ __forceinline__ T load_global(uint64_t addr) {
+    return *reinterpret_cast<const __attribute__((address_space(1))) T *>(addr);
+}
+
+constexpr int kWaitVm0 = 0x0f70;  // s_waitcnt vmcnt(0), other counters unconstrained
+
 template <typename YT>
 __device__ __forceinline__ void store_y(void *y, int r, double v) {
     reinterpret_cast<YT *>(y)[r] = (YT)v;
@@ -152,26 +162,33 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock, (COL16
                     if ((cv & kHotFlag) && idx < (uint32_t)a.n_lds) return tab[idx];
                 }
                 const uint64_t b = (cv & kHotFlag) ? ah : ax;
-                return *reinterpret_cast<const XT *>(b + (uint64_t)idx * sizeof(XT));
+                return load_global<XT>(b + (uint64_t)idx * sizeof(XT));
             }
         };
-        constexpr int GE = COL16 ? G : 1;  // end masks are stored only for 16-bit layouts
-        auto load_cols = [&](int g0, int (&cc)[G], uint64_t (&em)[GE]) {
+        // 16-bit layouts: the G end masks of a group arrive in one vector load,
+        // mask i in lane i, and are read out with readlane when used (a
+        // wave-uniform load would be turned into a readfirstlane right after
+        // it, which waits for every load issued before it)
+        auto step_mask = [&](uint64_t em, int i) -> uint64_t {
+            const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)em, i);
+            const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(em >> 32), i);
+            return ((uint64_t)hi << 32) | lo;
+        };
+        auto load_cols = [&](int g0, int (&cc)[G], uint64_t &em) {
             if constexpr (COL16) {
 #pragma unroll
                 for (int i = 0; i < G; ++i) cc[i] = (int)cp16[(size_t)(g0 + i) * 64];
-#pragma unroll
-                for (int i = 0; i < G; ++i) em[i] = emp[g0 + i];
+                em = emp[g0 + (lane & (G - 1))];
             } else {
 #pragma unroll
                 for (int i = 0; i < G; ++i) cc[i] = __builtin_nontemporal_load(&cp32[(size_t)(g0 + i) * 64]);
             }
         };
-        auto group_ends = [&](const int (&cc)[G], const uint64_t (&em)[GE]) -> int {
+        auto group_ends = [&](const int (&cc)[G], uint64_t em) -> int {
             int e = 0;
             if constexpr (COL16) {
 #pragma unroll
-                for (int i = 0; i < G; ++i) e += __popcll(em[i]);
+                for (int i = 0; i < G; ++i) e += __popcll(step_mask(em, i));
             } else {
 #pragma unroll
                 for (int i = 0; i < G; ++i) e += __popcll(__ballot(cc[i] < 0));
@@ -187,7 +204,7 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock, (COL16
         Acc<VT, XT, COMP, KAPPA> acc;
         VT vb[G], vn[G];
         int cb[G], cn[G], cnn[G];
-        uint64_t eb[GE], en[GE], enn[GE];
+        uint64_t eb = 0, en = 0, enn = 0;
         XT xb[G], xn[G];
         int sc[G], sn[G];
 
@@ -200,6 +217,11 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock, (COL16
         int ecur = group_ends(cb, eb);
         load_segs(0, ecur, sc);
         if (G < steps) load_cols(G, cn, en);
+        // drain the prologue's loads: the loop is then entered with nothing in
+        // flight, so the compiler's wait counts inside it follow the loop's own
+        // loads only (otherwise the entry state makes the first group wait for
+        // most of the next group's loads)
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
         int sbase = 0, segp = 0;
 
         for (int g = 0; g < steps; g += G) {
@@ -226,7 +248,7 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock, (COL16
                 bool end;
                 unsigned long long m;
                 if constexpr (COL16) {
-                    m = eb[i];
+                    m = step_mask(eb, i);
                     end = (m >> lane) & 1ull;
                 } else {
                     end = cb[i] < 0;
@@ -257,11 +279,8 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock, (COL16
                     cn[i] = cnn[i];
                     sc[i] = sn[i];
                 }
-#pragma unroll
-                for (int i = 0; i < GE; ++i) {
-                    eb[i] = en[i];
-                    en[i] = enn[i];
-                }
+                eb = en;
+                en = enn;
                 sbase = sbase_next;
                 ecur = enext;
             }

@@ -100,6 +100,24 @@ def test_hot_x_staging_decodes_and_is_bit_identical(P, O, mats, hot_kb, lds_kb):
         assert np.array_equal(y, O.cvr64_spmv(A, emu, x)), name
 
 
+def test_automatic_hot_x_only_on_skewed_column_use(P, O):
+    """x_hot_kb = -1 stages the hot columns of a power-law matrix and nothing
+    for meshes whose columns are used evenly; y is the emulation's either way."""
+    cases = {"rmat": (P.rmat(14, 16, seed=6), True), "stencil": (P.stencil27(24, 0.01, 2), False),
+             "laplacian": (P.laplacian2d(200, 150), False)}
+    for name, (A, skewed) in cases.items():
+        x = P.fill_uniform(A.numCols, 3)
+        with P.CvrHip(A, steps_per_chunk=64, x_hot_kb=-1) as h:
+            n_hot = h.info()["n_hot_cols"]
+            y = h.spmv_host(x)
+        counts = np.bincount(A.cols, minlength=A.numCols)
+        _say(f"auto hot-x {name}: hot columns {n_hot}, max/mean use {counts.max() / counts[counts > 0].mean():.1f}")
+        assert (n_hot > 0) == skewed, name
+        if skewed:
+            assert n_hot <= 2048 * 1024 // 8
+        assert np.array_equal(y, O.cvr64_spmv(A, O.cvr64_build(A, 64, 0), x)), name
+
+
 @pytest.mark.parametrize("prec,recompute", [(0, False), (1, False), (1, True), (2, False)])
 def test_lds_hot_table_bit_identical(P, O, prec, recompute):
     """The LDS hot-x variant (one workgroup per CU walking a chunk range) gives
