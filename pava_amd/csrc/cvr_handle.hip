@@ -34,6 +34,10 @@ struct cvr_hip_handle {
     double *d_carry_mm = nullptr;  // SpMM carries: 2 * nchunks * 32
     unsigned char *d_rflag = nullptr;
     int *d_rc_count = nullptr;
+    // recompute of long rows: the rows longer than kLongRow, cut in pieces
+    int n_long = 0, n_pieces = 0;
+    int *d_long_rows = nullptr, *d_long_poff = nullptr, *d_piece_row = nullptr, *d_rc_long = nullptr;
+    double *d_partial = nullptr;
     unsigned *d_empty = nullptr;
     int *d_status = nullptr;
     // FP64 CSR kept resident (recompute); owned by the handle when set
@@ -173,6 +177,35 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
         HIP_TRY(hipMemsetAsync(h->d_rflag, 0, cvr::rflag_bytes(M), h->stream));
         HIP_TRY(hipMemsetAsync(h->d_carry_abs, 0, (size_t)h->nchunks * 16 + 16, h->stream));
         HIP_TRY(hipMemsetAsync(h->d_rc_count, 0, cvr::rc_count_len(M, h->nchunks) * 4 + 4, h->stream));
+    }
+
+    if (h->d_rflag) {
+        // the rows longer than kLongRow and their pieces (static per matrix)
+        std::vector<int> rp((size_t)M + 1);
+        HIP_TRY(hipMemcpy(rp.data(), d_rp, ((size_t)M + 1) * 4, hipMemcpyDeviceToHost));
+        std::vector<int> rows, poff, prow;
+        for (int r = 0; r < M; ++r) {
+            const int len = rp[(size_t)r + 1] - rp[(size_t)r];
+            if (len <= cvr::kLongRow) continue;
+            poff.push_back((int)prow.size());
+            for (int k = 0; k < (len + cvr::kPiece - 1) / cvr::kPiece; ++k) prow.push_back((int)rows.size());
+            rows.push_back(r);
+        }
+        poff.push_back((int)prow.size());
+        h->n_long = (int)rows.size();
+        h->n_pieces = (int)prow.size();
+        if (h->n_long > 0) {
+            if ((rc = dev_alloc(h, (void **)&h->d_long_rows, rows.size() * 4)) ||
+                (rc = dev_alloc(h, (void **)&h->d_long_poff, poff.size() * 4)) ||
+                (rc = dev_alloc(h, (void **)&h->d_piece_row, prow.size() * 4)) ||
+                (rc = dev_alloc(h, (void **)&h->d_partial, prow.size() * 8)) ||
+                (rc = dev_alloc(h, (void **)&h->d_rc_long, rows.size() * 4)))
+                return rc;
+            HIP_TRY(hipMemcpy(h->d_long_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(h->d_long_poff, poff.data(), poff.size() * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(h->d_piece_row, prow.data(), prow.size() * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemsetAsync(h->d_rc_long, 0, rows.size() * 4, h->stream));
+        }
     }
 
     hipEvent_t e0, e1;
@@ -441,6 +474,23 @@ static int enqueue_tail(cvr_hip_handle *h, void *d_y, const double *d_x64, hipSt
         f.nchunks = h->nchunks;
         f.M = h->M;
         HIP_TRY(cvr::launch_finish(f, st));
+        if (h->n_long > 0) {
+            cvr::LongArgs l{};
+            l.rp = h->d_rp;
+            l.ci = h->d_ci;
+            l.va = h->d_va;
+            l.x64 = d_x64;
+            l.rflag = h->d_rflag;
+            l.long_rows = h->d_long_rows;
+            l.poff = h->d_long_poff;
+            l.piece_row = h->d_piece_row;
+            l.partial = h->d_partial;
+            l.y = (float *)d_y;
+            l.rc_long = h->d_rc_long;
+            l.n_long = h->n_long;
+            l.n_pieces = h->n_pieces;
+            if (d_x64) HIP_TRY(cvr::launch_long_rows(l, st));
+        }
     } else if (h->n_fix > 0) {
         HIP_TRY(cvr::launch_fixup(h->d_fix, h->d_carry, d_y, h->nchunks, h->prec, st));
     }
@@ -693,8 +743,11 @@ extern "C" int cvr_hip_last_recompute_count(cvr_hip_handle *h, int *count) {
     if (!h->d_rc_count || h->nchunks == 0) return 0;
     HIP_TRY(hipSetDevice(h->device));
     HIP_TRY(hipDeviceSynchronize());
-    std::vector<int> v(cvr::rc_count_len(h->M, h->nchunks));
-    HIP_TRY(hipMemcpy(v.data(), h->d_rc_count, v.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<int> v(cvr::rc_count_len(h->M, h->nchunks) + (size_t)h->n_long);
+    HIP_TRY(hipMemcpy(v.data(), h->d_rc_count, cvr::rc_count_len(h->M, h->nchunks) * 4, hipMemcpyDeviceToHost));
+    if (h->n_long > 0)
+        HIP_TRY(hipMemcpy(v.data() + cvr::rc_count_len(h->M, h->nchunks), h->d_rc_long, (size_t)h->n_long * 4,
+                          hipMemcpyDeviceToHost));
     long long t = 0;
     for (int c : v) t += c;
     *count = (int)t;

@@ -87,6 +87,11 @@ inline size_t rc_count_len(int M, int nchunks) {
     return (size_t)finish_fix_waves(nchunks) + (size_t)finish_row_blocks(M);
 }
 
+// Flagged rows longer than kLongRow are recomputed apart from the others, in
+// pieces of kPiece elements (one wave each), the pieces then added in order.
+constexpr int kLongRow = 128;
+constexpr int kPiece = 2048;
+
 struct FinishArgs {
     const int *fix;
     const double *carry;
@@ -137,6 +142,23 @@ hipError_t plan_spmv_lds(int prec, int recompute, int n_hot, long long max_bytes
                          int *lds_blocks);
 hipError_t launch_fixup(const int *fix, const double *carry, void *y, int nchunks, int prec, hipStream_t st);
 hipError_t launch_finish(const FinishArgs &a, hipStream_t st);
+
+// Long flagged rows (static list of the rows longer than kLongRow).
+struct LongArgs {
+    const int *rp;
+    const int *ci;
+    const double *va;
+    const double *x64;
+    unsigned char *rflag;
+    const int *long_rows;  // n_long rows, ascending
+    const int *poff;       // n_long + 1: first piece of each long row
+    const int *piece_row;  // n_pieces: index into long_rows
+    double *partial;       // n_pieces
+    float *y;
+    int *rc_long;          // n_long: 1 if the row was recomputed by the last SpMV
+    int n_long, n_pieces;
+};
+hipError_t launch_long_rows(const LongArgs &a, hipStream_t st);
 // Hot-x staging. min_skew > 0: stage nothing unless the most used eighth of
 // the used columns is used at least min_skew times as often as an average
 // used column.

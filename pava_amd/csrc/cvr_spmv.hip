@@ -378,10 +378,10 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
     return v;
 }
 
-// Long rows: lane-strided FMA chains (loads issued 8 elements per lane at a
-// time), then a fixed xor-butterfly reduction.
-__device__ __forceinline__ double row_fp64(const FinishArgs &a, int row, int lane) {
-    const int b = a.rp[row], e = a.rp[row + 1];
+// Elements [b, e) of the FP64 CSR: lane-strided FMA chains (loads issued 8
+// elements per lane at a time), then a fixed xor-butterfly reduction.
+__device__ __forceinline__ double range_fp64(const int *__restrict__ ci, const double *__restrict__ va,
+                                             const double *__restrict__ x64, int b, int e, int lane) {
     double s = 0.0;
     for (int j0 = b + lane; j0 < e; j0 += 8 * 64) {
         int cc[8];
@@ -389,11 +389,11 @@ __device__ __forceinline__ double row_fp64(const FinishArgs &a, int row, int lan
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int j = j0 + u * 64;
-            cc[u] = j < e ? a.ci[j] : 0;
-            vv[u] = j < e ? a.va[j] : 0.0;
+            cc[u] = j < e ? ci[j] : 0;
+            vv[u] = j < e ? va[j] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) xx[u] = a.x64[cc[u]];
+        for (int u = 0; u < 8; ++u) xx[u] = x64[cc[u]];
 #pragma unroll
         for (int u = 0; u < 8; ++u)
             if (j0 + u * 64 < e) s = __builtin_fma(vv[u], xx[u], s);
@@ -401,8 +401,7 @@ __device__ __forceinline__ double row_fp64(const FinishArgs &a, int row, int lan
     return wave_sum_f64(s);
 }
 
-constexpr int kLongRow = 128;  // longer flagged rows are recomputed by the whole wave
-constexpr int kStage = 512;    // elements of short rows staged in LDS per pass (8 per lane)
+constexpr int kStage = 512;  // elements of short rows staged in LDS per pass (8 per lane)
 
 struct FinishLds {
     int flist[kFinishRows];  // flagged rows of the wave's block
@@ -415,8 +414,9 @@ struct FinishLds {
 // i owns rows[i]; the values and x of all of them are loaded side by side
 // (kStage elements per pass) into LDS, then each owning lane runs its row's
 // FMA chain in element order -- the sequential FMA sum of the row, rounded
-// once to FP32. Longer rows: the whole wave, one after another (row_fp64).
-__device__ void recompute_rows(const FinishArgs &a, const int *rows, int n, int lane, FinishLds &L) {
+// once to FP32 -- and clears the row's flag. Longer rows are left flagged (or
+// flagged) for the long-row kernels. Returns the number of rows recomputed.
+__device__ int recompute_rows(const FinishArgs &a, const int *rows, int n, int lane, FinishLds &L) {
     int row = -1, b = 0, len = 0;
     if (lane < n) {
         row = rows[lane];
@@ -480,15 +480,15 @@ __device__ void recompute_rows(const FinishArgs &a, const int *rows, int n, int 
         for (; e < hi; ++e) s = __builtin_fma(L.sv[e - p0], L.sx[e - p0], s);
         wave_sync_lds();  // the next pass overwrites the stage
     }
-    if (lane < n && !lng) a.y[row] = (float)s;
-    unsigned long long m = __ballot(lng);
-    while (m) {
-        const int k = __ffsll((long long)m) - 1;
-        const int r = __shfl(row, k);
-        const double v = row_fp64(a, r, lane);
-        if (lane == 0) a.y[r] = (float)v;
-        m &= m - 1;
+    if (lane < n) {
+        if (lng) {
+            a.rflag[row] = 1;
+        } else {
+            a.y[row] = (float)s;
+            a.rflag[row] = 0;
+        }
     }
+    return __popcll(__ballot(lane < n && !lng));
 }
 
 __global__ __launch_bounds__(64 * kWavesPerBlock) void finish_kernel(FinishArgs a) {
@@ -516,8 +516,8 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void finish_kernel(FinishArgs 
         if (need) L.flist[mbcnt64s(m)] = frow;
         const int nf = __popcll(m);
         wave_sync_lds();
-        if (nf) recompute_rows(a, L.flist, nf, lane, L);
-        if (lane == 0) a.rc_count[gw] = nf;
+        const int done = nf ? recompute_rows(a, L.flist, nf, lane, L) : 0;
+        if (lane == 0) a.rc_count[gw] = done;
         return;
     }
     const int rw = gw - nfw;
@@ -536,8 +536,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void finish_kernel(FinishArgs 
         bits |= ((f0 >> (8 * i)) & 0xffull) ? (1u << i) : 0u;
         bits |= ((f1 >> (8 * i)) & 0xffull) ? (1u << (8 + i)) : 0u;
     }
-    if (f0) *reinterpret_cast<uint64_t *>(a.rflag + rb) = 0ull;
-    if (f1) *reinterpret_cast<uint64_t *>(a.rflag + rb + 8) = 0ull;
+    if (!a.x64) {  // nothing to recompute with: drop the flags
+        if (f0) *reinterpret_cast<uint64_t *>(a.rflag + rb) = 0ull;
+        if (f1) *reinterpret_cast<uint64_t *>(a.rflag + rb + 8) = 0ull;
+        bits = 0;
+    }
     const int cnt = __popc(bits);
     int inc = cnt;
 #pragma unroll
@@ -552,9 +555,40 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void finish_kernel(FinishArgs 
         bits &= bits - 1;
     }
     wave_sync_lds();
-    if (a.x64)
-        for (int g = 0; g < nf; g += 64) recompute_rows(a, L.flist + g, nf - g < 64 ? nf - g : 64, lane, L);
-    if (lane == 0) a.rc_count[gw] = a.x64 ? nf : 0;
+    int done = 0;
+    for (int g = 0; g < nf; g += 64) done += recompute_rows(a, L.flist + g, nf - g < 64 ? nf - g : 64, lane, L);
+    if (lane == 0) a.rc_count[gw] = done;
+}
+
+// Long flagged rows: one wave per piece of kPiece elements computes the
+// piece's FP64 sum; then one thread per long row adds its pieces in order,
+// stores the row and clears its flag.
+__global__ __launch_bounds__(64 * kWavesPerBlock) void long_piece_kernel(LongArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (p >= a.n_pieces) return;
+    const int li = a.piece_row[p];
+    const int r = a.long_rows[li];
+    if (!a.rflag[r]) return;
+    const int b = a.rp[r] + (p - a.poff[li]) * kPiece;
+    const int re = a.rp[r + 1];
+    const double s = range_fp64(a.ci, a.va, a.x64, b, b + kPiece < re ? b + kPiece : re, lane);
+    if (lane == 0) a.partial[p] = s;
+}
+
+__global__ void long_sum_kernel(LongArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n_long) return;
+    const int r = a.long_rows[t];
+    int done = 0;
+    if (a.rflag[r]) {
+        double s = a.partial[a.poff[t]];
+        for (int k = a.poff[t] + 1; k < a.poff[t + 1]; ++k) s += a.partial[k];
+        a.y[r] = (float)s;
+        a.rflag[r] = 0;
+        done = 1;
+    }
+    a.rc_long[t] = done;
 }
 
 // xhot[k] = x[hot_cols[k]] (before each SpMV on layouts with hot columns)
@@ -680,6 +714,14 @@ hipError_t launch_finish(const FinishArgs &a, hipStream_t st) {
     const long long waves = (long long)finish_fix_waves(a.nchunks) + finish_row_blocks(a.M);
     dim3 grid((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock)), block(64 * kWavesPerBlock);
     hipLaunchKernelGGL(finish_kernel, grid, block, 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_long_rows(const LongArgs &a, hipStream_t st) {
+    if (a.n_long == 0) return hipSuccess;
+    hipLaunchKernelGGL(long_piece_kernel, dim3((a.n_pieces + kWavesPerBlock - 1) / kWavesPerBlock),
+                       dim3(64 * kWavesPerBlock), 0, st, a);
+    hipLaunchKernelGGL(long_sum_kernel, dim3((a.n_long + 255) / 256), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

@@ -296,15 +296,16 @@ def test_fp32_recompute_meets_1e5_on_cancelling_rows(P, O):
 @pytest.mark.parametrize("S", [8, 64])
 def test_fp32_recompute_every_row_flagged(P, O, S):
     """Every row cancels (values +-1 alternating, x = 1 + tiny): all rows are
-    flagged, including rows spanning chunks and rows longer than one lane's
-    share. Rows of at most 128 elements are the serial FMA sum rounded to FP32
-    exactly; longer rows (whole-wave recompute) are within FP64 rounding of it."""
+    flagged, including rows spanning chunks and rows recomputed in several
+    pieces. Rows of at most 128 elements are the serial FMA sum rounded to FP32
+    exactly; longer rows (pieces of 2048 summed by whole waves, then added in
+    order) are within FP64 rounding of it."""
     rng = np.random.default_rng(11)
-    lens = np.concatenate([rng.integers(2, 40, 3000), rng.integers(100, 160, 300), [700, 2000]])
+    lens = np.concatenate([rng.integers(2, 40, 3000), rng.integers(100, 160, 300), [700, 2000, 5000]])
     rng.shuffle(lens)
     M = lens.size
     rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
-    n = 4096
+    n = 8192
     cols = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in lens]).astype(np.int32)
     vals = np.concatenate([np.where(np.arange(l) % 2 == 0, 1.0, -1.0) * (1 + 0.01 * rng.random(l)) for l in lens])
     A = P.CSR(M, n, rp, cols, vals)
