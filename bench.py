@@ -150,7 +150,8 @@ def cpu_baseline(O, A, x, budget_s: float = 12.0):
 def run_precision(P, A, x, steps: int, S: int = 0) -> dict:
     """FP64 vs FP32 / BF16 storage (FP32 compensated accumulation), with and
     without FP64 recompute of rows whose cancellation ratio exceeds 64, and with
-    16-bit chunk-relative columns; per-row error vs the FP64 reference SpMV."""
+    16-bit chunk-relative columns; per-row error vs the FP64 reference SpMV.
+    32-bit layouts use automatic hot-x staging, as the main run does."""
     import torch
     dev = torch.device("cuda:0")
     yref = P.reference_spmv(A, x)
@@ -165,7 +166,7 @@ def run_precision(P, A, x, steps: int, S: int = 0) -> dict:
     out = {}
     for name, kw, sv, si, sx in variants:
         try:
-            h = P.CvrHip(A, steps_per_chunk=S, **kw)
+            h = P.CvrHip(A, steps_per_chunk=S, x_hot_kb=-1, **kw)
         except P.PavaError as e:
             out[name] = {"error": str(e)}
             continue

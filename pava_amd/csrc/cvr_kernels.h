@@ -90,7 +90,7 @@ inline size_t rc_count_len(int M, int nchunks) {
 // Flagged rows longer than kLongRow are recomputed apart from the others, in
 // pieces of kPiece elements (one wave each), the pieces then added in order.
 constexpr int kLongRow = 128;
-constexpr int kPiece = 2048;
+constexpr int kPiece = 512;
 
 struct FinishArgs {
     const int *fix;

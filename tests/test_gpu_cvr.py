@@ -298,7 +298,7 @@ def test_fp32_recompute_every_row_flagged(P, O, S):
     """Every row cancels (values +-1 alternating, x = 1 + tiny): all rows are
     flagged, including rows spanning chunks and rows recomputed in several
     pieces. Rows of at most 128 elements are the serial FMA sum rounded to FP32
-    exactly; longer rows (pieces of 2048 summed by whole waves, then added in
+    exactly; longer rows (pieces of 512 summed by whole waves, then added in
     order) are within FP64 rounding of it."""
     rng = np.random.default_rng(11)
     lens = np.concatenate([rng.integers(2, 40, 3000), rng.integers(100, 160, 300), [700, 2000, 5000]])
