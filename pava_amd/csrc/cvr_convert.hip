@@ -412,6 +412,68 @@ __global__ __launch_bounds__(256) void col_hist_kernel(const int *__restrict__ c
         if (val[i]) atomicAdd(&cnt[key[i]], val[i]);
 }
 
+// Histogram of the column counts: bin v counts the columns used v times
+// (bins saturate at kCountBins - 1). Small counts (most columns) go to an LDS
+// copy per workgroup, larger ones straight to global memory.
+constexpr int kCountBins = 65536;
+constexpr int kCountLds = 2048;
+
+__global__ __launch_bounds__(256) void count_hist_kernel(const int *__restrict__ cnt, int N, int *__restrict__ hist) {
+    __shared__ int lh[kCountLds];
+    for (int i = threadIdx.x; i < kCountLds; i += blockDim.x) lh[i] = 0;
+    __syncthreads();
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+        const int v = min(cnt[c], kCountBins - 1);
+        if (v < kCountLds)
+            atomicAdd(&lh[v], 1);
+        else
+            atomicAdd(&hist[v], 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kCountLds; i += blockDim.x)
+        if (lh[i]) atomicAdd(&hist[i], lh[i]);
+}
+
+// Order-preserving compaction of the columns used at least `lo` times:
+// pass 1 counts per block of kCompact columns, pass 2 (after an exclusive
+// scan of the block counts) writes (column, count) in column order.
+constexpr int kCompact = 1024;
+
+__global__ __launch_bounds__(256) void compact_count_kernel(const int *__restrict__ cnt, int N, int lo,
+                                                            int *__restrict__ bcount) {
+    __shared__ int part[4];
+    const int c0 = blockIdx.x * kCompact;
+    int k = 0;
+    for (int i = threadIdx.x; i < kCompact; i += blockDim.x) k += (c0 + i < N && cnt[c0 + i] >= lo) ? 1 : 0;
+    for (int off = 32; off > 0; off >>= 1) k += __shfl_xor(k, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = k;
+    __syncthreads();
+    if (threadIdx.x == 0) bcount[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+__global__ __launch_bounds__(256) void compact_write_kernel(const int *__restrict__ cnt, int N, int lo,
+                                                            const int *__restrict__ boff, int2 *__restrict__ out) {
+    __shared__ int wsum[4];
+    const int c0 = blockIdx.x * kCompact;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int base = boff[blockIdx.x];
+    // four rounds of 256 columns, each in thread order
+    for (int r = 0; r < kCompact / 256; ++r) {
+        const int c = c0 + r * 256 + threadIdx.x;
+        const int v = c < N ? cnt[c] : 0;
+        const bool f = c < N && v >= lo;
+        const unsigned long long m = __ballot(f);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wv] = __popcll(m);
+        __syncthreads();
+        int wbase = base;
+        for (int w = 0; w < wv; ++w) wbase += wsum[w];
+        if (f) out[wbase + before] = make_int2(c, v);
+        base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+}
+
 __global__ void hot_scatter_kernel(const int *__restrict__ hot_cols, int n, int *__restrict__ colmap) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) colmap[hot_cols[k]] = k;
@@ -425,56 +487,47 @@ __global__ void remap_hot_kernel(int *__restrict__ cols, long long L, const int 
     }
 }
 
-hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, double min_skew, int *cnt, int *colmap,
-                              int *hot_cols, int *cols, long long L, int *n_hot, hipStream_t st) {
-    *n_hot = 0;
-    hipError_t e;
-    if ((e = hipMemsetAsync(cnt, 0, (size_t)N * 4, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(col_hist_kernel, dim3(2048), dim3(256), 0, st, ci, nnz, cnt);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    std::vector<int> h((size_t)N);
-    if ((e = hipMemcpyAsync(h.data(), cnt, (size_t)N * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
-    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-    // the max_hot most used columns (each used at least twice), ordered by
-    // count descending, then column ascending: a counting sort over counts
-    int maxc = 0;
-    for (int c = 0; c < N; ++c) maxc = std::max(maxc, h[(size_t)c]);
-    std::vector<long long> nwith((size_t)maxc + 2, 0);  // columns per count value
-    for (int c = 0; c < N; ++c) ++nwith[(size_t)h[(size_t)c]];
+// Selection of the max_hot most used columns (each used at least twice),
+// ordered by count descending, then column ascending. From the number of
+// columns per count value: every column with count >= t is taken, and of the
+// columns with count part = t - 1 the first part_take in column order.
+struct Selection {
+    int t = 0, part = 0, n = 0;
+    long long part_take = 0;
+};
+
+static Selection select_hot(const std::vector<long long> &nwith, int maxc, int max_hot) {
+    Selection s;
     long long taken = 0;
-    int t = maxc + 1;  // threshold: every column with count > t is taken
+    int t = maxc + 1;
     while (t > 2 && taken + nwith[(size_t)t - 1] <= max_hot) taken += nwith[(size_t)--t];
-    // t: smallest count fully taken; columns with count t-1 fill what is left
-    const int part = t - 1;
-    const long long part_take = part >= 2 ? std::min<long long>(max_hot - taken, nwith[(size_t)part]) : 0;
-    const int n = (int)(taken + part_take);
-    if (n == 0) return hipSuccess;
-    if (min_skew > 0) {
-        // skew of the column use: uses of the most used eighth of the used
-        // columns (at most the selection) against an even spread
-        const long long used = (long long)N - nwith[0];
-        const long long K = std::max<long long>(1, std::min<long long>(n, used / 8));
-        long long left = K;
-        double top_uses = 0;
-        for (int v = maxc; v >= 1 && left > 0; --v) {
-            const long long k = std::min<long long>(left, nwith[(size_t)v]);
-            top_uses += (double)v * (double)k;
-            left -= k;
-        }
-        if (top_uses * (double)used < min_skew * (double)K * (double)nnz) return hipSuccess;
+    s.t = t;
+    s.part = t - 1;
+    s.part_take = s.part >= 2 ? std::min<long long>(max_hot - taken, nwith[(size_t)s.part]) : 0;
+    s.n = (int)(taken + s.part_take);
+    return s;
+}
+
+// Skew of the column use: uses of the most used eighth of the used columns
+// (at most the selection) against an even spread over the used columns.
+static bool skewed(const std::vector<long long> &nwith, int maxc, int N, long long nnz, int n, double min_skew) {
+    const long long used = (long long)N - nwith[0];
+    const long long K = std::max<long long>(1, std::min<long long>(n, used / 8));
+    long long left = K;
+    double top_uses = 0;
+    for (int v = maxc; v >= 1 && left > 0; --v) {
+        const long long k = std::min<long long>(left, nwith[(size_t)v]);
+        top_uses += (double)v * (double)k;
+        left -= k;
     }
-    std::vector<long long> pos((size_t)maxc + 2, 0);  // first output position per count
-    long long acc = 0;
-    for (int v = maxc; v >= std::max(part, 2); --v) pos[(size_t)v] = acc, acc += nwith[(size_t)v];
-    std::vector<int> hc((size_t)n);
-    long long part_used = 0;
-    for (int c = 0; c < N; ++c) {
-        const int v = h[(size_t)c];
-        if (v >= t || (v == part && v >= 2 && part_used < part_take)) {
-            if (v == part && v < t) ++part_used;
-            hc[(size_t)pos[(size_t)v]++] = c;
-        }
-    }
+    return top_uses * (double)used >= min_skew * (double)K * (double)nnz;
+}
+
+// Upload the ordered hot columns and renumber them in the layout.
+static hipError_t finish_hot(const std::vector<int> &hc, int N, int *colmap, int *hot_cols, int *cols, long long L,
+                             int *n_hot, hipStream_t st) {
+    const int n = (int)hc.size();
+    hipError_t e;
     if ((e = hipMemcpyAsync(hot_cols, hc.data(), (size_t)n * 4, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
     if ((e = hipMemsetAsync(colmap, 0xff, (size_t)N * 4, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(hot_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, st, hot_cols, n, colmap);
@@ -483,6 +536,111 @@ hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, 
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
     *n_hot = n;
     return hipSuccess;
+}
+
+// The same selection from all the counts copied to the host (small matrices,
+// and counts beyond the histogram's range).
+static hipError_t hot_columns_host(const int *cnt, int N, int max_hot, double min_skew, long long nnz, int *colmap,
+                                   int *hot_cols, int *cols, long long L, int *n_hot, hipStream_t st) {
+    hipError_t e;
+    std::vector<int> h((size_t)N);
+    if ((e = hipMemcpyAsync(h.data(), cnt, (size_t)N * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    int maxc = 0;
+    for (int c = 0; c < N; ++c) maxc = std::max(maxc, h[(size_t)c]);
+    std::vector<long long> nwith((size_t)maxc + 2, 0);
+    for (int c = 0; c < N; ++c) ++nwith[(size_t)h[(size_t)c]];
+    const Selection sel = select_hot(nwith, maxc, max_hot);
+    if (sel.n == 0) return hipSuccess;
+    if (min_skew > 0 && !skewed(nwith, maxc, N, nnz, sel.n, min_skew)) return hipSuccess;
+    std::vector<long long> pos((size_t)maxc + 2, 0);
+    long long acc = 0;
+    for (int v = maxc; v >= std::max(sel.part, 2); --v) pos[(size_t)v] = acc, acc += nwith[(size_t)v];
+    std::vector<int> hc((size_t)sel.n);
+    long long part_used = 0;
+    for (int c = 0; c < N; ++c) {
+        const int v = h[(size_t)c];
+        if (v >= sel.t || (v == sel.part && v >= 2 && part_used < sel.part_take)) {
+            if (v == sel.part && v < sel.t) ++part_used;
+            hc[(size_t)pos[(size_t)v]++] = c;
+        }
+    }
+    return finish_hot(hc, N, colmap, hot_cols, cols, L, n_hot, st);
+}
+
+hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, double min_skew, int *cnt, int *colmap,
+                              int *hot_cols, int *cols, long long L, int *n_hot, hipStream_t st) {
+    *n_hot = 0;
+    hipError_t e;
+    if ((e = hipMemsetAsync(cnt, 0, (size_t)N * 4, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(col_hist_kernel, dim3(2048), dim3(256), 0, st, ci, nnz, cnt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // histogram of the counts (colmap is free until the scatter below)
+    int *hist = colmap;
+    if (N < kCountBins) return hot_columns_host(cnt, N, max_hot, min_skew, nnz, colmap, hot_cols, cols, L, n_hot, st);
+    if ((e = hipMemsetAsync(hist, 0, (size_t)kCountBins * 4, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(count_hist_kernel, dim3(1024), dim3(256), 0, st, cnt, N, hist);
+    std::vector<long long> nwith(kCountBins, 0);
+    {
+        std::vector<int> h32(kCountBins);
+        if ((e = hipMemcpyAsync(h32.data(), hist, (size_t)kCountBins * 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
+            return e;
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+        for (int v = 0; v < kCountBins; ++v) nwith[(size_t)v] = h32[(size_t)v];
+    }
+    int maxc = kCountBins - 1;
+    while (maxc > 0 && nwith[(size_t)maxc] == 0) --maxc;
+    // columns used kCountBins - 1 times or more share the last bin: they are
+    // few, and are taken first, ordered by their own counts below
+    const Selection sel = select_hot(nwith, maxc, max_hot);
+    if (sel.n == 0) return hipSuccess;
+    if (sel.t >= kCountBins || sel.part >= kCountBins - 1)  // the last bin not taken whole
+        return hot_columns_host(cnt, N, max_hot, min_skew, nnz, colmap, hot_cols, cols, L, n_hot, st);
+    if (min_skew > 0 && !skewed(nwith, maxc, N, nnz, sel.n, min_skew)) return hipSuccess;
+    // the candidates (count >= the lowest count taken), in column order
+    const int lo = sel.part_take > 0 ? sel.part : sel.t;
+    const int nb = (N + kCompact - 1) / kCompact;
+    int *bcount = hist;  // reuse: nb <= N / 1024 + 1 ints
+    hipLaunchKernelGGL(compact_count_kernel, dim3(nb), dim3(256), 0, st, cnt, N, lo, bcount);
+    std::vector<int> boff((size_t)nb + 1, 0);
+    if ((e = hipMemcpyAsync(boff.data() + 1, bcount, (size_t)nb * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    for (int b = 0; b < nb; ++b) boff[(size_t)b + 1] += boff[(size_t)b];
+    const int m = boff[(size_t)nb];
+    int2 *cand = nullptr;
+    if ((e = hipMalloc(&cand, (size_t)std::max(m, 1) * 8)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(bcount, boff.data(), (size_t)nb * 4, hipMemcpyHostToDevice, st)) == hipSuccess) {
+        hipLaunchKernelGGL(compact_write_kernel, dim3(nb), dim3(256), 0, st, cnt, N, lo, bcount, cand);
+        e = hipGetLastError();
+    }
+    std::vector<int2> hcand((size_t)m);
+    if (e == hipSuccess) e = hipMemcpyAsync(hcand.data(), cand, (size_t)m * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(cand);
+    if (e != hipSuccess) return e;
+    // stable counting sort of the candidates by count, descending; the
+    // saturated-bin columns first, sorted by (count descending, column)
+    std::vector<int2> big;
+    for (const int2 &cv : hcand)
+        if (cv.y >= kCountBins - 1) big.push_back(cv);
+    std::sort(big.begin(), big.end(), [](const int2 &a, const int2 &b) { return a.y != b.y ? a.y > b.y : a.x < b.x; });
+    std::vector<long long> pos((size_t)kCountBins, 0);
+    long long acc = (long long)big.size();
+    for (int v = std::min(maxc, kCountBins - 2); v >= lo; --v) pos[(size_t)v] = acc, acc += nwith[(size_t)v];
+    std::vector<int> hc((size_t)sel.n);
+    for (size_t i = 0; i < big.size(); ++i) hc[i] = big[i].x;
+    long long part_used = 0;
+    for (const int2 &cv : hcand) {
+        const int v = cv.y;
+        if (v >= kCountBins - 1) continue;
+        if (v >= sel.t) {
+            hc[(size_t)pos[(size_t)v]++] = cv.x;
+        } else if (part_used < sel.part_take) {  // v == part: the first part_take in column order
+            ++part_used;
+            hc[(size_t)pos[(size_t)v]++] = cv.x;
+        }
+    }
+    return finish_hot(hc, N, colmap, hot_cols, cols, L, n_hot, st);
 }
 
 // Host-side launchers --------------------------------------------------------

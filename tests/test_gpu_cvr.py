@@ -100,6 +100,57 @@ def test_hot_x_staging_decodes_and_is_bit_identical(P, O, mats, hot_kb, lds_kb):
         assert np.array_equal(y, O.cvr64_spmv(A, emu, x)), name
 
 
+def _check_hot_ranking(A, hot, budget):
+    counts = np.bincount(A.cols, minlength=A.numCols)
+    assert len(hot) <= budget
+    key = np.stack([-counts[hot], hot])
+    assert np.all((np.diff(key[0]) > 0) | ((np.diff(key[0]) == 0) & (np.diff(key[1]) > 0)))
+    assert counts[hot].min() >= 2
+    rest = np.setdiff1d(np.nonzero(counts >= 2)[0], hot)
+    if len(rest):
+        last = (counts[hot[-1]], hot[-1])
+        assert not np.any((counts[rest] > last[0]) | ((counts[rest] == last[0]) & (rest < last[1])))
+    if len(hot) < budget:  # a short selection takes every column used twice or more
+        assert len(rest) == 0
+
+
+@pytest.mark.parametrize("hot_kb", [64, 2048])
+def test_hot_x_selection_on_device(P, O, hot_kb):
+    """Matrices with >= 65,536 columns select the hot columns from a device
+    histogram of the counts: same ranking contract, including columns used
+    65,535 times or more (ordered by their own counts) and a budget that cuts
+    through a count value."""
+    rng = np.random.default_rng(21)
+    M, N = 72000, 100000
+    # columns 0, 1, 2 in most rows; then k = 0..len-1 distinct columns
+    # 3 + (start + 4732 k) mod (N - 3) with a random start per row
+    lens = rng.integers(3, 12, M)
+    start = rng.integers(0, N - 3, M)
+    r = np.arange(M)
+    fixed = np.stack([np.zeros(M), np.where(r < 69000, 1, -1), np.where(r % 12 < 11, 2, -1)], 1)
+    k = np.arange(12)
+    rest = 3 + (start[:, None] + 4732 * k[None, :]) % (N - 3)
+    rest = np.where(k[None, :] < lens[:, None], rest, -1)
+    allc = np.sort(np.concatenate([fixed, rest], 1).astype(np.int64), 1)
+    keep = allc >= 0
+    rp = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int32)
+    cols = allc[keep].astype(np.int32)
+    A = P.CSR(M, N, rp, cols, rng.uniform(-1, 1, len(cols)))
+    big = np.bincount(cols, minlength=N)[:3]
+    assert np.all(big >= 65535)
+    for name, B in [("synthetic", A), ("rmat17", P.rmat(17, 16, seed=8))]:
+        x = P.fill_uniform(B.numCols, 4)
+        with P.CvrHip(B, steps_per_chunk=64, x_hot_kb=hot_kb) as h:
+            hot = h.hot_cols()
+            y = h.spmv_host(x)
+        _say(f"device hot selection {name} hot_kb={hot_kb}: {len(hot)} hot columns")
+        assert len(hot) > 0
+        _check_hot_ranking(B, hot, hot_kb * 1024 // 8)
+        if name == "synthetic":
+            assert list(hot[:3]) == [0, 1, 2]
+        assert np.array_equal(y, O.cvr64_spmv(B, O.cvr64_build(B, 64, 0), x)), name
+
+
 def test_automatic_hot_x_only_on_skewed_column_use(P, O):
     """x_hot_kb = -1 stages the hot columns of a power-law matrix and nothing
     for meshes whose columns are used evenly; y is the emulation's either way."""
