@@ -2,11 +2,10 @@
 // for gfx950, single matrix or a persistent batched launch.
 //
 // Same CVR-64 layout and schedule as the SpMV (cvr_spmv.hip); each nonzero
-// scales one row of X (k contiguous FP32 values) into k accumulators. For
-// k <= 16 the accumulators are held 4 per thread by the KP/4 consecutive
-// threads that serve a CVR lane (a row of X is one coalesced request, a chunk
-// runs on KP/4 waves); for k > 16 one lane holds all 32 and loads its row with
-// 8 16-byte loads. X and Y are FP32, row-major
+// scales one row of X (k contiguous FP32 values) into k accumulators, held 4
+// per thread by the KP/4 consecutive threads that serve a CVR lane (KP = k
+// rounded up to 8, 16 or 32): a row of X is one coalesced request, and a chunk
+// runs on the KP/4 waves of a workgroup. X and Y are FP32, row-major
 // (X: numCols x k, Y: numRows x k). Accumulation is FP64: the product of two
 // FP32 numbers is exact in FP64 and the FP64 FMA issues at the same VALU rate
 // as the (non-packed) FP32 FMA on CDNA4, so every output is the FP64 sum of
@@ -37,6 +36,18 @@ __device__ __forceinline__ void wave_sync_lds_m() {
 
 constexpr int kBatchQueues = 8;  // queue ranges of the batched launch (one per XCD); counters in use
 
+// The layout, X and Y live in global memory. Pointers read from a descriptor
+// in memory (the batched path) are generic to the compiler, which then emits
+// flat accesses and waits on both counters around them; these casts make the
+// address space explicit.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T *p) {
+    return reinterpret_cast<gptr<T>>(reinterpret_cast<uintptr_t>(p));
+}
+typedef float vfloat4 __attribute__((ext_vector_type(4)));  // 16-byte loads and stores of X and Y rows
+
 template <typename VT>
 __device__ __forceinline__ double val_f64(VT v);
 template <>
@@ -57,6 +68,19 @@ __device__ __forceinline__ double val_f64<unsigned short>(unsigned short v) {
 // segment its rank in the chunk's segment list) and a private copy of each
 // 8-step group's segment entries, staged in LDS one group ahead as in the
 // SpMV. `slots` (max_slots x KP doubles) is shared by the workgroup.
+constexpr int kWaitVm0m = 0x0f70;  // s_waitcnt vmcnt(0), other counters unconstrained
+
+// Columns [j0, j0 + 4) of an X row when k is not the block width: element by
+// element (kept out of line so the 16-byte load of full rows stays one load).
+__device__ __noinline__ float4 xpart_partial(gptr<const float> p, int j0, int k) {
+    float4 q;
+    q.x = j0 < k ? p[0] : 0.f;
+    q.y = j0 + 1 < k ? p[1] : 0.f;
+    q.z = j0 + 2 < k ? p[2] : 0.f;
+    q.w = j0 + 3 < k ? p[3] : 0.f;
+    return q;
+}
+
 template <typename VT, int KP>
 __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const int lane, double *slots,
                            int *segbuf) {
@@ -70,9 +94,12 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
     const size_t base = (size_t)C * (size_t)c;
     const int steps = (c == a.nchunks - 1) ? (int)((a.layout_elems - (long long)base) >> 6) : a.S;
     const int zr = a.chunk_rows[c];
-    const int *__restrict__ segs = a.seg + zr + kSegExtra * c;
-    const VT *__restrict__ vp = reinterpret_cast<const VT *>(a.vals) + base + cl;
-    const int *__restrict__ cpa = a.cols + base + lane;  // the step's 64 columns, one per thread
+    const gptr<const int> segs = to_global(a.seg + zr + kSegExtra * c);
+    const gptr<const VT> vp = to_global(reinterpret_cast<const VT *>(a.vals) + base + cl);
+    const gptr<const int> cpa = to_global(a.cols + base + lane);  // the step's 64 columns, one per thread
+    const gptr<const float> X = to_global(a.X);
+    const gptr<const int> hot_cols = to_global(a.hot_cols);
+    const gptr<float> Y = to_global(a.Y);
     const int k = a.k;
     const bool full = (k == KP);
 
@@ -80,18 +107,32 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
 #pragma unroll
         for (int i = 0; i < G; ++i) cc[i] = __builtin_nontemporal_load(&cpa[(size_t)(g0 + i) * 64]);
     };
-    // this thread's 4 values of the X row of stored column cv
-    auto xpart = [&](int cv) -> float4 {
-        int col = cv & kColMask;
-        if (cv & kHotFlag) col = a.hot_cols[col];
-        const float *p = a.X + (size_t)col * k + j0;
-        if (full) return *reinterpret_cast<const float4 *>(p);
-        float4 q;
-        q.x = j0 < k ? p[0] : 0.f;
-        q.y = j0 + 1 < k ? p[1] : 0.f;
-        q.z = j0 + 2 < k ? p[2] : 0.f;
-        q.w = j0 + 3 < k ? p[3] : 0.f;
-        return q;
+    // this thread's 4 values of the X rows of a group's stored columns: the
+    // hot-column lookups (layouts with hot-x renumbering) are issued first, so
+    // the X loads wait for them once, not each for the loads before it
+    const bool has_hot = a.hot_cols != nullptr;
+    auto xgroup = [&](const int (&cc)[G], float4 (&xo)[G]) {
+        int col[G];
+#pragma unroll
+        for (int i = 0; i < G; ++i) col[i] = __shfl(cc[i], cl);
+        if (has_hot) {
+#pragma unroll
+            for (int i = 0; i < G; ++i)
+                col[i] = (col[i] & kHotFlag) ? hot_cols[col[i] & kColMask] : (col[i] & kColMask);
+        } else {
+#pragma unroll
+            for (int i = 0; i < G; ++i) col[i] &= kColMask;
+        }
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+            const gptr<const float> p = X + (size_t)col[i] * k + j0;
+            if (full) {
+                const vfloat4 q = *reinterpret_cast<gptr<const vfloat4>>(p);
+                xo[i] = make_float4(q.x, q.y, q.z, q.w);
+            } else {
+                xo[i] = xpart_partial(p, j0, k);
+            }
+        }
     };
     auto group_ends = [&](const int (&cc)[G]) -> int {
         int e = 0;
@@ -115,11 +156,13 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
     load_cols(0, cb);
 #pragma unroll
     for (int i = 0; i < G; ++i) vb[i] = __builtin_nontemporal_load(&vp[(size_t)i * 64]);
-#pragma unroll
-    for (int i = 0; i < G; ++i) xb[i] = xpart(__shfl(cb[i], cl));
+    xgroup(cb, xb);
     int ecur = group_ends(cb);
     load_segs(0, ecur, sc);
     if (G < steps) load_cols(G, cn);
+    // the loop is entered with nothing in flight (its wait counts then follow
+    // its own loads only)
+    __builtin_amdgcn_s_waitcnt(kWaitVm0m);
     int sbase = 0, segp = 0;
 
     for (int g = 0; g < steps; g += G) {
@@ -127,8 +170,7 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
         const int sbase_next = sbase + ecur;
         int enext = 0;
         if (has1) {
-#pragma unroll
-            for (int i = 0; i < G; ++i) xn[i] = xpart(__shfl(cn[i], cl));
+            xgroup(cn, xn);
             enext = group_ends(cn);
             load_segs(sbase_next, enext, sn);
 #pragma unroll
@@ -153,10 +195,10 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
                     const int rank = __popcll(m & ((1ull << cl) - 1ull));
                     const int d = segbuf[segp - sbase + rank];
                     if (d >= 0) {
-                        float *yr = a.Y + (size_t)d * k + j0;
+                        const gptr<float> yr = Y + (size_t)d * k + j0;
                         if (full) {
-                            *reinterpret_cast<float4 *>(yr) =
-                                make_float4((float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
+                            const vfloat4 q = {(float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]};
+                            *reinterpret_cast<gptr<vfloat4>>(yr) = q;
                         } else {
 #pragma unroll
                             for (int j = 0; j < 4; ++j)
@@ -229,24 +271,21 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
     __syncthreads();
 }
 
-// Threads per workgroup: KP/4 waves (one chunk) for thread groups, one wave
-// for the lane-per-nonzero scheme of KP = 32.
+// Threads per workgroup: KP/4 waves, one chunk.
 template <int KP>
-constexpr int spmm_threads() { return KP >= 32 ? 64 : 16 * KP; }
+constexpr int spmm_threads() { return 16 * KP; }
+static inline int spmm_block(int kp) { return 16 * kp; }
 
 template <typename VT, int KP>
 __device__ __forceinline__ void spmm_any_chunk(const SpmmDesc &a, int c, double *smem, int *segbuf) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if constexpr (KP >= 32)
-        spmm_chunk_lane<VT, KP>(a, c, lane, smem);
-    else
-        spmm_chunk<VT, KP>(a, c, wv, lane, smem, segbuf + wv * 64 * kStepGroup);
+    spmm_chunk<VT, KP>(a, c, wv, lane, smem, segbuf + wv * 64 * kStepGroup);
 }
 
 template <typename VT, int KP>
 __global__ __launch_bounds__(spmm_threads<KP>()) void spmm_kernel(SpmmDesc a) {
     extern __shared__ double smem[];
-    __shared__ int segbuf[KP >= 32 ? 1 : KP / 4][64 * kStepGroup];
+    __shared__ int segbuf[KP / 4][64 * kStepGroup];
     const int nb = gridDim.x;
     // XCD-aware: each XCD gets a contiguous chunk range
     const int q = nb >> 3, r = nb & 7, x = blockIdx.x & 7, i = blockIdx.x >> 3;
@@ -266,7 +305,7 @@ __global__ __launch_bounds__(spmm_threads<KP>()) void spmm_batch_kernel(const Sp
                                                                         const int2 *__restrict__ queue, int total,
                                                                         int *__restrict__ counter) {
     extern __shared__ double smem[];
-    __shared__ int segbuf[KP >= 32 ? 1 : KP / 4][64 * kStepGroup];
+    __shared__ int segbuf[KP / 4][64 * kStepGroup];
     __shared__ int item;
     const int nq = (int)gridDim.x < kBatchQueues ? (int)gridDim.x : kBatchQueues;
     const int own = (int)blockIdx.x % nq;
@@ -280,165 +319,14 @@ __global__ __launch_bounds__(spmm_threads<KP>()) void spmm_batch_kernel(const Sp
             __syncthreads();  // every thread has read `item` before it is overwritten
             if (idx >= q1) break;
             const int2 w = queue[idx];
-            spmm_any_chunk<VT, KP>(descs[w.x], w.y, smem, &segbuf[0][0]);
+            // a register copy: fields read through a pointer into global
+            // memory would be reloaded after every store (and be flat loads)
+            const SpmmDesc d = descs[w.x];
+            spmm_any_chunk<VT, KP>(d, w.y, smem, &segbuf[0][0]);
         }
     }
 }
 
-// Loads row `col` of X (k <= KP floats) into xv[0..KP).
-template <int KP>
-__device__ __forceinline__ void load_xrow(const float *__restrict__ X, int k, int col, float (&xv)[KP]) {
-    const float *p = X + (size_t)col * k;
-    if (k == KP) {
-#pragma unroll
-        for (int j = 0; j < KP; j += 4) {
-            const float4 q = *reinterpret_cast<const float4 *>(p + j);
-            xv[j] = q.x, xv[j + 1] = q.y, xv[j + 2] = q.z, xv[j + 3] = q.w;
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < KP; ++j) xv[j] = (j < k) ? p[j] : 0.f;
-    }
-}
-
-// Wide blocks (KP = 32): one chunk of one matrix on one wave, each lane
-// loading the whole X row of its nonzero (KP/4 16-byte loads, two steps in
-// flight) into its own KP accumulators. Measured faster than thread groups at
-// this width: every lane keeps 16 independent loads in flight. `slots` is the
-// wave's LDS area of max_slots * KP doubles.
-template <typename VT, int KP>
-__device__ void spmm_chunk_lane(const SpmmDesc &a, int c, int lane, double *slots) {
-    const long long C = 64LL * a.S;
-    const size_t base = (size_t)C * (size_t)c;
-    const int steps = (c == a.nchunks - 1) ? (int)((a.layout_elems - (long long)base) >> 6) : a.S;
-    const int zr = a.chunk_rows[c];
-    const int *__restrict__ segs = a.seg + zr + kSegExtra * c;
-    const int k = a.k;
-    int w0 = segs[lane], w1 = segs[64 + lane];
-    int wbase = 0, segp = 0;
-    const VT *__restrict__ vp = reinterpret_cast<const VT *>(a.vals) + base + lane;
-    const int *__restrict__ cp = a.cols + base + lane;
-
-    double acc[KP];
-#pragma unroll
-    for (int j = 0; j < KP; ++j) acc[j] = 0.0;
-
-    constexpr int U = (KP >= 32) ? 2 : 4;  // steps whose X rows are in flight together
-    VT vb[U];
-    int cb[U];
-#pragma unroll
-    for (int i = 0; i < U; ++i) {
-        vb[i] = vp[i * 64];
-        cb[i] = cp[i * 64];
-    }
-    for (int g = 0; g < steps; g += U) {
-        float xv[U][KP];
-#pragma unroll
-        for (int i = 0; i < U; ++i) {
-            const int cv = cb[i];
-            const int col = (cv & kHotFlag) ? a.hot_cols[cv & kColMask] : (cv & kColMask);
-            load_xrow<KP>(a.X, k, col, xv[i]);
-        }
-        VT vn[U];
-        int cn[U];
-        const bool more = (g + U) < steps;
-        if (more) {
-#pragma unroll
-            for (int i = 0; i < U; ++i) {
-                vn[i] = vp[(size_t)(g + U + i) * 64];
-                cn[i] = cp[(size_t)(g + U + i) * 64];
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < U; ++i) {
-            const double v = val_f64<VT>(vb[i]);
-#pragma unroll
-            for (int j = 0; j < KP; ++j) acc[j] = __builtin_fma(v, (double)xv[i][j], acc[j]);
-            const bool end = cb[i] < 0;
-            const unsigned long long m = __ballot(end);
-            if (m) {
-                const int rel = segp - wbase + mbcnt64m(m);
-                const int d0 = __shfl(w0, rel & 63);
-                const int d1 = __shfl(w1, rel & 63);
-                const int d = rel < 64 ? d0 : d1;
-                if (end) {
-                    if (d >= 0) {
-                        float *yr = a.Y + (size_t)d * k;
-                        if (k == KP) {
-#pragma unroll
-                            for (int j = 0; j < KP; j += 4)
-                                *reinterpret_cast<float4 *>(yr + j) =
-                                    make_float4((float)acc[j], (float)acc[j + 1], (float)acc[j + 2], (float)acc[j + 3]);
-                        } else {
-#pragma unroll
-                            for (int j = 0; j < KP; ++j)
-                                if (j < k) yr[j] = (float)acc[j];
-                        }
-                    } else {
-                        double *sl = slots + (size_t)(~d) * KP;
-#pragma unroll
-                        for (int j = 0; j < KP; ++j) sl[j] = acc[j];
-                    }
-#pragma unroll
-                    for (int j = 0; j < KP; ++j) acc[j] = 0.0;
-                }
-                segp += __popcll(m);
-                if (segp - wbase >= 64) {
-                    w0 = w1;
-                    w1 = segs[wbase + 128 + lane];
-                    wbase += 64;
-                }
-            }
-        }
-        if (more) {
-#pragma unroll
-            for (int i = 0; i < U; ++i) {
-                vb[i] = vn[i];
-                cb[i] = cn[i];
-            }
-        }
-    }
-
-    // split rows: (group, column) pairs over the lanes, slots summed in element order
-    wave_sync_lds_m();
-    const int ns = a.split_n[c];
-    const int2 *tab = a.split_tab + (size_t)c * kSplitCap;
-    for (int t = lane; t < ns * KP; t += 64) {
-        const int gi = t / KP, j = t % KP;
-        if (j >= k) continue;
-        const int2 e = tab[gi];
-        const int se = tab[gi + 1].y;
-        double s = slots[(size_t)e.y * KP + j];
-        for (int q = e.y + 1; q < se; ++q) s += slots[(size_t)q * KP + j];
-        if (e.x >= 0)
-            a.Y[(size_t)e.x * k + j] = (float)s;
-        else if (e.x == kCodeHead)
-            a.carry[(size_t)(2 * c) * k + j] = s;
-        else if (e.x == kCodeTail)
-            a.carry[(size_t)(2 * c + 1) * k + j] = s;
-    }
-    // empty rows of the chunk's row range
-    if (a.has_empty) {
-        const int r0 = zr, r1 = a.chunk_rows[c + 1];
-        if (r1 > r0) {
-            const int wlo = r0 >> 5, whi = (r1 - 1) >> 5;
-            for (int w = wlo + lane; w <= whi; w += 64) {
-                unsigned bits = a.empty_bits[w];
-                const int rb = w << 5;
-                if (rb < r0) bits &= ~0u << (r0 - rb);
-                if (rb + 32 > r1) bits &= (r1 - rb >= 32) ? ~0u : ((1u << (r1 - rb)) - 1u);
-                while (bits) {
-                    const int jj = __ffs(bits) - 1;
-                    float *yr = a.Y + (size_t)(rb + jj) * k;
-                    for (int j = 0; j < k; ++j) yr[j] = 0.f;
-                    bits &= bits - 1;
-                }
-            }
-        }
-    }
-    // the LDS slots are reused by the wave's next chunk (batched path)
-    wave_sync_lds_m();
-}
 
 
 // Cross-chunk rows of one matrix, all k columns; chunk order.
@@ -473,7 +361,7 @@ __global__ void spmm_batch_fixup_kernel(const SpmmDesc *__restrict__ descs, cons
 // ---------------------------------------------------------------- launchers
 template <typename VT>
 static hipError_t launch_spmm_vt(const SpmmDesc &a, int kp, size_t lds, hipStream_t st) {
-    dim3 grid(a.nchunks), block(kp >= 32 ? 64 : 16 * kp);
+    dim3 grid(a.nchunks), block(spmm_block(kp));
     switch (kp) {
     case 8: hipLaunchKernelGGL((spmm_kernel<VT, 8>), grid, block, lds, st, a); break;
     case 16: hipLaunchKernelGGL((spmm_kernel<VT, 16>), grid, block, lds, st, a); break;
@@ -518,7 +406,7 @@ hipError_t spmm_batch_grid(int prec, int k, size_t lds, int *grid) {
         f = kp == 8 ? batch_kernel_ptr<unsigned short, 8>()
                     : kp == 16 ? batch_kernel_ptr<unsigned short, 16>() : batch_kernel_ptr<unsigned short, 32>();
     int per_cu = 0, dev = 0, ncu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kp >= 32 ? 64 : 16 * kp, lds);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, spmm_block(kp), lds);
     if (e != hipSuccess) return e;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
@@ -530,7 +418,7 @@ hipError_t launch_spmm_batch(const SpmmDesc *d_descs, const int2 *d_queue, int t
                              size_t lds, int grid, hipStream_t st) {
     if (total == 0) return hipSuccess;
     const int kp = spmm_kp(k);
-    dim3 g(grid), b(kp >= 32 ? 64 : 16 * kp);
+    dim3 g(grid), b(spmm_block(kp));
 #define PAVA_BATCH(VT)                                                                                          \
     switch (kp) {                                                                                              \
     case 8: hipLaunchKernelGGL((spmm_batch_kernel<VT, 8>), g, b, lds, st, d_descs, d_queue, total, d_counter); break;  \
