@@ -172,6 +172,11 @@ __global__ __launch_bounds__(64 * kWaves) void spmv_kernel(SpmvArgs a) {
     }
 #pragma unroll
     for (int i = 0; i < SIGMA; ++i) xv[i] = a.x[c[i]];
+    // the rows of the lane's first two row starts, loaded with the gathers
+    // (most lanes start at most two rows; later ones are loaded when reached)
+    const int nst = __popc(f);
+    const int r0 = nst > 0 ? a.nerows[k] : 0;
+    const int r1 = nst > 1 ? a.nerows[k + 1] : 0;
 
     double acc = 0.0, first = 0.0;
     int nseg = 0, row = -1;
@@ -183,7 +188,7 @@ __global__ __launch_bounds__(64 * kWaves) void spmv_kernel(SpmvArgs a) {
                     first = acc;
                 else
                     a.y[row] = acc;  // a row wholly inside the lane
-                row = a.nerows[k++];
+                row = nseg == 0 ? r0 : nseg == 1 ? r1 : a.nerows[k + nseg];
                 ++nseg;
                 acc = 0.0;
             }
