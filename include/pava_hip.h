@@ -246,6 +246,18 @@ int csr_hip_create(const pava_csr_matrix *A, int lanes_per_row, int precision, i
 int csr_hip_spmv(csr_hip_handle *h, const void *d_x, void *d_y, void *stream);
 int csr_hip_destroy(csr_hip_handle *h);
 
+/* "coo_hip": COO SpMV, FP64. The CSR's nonzeros in row order with a row index
+ * per nonzero (the conversion expands the row pointers on the GPU, as
+ * convertCSR2COO does on the CPU, utils/pava_formats.cpp:147+). The GPU
+ * stand-in for the MKL COO block benchmark_COO_SpMV (utils/pava_formats.cpp:
+ * 50-61). One wave per tile of 512 nonzeros, segmented sums in a fixed order:
+ * deterministic; y is written completely (rows without nonzeros get 0). */
+typedef struct coo_hip_handle coo_hip_handle;
+int coo_hip_create(const pava_csr_matrix *A, int device, coo_hip_handle **out);
+int coo_hip_spmv(coo_hip_handle *h, const double *d_x, double *d_y, void *stream);
+int coo_hip_get_info(coo_hip_handle *h, int *ntiles, double *convert_ms);
+int coo_hip_destroy(coo_hip_handle *h);
+
 /* "csr5_hip": CSR5 (csr5/detail/avx512/format_avx512.h, csr5_spmv_avx512.h;
  * anonymouslibHandle::asCSR5 / spmv, csr5/anonymouslib_avx512.h:107-221)
  * re-derived for 64-lane wavefronts: tiles of 64 x sigma nonzeros stored
@@ -329,6 +341,12 @@ int conduct_cvr_hip(int numRows, int numCols, int nItems_raw, int *h_rowDelimite
 int conduct_csr_hip(int numRows, int numCols, int nItems, int *h_rowDelimiters, int *h_cols, double *h_val,
                     double *h_vec, double *refOut, int lanes_per_row, char *matrixName, int numIterations);
 
+/* The coo_hip block (same protocol, report format "COO_HIP", thread column =
+ * tiles); stands in for the MKL COO block (main.cpp COO block,
+ * utils/pava_formats.cpp:50-61). */
+int conduct_coo_hip(int numRows, int numCols, int nItems, int *h_rowDelimiters, int *h_cols, double *h_val,
+                    double *h_vec, double *refOut, char *matrixName, int numIterations);
+
 /* conduct_csr5 (csr5/csr5.h:72-179) on the GPU, same arguments (alpha is a
  * double there and ignored here as there): conversion (timed, sigma 16), one
  * verification SpMV into y (written completely), numIterations timed SpMVs,
@@ -337,7 +355,7 @@ int conduct_csr5_hip(int m, int n, int nnz, int *csrRowPtr, int *csrColIdx, doub
                      double *y, double alpha, char *matrixName, int numIterations);
 
 /* benchFile with the GPU format registry: read, size filters (main.cpp:82-99),
- * x = i%3, referenceSpMV, then each format block in turn (cvr_hip, csr5_hip, csr_hip)
+ * x = i%3, referenceSpMV, then each format block in turn (cvr_hip, csr5_hip, csr_hip, coo_hip)
  * with checkResults. Returns the total error count (>= 0), -1 if the matrix
  * was skipped, or a PAVA_HIP_E* code. */
 int pava_bench_file(const char *path, int numIterations);

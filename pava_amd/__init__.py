@@ -84,7 +84,8 @@ EXPORTED = [
     "pava_mlp_create", "pava_mlp_num_params", "pava_mlp_get_params", "pava_mlp_set_params", "pava_mlp_predict",
     "pava_mlp_loss_grad", "pava_mlp_train", "pava_mlp_destroy", "cvr_hip_export_hot", "pava_write_csr_binary",
     "pava_read_csr_binary", "conduct_csr_hip", "csr5_hip_create", "csr5_hip_spmv", "csr5_hip_spmv_host",
-    "csr5_hip_get_info", "csr5_hip_export_layout", "csr5_hip_destroy", "conduct_csr5_hip",
+    "csr5_hip_get_info", "csr5_hip_export_layout", "csr5_hip_destroy", "conduct_csr5_hip", "coo_hip_create",
+    "coo_hip_spmv", "coo_hip_get_info", "coo_hip_destroy", "conduct_coo_hip",
 ]
 
 
@@ -512,6 +513,57 @@ def conduct_csr5_hip(A: CSR, x: np.ndarray, name: str = "matrix", iterations: in
     _check(L.conduct_csr5_hip(A.numRows, A.numCols, A.nnz, _ptr(A.rowOffsets), _ptr(A.cols), _ptr(A.vals),
                               _ptr(x), _ptr(y), 1.0, name.encode(), iterations))
     return y
+
+
+def conduct_coo_hip(A: CSR, x: np.ndarray, name: str = "matrix", iterations: int = 10) -> np.ndarray:
+    """The COO_HIP driver block: converts, verifies into y, times, prints the report line."""
+    L = lib()
+    L.conduct_coo_hip.argtypes = [C.c_int] * 3 + [C.c_void_p] * 5 + [C.c_char_p, C.c_int]
+    y = np.empty(A.numRows)
+    x = np.ascontiguousarray(x, np.float64)
+    _check(L.conduct_coo_hip(A.numRows, A.numCols, A.nnz, _ptr(A.rowOffsets), _ptr(A.cols), _ptr(A.vals),
+                             _ptr(x), _ptr(y), name.encode(), iterations))
+    return y
+
+
+class CooHip:
+    """The coo_hip format: COO SpMV (row index per nonzero), FP64, deterministic."""
+
+    def __init__(self, A: CSR, device: int = 0):
+        L = lib()
+        L.coo_hip_create.argtypes = [C.POINTER(_CSR), C.c_int, C.POINTER(C.c_void_p)]
+        L.coo_hip_spmv.argtypes = [C.c_void_p] * 4
+        L.coo_hip_get_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.coo_hip_destroy.argtypes = [C.c_void_p]
+        self._h = C.c_void_p()
+        s = A.c_struct()
+        _check(L.coo_hip_create(C.byref(s), device, C.byref(self._h)))
+        self.shape = (A.numRows, A.numCols)
+
+    def info(self) -> dict:
+        nt, ms = C.c_int(), C.c_double()
+        _check(lib().coo_hip_get_info(self._h, C.byref(nt), C.byref(ms)))
+        return {"ntiles": nt.value, "convert_ms": ms.value}
+
+    def spmv(self, x_ptr: int, y_ptr: int, stream: int = 0) -> None:
+        _check(lib().coo_hip_spmv(self._h, x_ptr, y_ptr, stream))
+
+    def close(self) -> None:
+        if self._h:
+            lib().coo_hip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class CsrHip:

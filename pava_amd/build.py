@@ -20,7 +20,7 @@ LIB = HERE / "libpava_hip.so"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
 HIP_SOURCES = ["cvr_convert.hip", "cvr_spmv.hip", "cvr_spmm.hip", "cvr_handle.hip", "csr_vector.hip", "csr5_hip.hip",
-               "mlp.hip", "driver.hip"]
+               "coo_hip.hip", "mlp.hip", "driver.hip"]
 CXX_SOURCES = ["host_io.cpp", "gen.cpp", "features.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}", "-Wall", "-Wno-unused-function"]

@@ -113,6 +113,49 @@ extern "C" int conduct_csr_hip(int numRows, int numCols, int nItems, int *h_rowD
     return done(0);
 }
 
+// The coo_hip block: conversion (upload and row expansion, timed), one
+// verification SpMV into refOut, numIterations timed SpMVs, the report line
+// with format "COO_HIP" and thread column = tiles. Stands in for pava's MKL
+// COO block (utils/pava_formats.cpp:50-61).
+extern "C" int conduct_coo_hip(int numRows, int numCols, int nItems, int *h_rowDelimiters, int *h_cols,
+                               double *h_val, double *h_vec, double *refOut, char *matrixName, int numIterations) {
+    pava_csr_matrix A{nItems, numRows, numCols, h_val, h_cols, h_rowDelimiters};
+    const char *nm = matrixName ? matrixName : "?";
+    coo_hip_handle *h = nullptr;
+    double t0 = pava_microtime();
+    int rc = coo_hip_create(&A, 0, &h);
+    double t1 = pava_microtime();
+    if (rc) {
+        pava_print_performance(nm, "COO_HIP", 0, -1, -1);
+        return rc;
+    }
+    int ntiles = 0;
+    (void)coo_hip_get_info(h, &ntiles, nullptr);
+    double *d_x = nullptr, *d_y = nullptr;
+    auto done = [&](int code) {
+        (void)hipFree(d_x);
+        (void)hipFree(d_y);
+        coo_hip_destroy(h);
+        return code;
+    };
+    if (hipMalloc(&d_x, (size_t)std::max(numCols, 1) * 8) != hipSuccess ||
+        hipMalloc(&d_y, (size_t)std::max(numRows, 1) * 8) != hipSuccess)
+        return done(set_error(PAVA_HIP_ENOMEM, "vector allocation failed"));
+    if (hipMemcpy(d_x, h_vec, (size_t)numCols * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return done(set_error(PAVA_HIP_EDEVICE, "x upload failed"));
+    if ((rc = coo_hip_spmv(h, d_x, d_y, nullptr))) return done(rc);
+    if (hipMemcpy(refOut, d_y, (size_t)numRows * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return done(set_error(PAVA_HIP_EDEVICE, "y download failed"));
+    (void)hipDeviceSynchronize();
+    const double t3 = pava_microtime();
+    for (int it = 0; it < numIterations; ++it)
+        if ((rc = coo_hip_spmv(h, d_x, d_y, nullptr))) return done(rc);
+    if (hipDeviceSynchronize() != hipSuccess) return done(set_error(PAVA_HIP_EDEVICE, "SpMV loop failed"));
+    const double t4 = pava_microtime();
+    pava_print_performance(nm, "COO_HIP", ntiles, t1 - t0, numIterations > 0 ? (t4 - t3) / numIterations : 0.0);
+    return done(0);
+}
+
 extern "C" int pava_bench_file(const char *path, int numIterations) {
     const int minSize = 4096;
     const long long maxSize = (1024LL + 128) * 1024 * 1024;
@@ -162,6 +205,12 @@ extern "C" int pava_bench_file(const char *path, int numIterations) {
                               nm.data(), numIterations);
     const int err2 = rc2 ? rc2 : pava_check_results(A.numRows, y.data(), y_ref.data());
     if (errors >= 0 && err2 != 0) errors = err2 < 0 ? err2 : errors + err2;
+    printf("**************\n    COO_HIP    \n**************\n");
+    pava_init_vectors(A.numRows, 0, nullptr, y.data(), nullptr);
+    int rc3 = conduct_coo_hip(A.numRows, A.numCols, A.nnz, A.rowOffsets, A.cols, A.vals, x.data(), y.data(),
+                              nm.data(), numIterations);
+    const int err3 = rc3 ? rc3 : pava_check_results(A.numRows, y.data(), y_ref.data());
+    if (errors >= 0 && err3 != 0) errors = err3 < 0 ? err3 : errors + err3;
     pava_delete_csr_matrix(&A);
     printf("\nCongratulations! This File comes to an Normal End! Flag[NormalEnding]\n\n");
     fflush(stdout);
