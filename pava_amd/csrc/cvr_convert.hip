@@ -377,15 +377,15 @@ __global__ __launch_bounds__(256) void convert_kernel(ConvertArgs a) {
 }
 
 // ---------------------------------------------------------------- hot columns
-// Column use counts (GPU), the most used columns in descending use order
-// (host: counts are copied back once), and the layout rewrite that renumbers
-// those columns into the dense hot-x buffer (bit 30 of a column marks them).
-// Hot number 0 is the most used column, so the first entries of the buffer
-// are the ones worth keeping in LDS.
-// Counts go through a direct-mapped LDS table first (a column claims a free
-// slot of its hash once per workgroup), so a heavily used column costs one
-// global atomic per workgroup instead of one per use; a column whose slot is
-// held by another column counts globally.
+// Column use counts (GPU), the most used columns in descending use order, and
+// the layout rewrite that renumbers those columns into the dense hot-x buffer
+// (bit 30 of a column marks them). Hot number 0 is the most used column, so
+// the first entries of the buffer are the ones worth keeping in LDS.
+// col_hist_kernel (matrices wider than the banded count below handles): counts
+// go through a direct-mapped LDS table first (a column claims a free slot of
+// its hash once per workgroup), so a heavily used column costs one global
+// atomic per workgroup instead of one per use; a column whose slot is held by
+// another column counts globally.
 constexpr int kHistSlots = 4096;
 
 __global__ __launch_bounds__(256) void col_hist_kernel(const int *__restrict__ ci, long long nnz, int *__restrict__ cnt) {
@@ -472,6 +472,158 @@ __global__ __launch_bounds__(256) void compact_write_kernel(const int *__restric
         base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
         __syncthreads();
     }
+}
+
+// Column counts without global atomics, for N <= kMaxBands * kBand columns.
+// (1) Each of kBandBlocks element ranges counts its elements per band of kBand
+// columns (LDS); (2) an exclusive scan over (band, range) gives every range
+// its write offset per band; (3) the ranges scatter their columns, as 16-bit
+// offsets within the band, into band order; (4) one workgroup per slice of a
+// band histograms its elements in LDS (all kBand counters of the band) and
+// writes them out; (5) the slices of each band are added per column.
+constexpr int kBandBits = 15;
+constexpr int kBand = 1 << kBandBits;  // columns per band: 128 KiB of LDS counters
+constexpr int kBandBlocks = 1024;      // element ranges of passes 1 and 3
+constexpr int kMaxBands = 4096;
+constexpr long long kSlice = 1 << 18;  // elements per workgroup of pass 4
+
+__global__ __launch_bounds__(256) void band_count_kernel(const int *__restrict__ ci, long long nnz, int nb,
+                                                         int *__restrict__ bcnt) {
+    extern __shared__ int bh[];
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) bh[i] = 0;
+    __syncthreads();
+    const long long per = (nnz + gridDim.x - 1) / gridDim.x;
+    const long long b0 = per * blockIdx.x, b1 = b0 + per < nnz ? b0 + per : nnz;
+    for (long long e = b0 + threadIdx.x; e < b1; e += blockDim.x) atomicAdd(&bh[ci[e] >> kBandBits], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) bcnt[(long long)i * gridDim.x + blockIdx.x] = bh[i];
+}
+
+// In-place exclusive scan of n ints by one workgroup (1024 threads, each a
+// contiguous run; the 1024 run sums are scanned serially).
+__global__ __launch_bounds__(1024) void exclusive_scan_kernel(int *__restrict__ a, long long n) {
+    __shared__ long long part[1024];
+    const long long per = (n + 1023) / 1024;
+    const long long b0 = per * threadIdx.x, b1 = b0 + per < n ? b0 + per : n;
+    long long s = 0;
+    for (long long i = b0; i < b1; ++i) s += a[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long acc = 0;
+        for (int i = 0; i < 1024; ++i) {
+            const long long v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    long long acc = part[threadIdx.x];
+    for (long long i = b0; i < b1; ++i) {
+        const int v = a[i];
+        a[i] = (int)acc;
+        acc += v;
+    }
+}
+
+__global__ __launch_bounds__(256) void band_scatter_kernel(const int *__restrict__ ci, long long nnz, int nb,
+                                                           const int *__restrict__ off, unsigned short *__restrict__ loc) {
+    extern __shared__ int cur[];
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) cur[i] = off[(long long)i * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const long long per = (nnz + gridDim.x - 1) / gridDim.x;
+    const long long b0 = per * blockIdx.x, b1 = b0 + per < nnz ? b0 + per : nnz;
+    for (long long e = b0 + threadIdx.x; e < b1; e += blockDim.x) {
+        const int c = ci[e];
+        const int pos = atomicAdd(&cur[c >> kBandBits], 1);
+        loc[pos] = (unsigned short)(c & (kBand - 1));
+    }
+}
+
+__global__ __launch_bounds__(256) void band_hist_kernel(const unsigned short *__restrict__ loc,
+                                                        const long long *__restrict__ sb, int *__restrict__ partial) {
+    extern __shared__ int sh[];
+    for (int i = threadIdx.x; i < kBand; i += blockDim.x) sh[i] = 0;
+    __syncthreads();
+    const long long e0 = sb[2 * blockIdx.x], e1 = sb[2 * blockIdx.x + 1];
+    for (long long e = e0 + threadIdx.x; e < e1; e += blockDim.x) atomicAdd(&sh[loc[e]], 1);
+    __syncthreads();
+    int *__restrict__ out = partial + (long long)blockIdx.x * kBand;
+    for (int i = threadIdx.x; i < kBand; i += blockDim.x) out[i] = sh[i];
+}
+
+__global__ void band_reduce_kernel(const int *__restrict__ partial, const int *__restrict__ first_slice, int N,
+                                   int *__restrict__ cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const int b = c >> kBandBits, l = c & (kBand - 1);
+    int s = 0;
+    for (int k = first_slice[b]; k < first_slice[b + 1]; ++k) s += partial[(long long)k * kBand + l];
+    cnt[c] = s;
+}
+
+// cnt[c] = uses of column c among ci[0, nnz).
+static hipError_t column_counts(const int *ci, long long nnz, int N, int *cnt, hipStream_t st) {
+    hipError_t e;
+    const int nb = (int)(((long long)N + kBand - 1) >> kBandBits);
+    if (nb > kMaxBands || nnz <= 0) {  // very wide matrices: the hashed atomic count
+        if ((e = hipMemsetAsync(cnt, 0, (size_t)N * 4, st)) != hipSuccess) return e;
+        hipLaunchKernelGGL(col_hist_kernel, dim3(2048), dim3(256), 0, st, ci, nnz, cnt);
+        return hipGetLastError();
+    }
+    // the pass-4 workgroups hold a band's counters in LDS
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(&band_hist_kernel),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, kBand * 4)) != hipSuccess)
+        return e;
+    const long long nbc = (long long)nb * kBandBlocks;
+    int *bcnt = nullptr, *partial = nullptr, *first = nullptr;
+    unsigned short *loc = nullptr;
+    long long *sb = nullptr;
+    auto done = [&](hipError_t r) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(bcnt);
+        (void)hipFree(loc);
+        (void)hipFree(partial);
+        (void)hipFree(first);
+        (void)hipFree(sb);
+        return r;
+    };
+    if ((e = hipMalloc(&bcnt, (size_t)nbc * 4)) != hipSuccess) return done(e);
+    if ((e = hipMalloc(&loc, (size_t)nnz * 2)) != hipSuccess) return done(e);
+    hipLaunchKernelGGL(band_count_kernel, dim3(kBandBlocks), dim3(256), (size_t)nb * 4, st, ci, nnz, nb, bcnt);
+    hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, st, bcnt, nbc);
+    if ((e = hipGetLastError()) != hipSuccess) return done(e);
+    // band starts: the offsets of range 0 of every band
+    std::vector<int> off((size_t)nbc);
+    if ((e = hipMemcpyAsync(off.data(), bcnt, (size_t)nbc * 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return done(e);
+    hipLaunchKernelGGL(band_scatter_kernel, dim3(kBandBlocks), dim3(256), (size_t)nb * 4, st, ci, nnz, nb, bcnt, loc);
+    if ((e = hipGetLastError()) != hipSuccess) return done(e);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return done(e);
+    std::vector<long long> slices;  // (begin, end) per slice, bands in order
+    std::vector<int> first_h((size_t)nb + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+        const long long s0 = off[(size_t)b * kBandBlocks];
+        const long long s1 = b + 1 < nb ? off[(size_t)(b + 1) * kBandBlocks] : nnz;
+        first_h[(size_t)b] = (int)(slices.size() / 2);
+        for (long long x0 = s0; x0 < s1; x0 += kSlice) {
+            slices.push_back(x0);
+            slices.push_back(x0 + kSlice < s1 ? x0 + kSlice : s1);
+        }
+    }
+    const int ns = (int)(slices.size() / 2);
+    first_h[(size_t)nb] = ns;
+    if ((e = hipMalloc(&partial, (size_t)std::max(ns, 1) * kBand * 4)) != hipSuccess) return done(e);
+    if ((e = hipMalloc(&first, ((size_t)nb + 1) * 4)) != hipSuccess) return done(e);
+    if ((e = hipMalloc(&sb, (size_t)std::max(ns, 1) * 16)) != hipSuccess) return done(e);
+    if ((e = hipMemcpyAsync(first, first_h.data(), ((size_t)nb + 1) * 4, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return done(e);
+    if (ns > 0) {
+        if ((e = hipMemcpyAsync(sb, slices.data(), (size_t)ns * 16, hipMemcpyHostToDevice, st)) != hipSuccess)
+            return done(e);
+        hipLaunchKernelGGL(band_hist_kernel, dim3(ns), dim3(256), (size_t)kBand * 4, st, loc, sb, partial);
+    }
+    hipLaunchKernelGGL(band_reduce_kernel, dim3((N + 255) / 256), dim3(256), 0, st, partial, first, N, cnt);
+    return done(hipGetLastError());
 }
 
 __global__ void hot_scatter_kernel(const int *__restrict__ hot_cols, int n, int *__restrict__ colmap) {
@@ -572,9 +724,7 @@ hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, 
                               int *hot_cols, int *cols, long long L, int *n_hot, hipStream_t st) {
     *n_hot = 0;
     hipError_t e;
-    if ((e = hipMemsetAsync(cnt, 0, (size_t)N * 4, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(col_hist_kernel, dim3(2048), dim3(256), 0, st, ci, nnz, cnt);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = column_counts(ci, nnz, N, cnt, st)) != hipSuccess) return e;
     // histogram of the counts (colmap is free until the scatter below)
     int *hist = colmap;
     if (N < kCountBins) return hot_columns_host(cnt, N, max_hot, min_skew, nnz, colmap, hot_cols, cols, L, n_hot, st);
