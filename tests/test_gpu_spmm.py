@@ -131,3 +131,34 @@ def test_spmm_bit_exact_vs_cvr_order(P, O, k):
         lay = O.cvr64_build(A, S, 0)
         want = np.stack([O.cvr64_spmv(A, lay, X[:, j].astype(np.float64)) for j in range(k)], 1).astype(np.float32)
         assert np.array_equal(Y, want), name
+
+
+@pytest.mark.parametrize("k", [8, 32])
+def test_spmm_with_hot_columns_bit_identical(P, O, k):
+    """Handles with hot-x renumbering map hot column numbers back through the
+    hot-column list (looked up for a whole group before the X-row loads): the
+    single-matrix and batched SpMM give the same Y, bit for bit, as handles
+    without hot columns."""
+    import torch
+    rng = np.random.default_rng(70 + k)
+    A = _f32_matrix(P.rmat(14, 16, seed=9))
+    X = torch.from_numpy(rng.uniform(-1, 1, (A.numCols, k)).astype(np.float32)).cuda()
+    out = []
+    for hot_kb in (64, 0):
+        with P.CvrHip(A, steps_per_chunk=32, precision=P.PREC_F32, x_hot_kb=hot_kb) as h:
+            if hot_kb:
+                assert len(h.hot_cols()) > 0
+            Y = torch.full((A.numRows, k), float("nan"), dtype=torch.float32, device="cuda")
+            h.spmm(k, X.data_ptr(), Y.data_ptr())
+            torch.cuda.synchronize()
+            b = P.Batch([h], k)
+            Yb = torch.full((A.numRows, k), float("nan"), dtype=torch.float32, device="cuda")
+            b.set_operands([X.data_ptr()], [Yb.data_ptr()])
+            b.spmm(0)
+            torch.cuda.synchronize()
+            b.close()
+            out.append((Y.cpu().numpy(), Yb.cpu().numpy()))
+    (y_hot, yb_hot), (y_plain, yb_plain) = out
+    assert np.array_equal(y_hot, y_plain)
+    assert np.array_equal(yb_hot, yb_plain)
+    assert np.array_equal(y_hot, yb_hot)
