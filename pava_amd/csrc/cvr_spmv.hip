@@ -112,9 +112,12 @@ __device__ __forceinline__ void store_y(void *y, int r, double v) {
 // each wave walks chunks of the workgroup's contiguous chunk range; a gather
 // of such a column is an LDS read instead of a divergent global load.
 template <typename VT, typename XT, typename YT, bool COMP, bool COL16, bool KAPPA, bool LDSX>
-// 32-bit layouts: >= 4 waves per SIMD (128 VGPRs, no spills); the 16-bit
-// layouts keep their end masks in registers and are left unconstrained
-__global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock, (COL16 && !LDSX) ? 1 : 4) void spmv_kernel(
+// 32-bit layouts: >= 4 waves per SIMD (128 VGPRs, no spills); the LDS variant
+// runs one workgroup of kWavesLds waves per CU (its table fills the LDS), so
+// 2 waves per SIMD and up to 256 VGPRs; the 16-bit layouts are left
+// unconstrained
+__global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
+                             LDSX ? kWavesLds / 4 : (COL16 ? 1 : 4)) void spmv_kernel(
     SpmvArgs a) {
     constexpr int WPB = LDSX ? kWavesLds : kWavesPerBlock;
     const int lane = threadIdx.x & 63;
