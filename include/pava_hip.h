@@ -310,6 +310,12 @@ int cvr_hip_time_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, int warmup,
 /* Same for a mixed-precision handle built with recompute (d_x64 = FP64 x). */
 int cvr_hip_time_spmv_x64(cvr_hip_handle *h, const void *d_x, const double *d_x64, void *d_y, int warmup,
                           int iters, double *ms_total, double *ms_main);
+/* The same measurement with the spread of the main kernel: stats[0] = mean ms
+ * per SpMV (all kernels), stats[1..4] = mean, median, min and max ms of the
+ * main SpMV kernel over the iters launches (d_x64 may be null unless the
+ * handle recomputes). */
+int cvr_hip_time_spmv_stats(cvr_hip_handle *h, const void *d_x, const double *d_x64, void *d_y, int warmup,
+                            int iters, double *stats);
 
 /* Copies of the layout for checking (host buffers sized from cvr_hip_info):
  * vals (layout_elems * value size), cols (layout_elems int32, or uint16 when

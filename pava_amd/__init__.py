@@ -77,7 +77,7 @@ EXPORTED = [
     "pava_microtime", "pava_gen_laplacian2d", "pava_gen_stencil27", "pava_gen_rmat", "pava_gen_synthetic",
     "cvr_hip_default_opts", "cvr_hip_create", "cvr_hip_create_device", "cvr_hip_destroy", "cvr_hip_get_info",
     "cvr_hip_spmv", "cvr_hip_spmv_host", "cvr_hip_spmv_mixed", "cvr_hip_spmm", "cvr_hip_last_recompute_count",
-    "cvr_hip_time_spmv", "cvr_hip_time_spmv_x64", "cvr_hip_export_layout", "conduct_cvr_hip", "pava_bench_file",
+    "cvr_hip_time_spmv", "cvr_hip_time_spmv_x64", "cvr_hip_time_spmv_stats", "cvr_hip_export_layout", "conduct_cvr_hip", "pava_bench_file",
     "pava_hip_last_error", "pava_hip_version", "cvr_hip_batch_create", "cvr_hip_batch_set_operands",
     "cvr_hip_batch_spmm", "cvr_hip_batch_graph", "cvr_hip_batch_replay", "cvr_hip_batch_info",
     "cvr_hip_batch_destroy", "csr_hip_create", "csr_hip_spmv", "csr_hip_destroy", "pava_csr_features",
@@ -351,6 +351,14 @@ class CvrHip:
         _check(lib().cvr_hip_time_spmv_x64(self._h, x_ptr, x64_ptr or None, y_ptr, warmup, iters, C.byref(tot),
                                            C.byref(main)))
         return tot.value, main.value
+
+    def time_spmv_stats(self, x_ptr: int, y_ptr: int, warmup: int = 10, iters: int = 100, x64_ptr: int = 0) -> dict:
+        """ms per SpMV (all kernels) and the main kernel's mean / median / min / max ms."""
+        st = (C.c_double * 5)()
+        L = lib()
+        L.cvr_hip_time_spmv_stats.argtypes = [C.c_void_p] * 4 + [C.c_int, C.c_int, C.c_void_p]
+        _check(L.cvr_hip_time_spmv_stats(self._h, x_ptr, x64_ptr or None, y_ptr, warmup, iters, st))
+        return {"ms_total": st[0], "main_mean": st[1], "main_median": st[2], "main_min": st[3], "main_max": st[4]}
 
     def last_recompute_count(self) -> int:
         n = C.c_int()

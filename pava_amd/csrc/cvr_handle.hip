@@ -759,51 +759,73 @@ extern "C" int cvr_hip_time_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, 
     return cvr_hip_time_spmv_x64(h, d_x, nullptr, d_y, warmup, iters, ms_total, ms_main);
 }
 
-extern "C" int cvr_hip_time_spmv_x64(cvr_hip_handle *h, const void *d_x, const double *d_x64, void *d_y, int warmup,
-                                     int iters, double *ms_total, double *ms_main) {
-    if (!h || iters <= 0) return set_error(PAVA_HIP_EINVAL, "bad timing arguments");
+// Timing on the handle's stream: warmup SpMVs, then `iters` SpMVs bracketed
+// by two events (stats[0]: mean ms per SpMV, every kernel included), then
+// `iters` SpMVs with events around each launch of the main SpMV kernel
+// (stats[1..4]: its mean, median, min and max in ms).
+extern "C" int cvr_hip_time_spmv_stats(cvr_hip_handle *h, const void *d_x, const double *d_x64, void *d_y, int warmup,
+                                       int iters, double *stats) {
+    if (!h || iters <= 0 || !stats) return set_error(PAVA_HIP_EINVAL, "bad timing arguments");
     if (h->d_rflag && !d_x64) return set_error(PAVA_HIP_EINVAL, "a handle built with recompute needs the FP64 x");
     HIP_TRY(hipSetDevice(h->device));
+    std::vector<hipEvent_t> ev;  // [0, 1]: whole SpMVs; then a pair per main-kernel launch
+    auto finish = [&](int code) {
+        (void)hipStreamSynchronize(h->stream);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        return code;
+    };
+    for (size_t i = 0; i < 2 + 2 * (size_t)iters; ++i) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return finish(set_error(PAVA_HIP_EDEVICE, "hipEventCreate failed"));
+        ev.push_back(e);
+    }
     int rc;
     for (int i = 0; i < warmup; ++i)
-        if ((rc = enqueue_spmv(h, d_x, d_y, h->stream, d_x64))) return rc;
-    hipEvent_t a, b;
-    HIP_TRY(hipEventCreate(&a));
-    HIP_TRY(hipEventCreate(&b));
-    HIP_TRY(hipEventRecord(a, h->stream));
+        if ((rc = enqueue_spmv(h, d_x, d_y, h->stream, d_x64))) return finish(rc);
+    if (hipEventRecord(ev[0], h->stream) != hipSuccess) return finish(set_error(PAVA_HIP_EDEVICE, "event record"));
     for (int i = 0; i < iters; ++i)
-        if ((rc = enqueue_spmv(h, d_x, d_y, h->stream, d_x64))) return rc;
-    HIP_TRY(hipEventRecord(b, h->stream));
-    HIP_TRY(hipEventSynchronize(b));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, a, b));
-    if (ms_total) *ms_total = ms / iters;
-    if (ms_main) {
-        // main kernel alone: events bracket each launch of the SpMV kernel
-        std::vector<hipEvent_t> ev(2 * (size_t)iters);
-        for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
-        cvr::SpmvArgs sa = spmv_args(h, d_x, d_y);
-        for (int i = 0; i < iters; ++i) {
-            if (h->n_hot > 0)
-                HIP_TRY(cvr::launch_hot_gather(d_x, h->d_hot_cols, h->d_xhot, h->n_hot,
-                                               h->prec == CVR_HIP_PREC_F64 ? 8 : 4, h->stream));
-            HIP_TRY(hipEventRecord(ev[2 * (size_t)i], h->stream));
-            HIP_TRY(cvr::launch_spmv(sa, h->prec, h->col16, h->stream));
-            HIP_TRY(hipEventRecord(ev[2 * (size_t)i + 1], h->stream));
-            if ((rc = enqueue_tail(h, d_y, (const double *)d_x64, h->stream))) return rc;
-        }
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        double tot = 0;
-        for (int i = 0; i < iters; ++i) {
-            float t = 0.f;
-            HIP_TRY(hipEventElapsedTime(&t, ev[2 * (size_t)i], ev[2 * (size_t)i + 1]));
-            tot += t;
-        }
-        *ms_main = tot / iters;
-        for (auto &e : ev) hipEventDestroy(e);
+        if ((rc = enqueue_spmv(h, d_x, d_y, h->stream, d_x64))) return finish(rc);
+    if (hipEventRecord(ev[1], h->stream) != hipSuccess) return finish(set_error(PAVA_HIP_EDEVICE, "event record"));
+    cvr::SpmvArgs sa = spmv_args(h, d_x, d_y);
+    for (int i = 0; i < iters; ++i) {
+        hipError_t e = hipSuccess;
+        if (h->n_hot > 0)
+            e = cvr::launch_hot_gather(d_x, h->d_hot_cols, h->d_xhot, h->n_hot, h->prec == CVR_HIP_PREC_F64 ? 8 : 4,
+                                       h->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[2 + 2 * (size_t)i], h->stream);
+        if (e == hipSuccess) e = cvr::launch_spmv(sa, h->prec, h->col16, h->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 + 2 * (size_t)i], h->stream);
+        if (e != hipSuccess) return finish(set_error(PAVA_HIP_EDEVICE, "timed launch: %s", hipGetErrorString(e)));
+        if ((rc = enqueue_tail(h, d_y, d_x64, h->stream))) return finish(rc);
     }
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return finish(set_error(PAVA_HIP_EDEVICE, "timed SpMVs failed"));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) return finish(set_error(PAVA_HIP_EDEVICE, "event time"));
+    stats[0] = ms / iters;
+    std::vector<double> t((size_t)iters);
+    for (int i = 0; i < iters; ++i) {
+        float x = 0.f;
+        if (hipEventElapsedTime(&x, ev[2 + 2 * (size_t)i], ev[3 + 2 * (size_t)i]) != hipSuccess)
+            return finish(set_error(PAVA_HIP_EDEVICE, "event time"));
+        t[(size_t)i] = x;
+    }
+    double tot = 0;
+    for (double x : t) tot += x;
+    std::sort(t.begin(), t.end());
+    stats[1] = tot / iters;
+    stats[2] = iters % 2 ? t[(size_t)iters / 2] : 0.5 * (t[(size_t)iters / 2 - 1] + t[(size_t)iters / 2]);
+    stats[3] = t.front();
+    stats[4] = t.back();
+    return finish(0);
+}
+
+extern "C" int cvr_hip_time_spmv_x64(cvr_hip_handle *h, const void *d_x, const double *d_x64, void *d_y, int warmup,
+                                     int iters, double *ms_total, double *ms_main) {
+    double st[5];
+    const int rc = cvr_hip_time_spmv_stats(h, d_x, d_x64, d_y, warmup, iters, st);
+    if (rc) return rc;
+    if (ms_total) *ms_total = st[0];
+    if (ms_main) *ms_main = st[1];
     return 0;
 }
 

@@ -429,3 +429,19 @@ def test_csr_hip_format(P, O, mats, L):
         if L == 1:
             assert np.array_equal(y, yref), name  # one lane per row: the oracle's order
         assert np.all(np.abs(y - yref) <= 2 * (lens + 64) * 2.0 ** -53 * sabs), name
+
+
+def test_time_spmv_stats(P):
+    """hipEvent timing: the main kernel's min <= median <= max, its mean
+    inside that range, and a whole SpMV at least as long as the kernel."""
+    import torch
+    A = P.stencil27(48, 0.01, 0)
+    x = torch.from_numpy(P.fill_uniform(A.numCols, 1)).cuda()
+    y = torch.empty(A.numRows, dtype=torch.float64, device="cuda")
+    with P.CvrHip(A) as h:
+        st = h.time_spmv_stats(x.data_ptr(), y.data_ptr(), 3, 101)
+        tot, main = h.time_spmv(x.data_ptr(), y.data_ptr(), 3, 50)
+    _say("timing", st, tot, main)
+    assert 0 < st["main_min"] <= st["main_median"] <= st["main_max"]
+    assert st["main_min"] <= st["main_mean"] <= st["main_max"]
+    assert st["ms_total"] >= 0.5 * st["main_mean"] and tot > 0 and main > 0
