@@ -161,7 +161,7 @@ int pava_gen_synthetic(const pava_gen_params *p, pava_csr_matrix *out);
 
 typedef struct cvr_hip_opts {
     int steps_per_chunk;     /* S: 64-lane steps per chunk; multiple of 8, 8..4096; 0 = automatic
-                                (default): about two chunks per resident wave slot of the device,
+                                (default): about four chunks per resident wave slot of the device,
                                 at most 256 (cvr_hip_get_info reports the value used) */
     int precision;           /* CVR_HIP_PREC_* */
     int col_index_bits;      /* 32, or 16 (chunk-relative offsets; mixed precision only) */

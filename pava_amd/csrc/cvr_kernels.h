@@ -29,6 +29,26 @@ constexpr int kCodeHead = -1;     // partial of the chunk's first row -> carry[2
 constexpr int kCodeTail = -2;     // partial of the chunk's last row  -> carry[2c+1]
 constexpr int kCodeDiscard = -3;  // padding
 
+// s_waitcnt vmcnt(0) with the other counters unconstrained: issued after a
+// loop's prologue so the loop is entered with no loads in flight (the
+// compiler's wait counts inside the loop then follow the loop's own loads).
+constexpr int kWaitVm0 = 0x0f70;
+
+// Global-memory access through pointers whose address space the compiler
+// cannot see (integer addresses, pointers read from memory): without the cast
+// they become flat accesses, which are also waited for as possible LDS
+// accesses.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T *p) {
+    return reinterpret_cast<gptr<T>>(reinterpret_cast<uintptr_t>(p));
+}
+template <typename T>
+__device__ __forceinline__ T load_global(uint64_t addr) {
+    return *reinterpret_cast<gptr<const T>>(addr);
+}
+
 struct ConvertArgs {
     const int *rp;
     const int *ci;

@@ -36,16 +36,8 @@ __device__ __forceinline__ void wave_sync_lds_m() {
 
 constexpr int kBatchQueues = 8;  // queue ranges of the batched launch (one per XCD); counters in use
 
-// The layout, X and Y live in global memory. Pointers read from a descriptor
-// in memory (the batched path) are generic to the compiler, which then emits
-// flat accesses and waits on both counters around them; these casts make the
-// address space explicit.
-template <typename T>
-using gptr = __attribute__((address_space(1))) T *;
-template <typename T>
-__device__ __forceinline__ gptr<T> to_global(T *p) {
-    return reinterpret_cast<gptr<T>>(reinterpret_cast<uintptr_t>(p));
-}
+// The layout, X and Y live in global memory; pointers read from a descriptor
+// in memory (the batched path) go through to_global.
 typedef float vfloat4 __attribute__((ext_vector_type(4)));  // 16-byte loads and stores of X and Y rows
 
 template <typename VT>
@@ -68,8 +60,6 @@ __device__ __forceinline__ double val_f64<unsigned short>(unsigned short v) {
 // segment its rank in the chunk's segment list) and a private copy of each
 // 8-step group's segment entries, staged in LDS one group ahead as in the
 // SpMV. `slots` (max_slots x KP doubles) is shared by the workgroup.
-constexpr int kWaitVm0m = 0x0f70;  // s_waitcnt vmcnt(0), other counters unconstrained
-
 // Columns [j0, j0 + 4) of an X row when k is not the block width: element by
 // element (kept out of line so the 16-byte load of full rows stays one load).
 __device__ __noinline__ float4 xpart_partial(gptr<const float> p, int j0, int k) {
@@ -165,7 +155,7 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
     if (G < steps) load_cols(G, cn);
     // the loop is entered with nothing in flight (its wait counts then follow
     // its own loads only)
-    __builtin_amdgcn_s_waitcnt(kWaitVm0m);
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
     int sbase = 0, segp = 0;
 
     for (int g = 0; g < steps; g += G) {

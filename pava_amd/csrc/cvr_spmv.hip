@@ -92,15 +92,6 @@ struct Acc<VT, float, true, KAPPA> {
     __device__ __forceinline__ void reset() { s = 0.f, c = 0.f, ab = 0.f; }
 };
 
-// Global-memory load from an integer address (a plain pointer cast would be
-// a flat access, which the compiler's wait-count tracking treats as possibly
-// LDS as well).
-template <typename T>
-__device__ __forceinline__ T load_global(uint64_t addr) {
-    return *reinterpret_cast<const __attribute__((address_space(1))) T *>(addr);
-}
-
-constexpr int kWaitVm0 = 0x0f70;  // s_waitcnt vmcnt(0), other counters unconstrained
 
 template <typename YT>
 __device__ __forceinline__ void store_y(void *y, int r, double v) {

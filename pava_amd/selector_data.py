@@ -10,6 +10,11 @@ large matrices (1e6 to 5e6 rows). The training and first held-out sets use rows
   python -m pava_amd.selector_data gen --first 0 --count 500 --out DIR
       autotunes matrices [first, first+count) over every config and writes
       DIR/shard_<first>_<count>.npz (features, per-config ms, sizes, params)
+  python -m pava_amd.selector_data retime --config NAME --first F --count C --out DIR
+      re-times one config (after a change to it) for matrices [F, F+C):
+      DIR/retime_<NAME>_<F>_<C>.npz
+  python -m pava_amd.selector_data patch --data FILE.npz --config NAME RETIME.npz ...
+      replaces that config's column of the dataset with the re-timed values
   python -m pava_amd.selector_data merge --out FILE.npz SHARD.npz ...
   python -m pava_amd.selector_data eval --data FILE.npz --out REPORT.json
       trains the MLP on the GPU (the library's own kernels) on the training
@@ -66,6 +71,44 @@ def gen(first: int, count: int, out: Path, iters: int = 5) -> Path:
     np.savez_compressed(path, index=np.array(idx), features=np.array(F, np.float32), ms=np.array(T),
                         rows=np.array(rows), nnz=np.array(nnz), configs=np.array([c.name for c in S.CONFIGS]))
     return path
+
+
+def retime(name: str, first: int, count: int, out: Path, iters: int = 5) -> Path:
+    import pava_amd as P
+    from . import selector as S
+    cfg = [c for c in S.CONFIGS if c.name == name]
+    if not cfg:
+        raise SystemExit(f"no config named {name}")
+    idx, ms = [], []
+    t0 = time.perf_counter()
+    for i in range(first, first + count):
+        A = matrix(i)
+        ms.append(S.autotune(A, P.fill_uniform(A.numCols, i), iters=iters, configs=cfg)[0])
+        idx.append(i)
+        if (i - first) % 50 == 49 or i == first + count - 1:
+            print(f"[selector] retime {name}: {i - first + 1}/{count} ({time.perf_counter() - t0:.0f}s)", flush=True)
+    out.mkdir(parents=True, exist_ok=True)
+    path = out / f"retime_{name}_{first}_{count}.npz"
+    np.savez_compressed(path, index=np.array(idx), ms=np.array(ms), config=np.array(name))
+    return path
+
+
+def patch(data: Path, name: str, parts: list[Path]) -> None:
+    d = dict(np.load(data, allow_pickle=False))
+    names = [str(n) for n in d["configs"]]
+    j = names.index(name)
+    pos = {int(i): k for k, i in enumerate(d["index"])}
+    ms = d["ms"].copy()
+    n = 0
+    for p in parts:
+        q = np.load(p, allow_pickle=False)
+        assert str(q["config"]) == name, (p, str(q["config"]))
+        for i, v in zip(q["index"], q["ms"]):
+            ms[pos[int(i)], j] = v
+            n += 1
+    d["ms"] = ms
+    np.savez_compressed(data, **d)
+    print(f"[selector] patched {n} timings of {name}")
 
 
 def merge(out: Path, shards: list[Path]) -> None:
@@ -170,6 +213,15 @@ def main(argv=None) -> int:
     g.add_argument("--count", type=int, required=True)
     g.add_argument("--out", type=Path, required=True)
     g.add_argument("--iters", type=int, default=5)
+    rt = sub.add_parser("retime")
+    rt.add_argument("--config", required=True)
+    rt.add_argument("--first", type=int, required=True)
+    rt.add_argument("--count", type=int, required=True)
+    rt.add_argument("--out", type=Path, required=True)
+    pt = sub.add_parser("patch")
+    pt.add_argument("--data", type=Path, required=True)
+    pt.add_argument("--config", required=True)
+    pt.add_argument("parts", type=Path, nargs="+")
     m = sub.add_parser("merge")
     m.add_argument("--out", type=Path, required=True)
     m.add_argument("shards", type=Path, nargs="+")
@@ -181,6 +233,10 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     if a.cmd == "gen":
         print(gen(a.first, a.count, a.out, a.iters))
+    elif a.cmd == "retime":
+        print(retime(a.config, a.first, a.count, a.out))
+    elif a.cmd == "patch":
+        patch(a.data, a.config, a.parts)
     elif a.cmd == "merge":
         merge(a.out, a.shards)
     else:

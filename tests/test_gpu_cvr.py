@@ -278,7 +278,7 @@ def test_low_precision_bit_exact_vs_emulation(P, O, mats, prec):
 
 
 def test_automatic_chunk_size(P, O):
-    """steps_per_chunk = 0: about two chunks per resident wave slot (16 per CU),
+    """steps_per_chunk = 0: about four chunks per resident wave slot (16 per CU),
     a multiple of 8 in [8, 256]; the layout is the emulation's at that size."""
     import torch
     ncu = torch.cuda.get_device_properties(0).multi_processor_count
@@ -287,7 +287,7 @@ def test_automatic_chunk_size(P, O):
             S = h.info()["steps_per_chunk"]
             x = P.fill_uniform(A.numCols, 1)
             y = h.spmv_host(x)
-        steps = -(-A.nnz // (64 * 2 * 16 * ncu))  # ceil: two chunks per wave slot
+        steps = -(-A.nnz // (64 * 4 * 16 * ncu))  # ceil: four chunks per wave slot
         want = min(max(-(-steps // 8) * 8, 8), 256)
         assert S == want, (S, want)
         assert np.array_equal(y, O.cvr64_spmv(A, O.cvr64_build(A, S, 0), x))
