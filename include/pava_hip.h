@@ -172,8 +172,8 @@ typedef struct cvr_hip_opts {
     int device;              /* HIP device ordinal */
     int x_hot_kb;            /* hot-x staging: the most used columns (each used >= 2x, at most
                                 x_hot_kb KiB of x) are renumbered, most used first, into a dense
-                                copy of x refreshed before each SpMV; 0 = off (default); -1 =
-                                automatic: 2048 KiB, staged only when the most used eighth of
+                                copy of x refreshed before each SpMV; 0 = off; -1 = automatic
+                                (default): 2048 KiB, staged only when the most used eighth of
                                 the used columns is used at least twice as often as an average
                                 used column (32-bit layouts only) */
     int x_lds_kb;            /* of the hot-x copy, the first x_lds_kb KiB are held in LDS by each

@@ -289,7 +289,7 @@ class CvrHip:
 
     def __init__(self, A: CSR, steps_per_chunk: int = 0, precision: int = PREC_F64, col_index_bits: int = 32,
                  recompute: bool = False, kappa_threshold: float = 64.0, keep_fp64: bool = False, device: int = 0,
-                 x_hot_kb: int = 0, x_lds_kb: int = -1):
+                 x_hot_kb: int = -1, x_lds_kb: int = -1):
         L = lib()
         o = _Opts()
         L.cvr_hip_default_opts(C.byref(o))

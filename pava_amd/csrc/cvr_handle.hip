@@ -85,7 +85,7 @@ extern "C" void cvr_hip_default_opts(cvr_hip_opts *o) {
     o->kappa_threshold = 64.0;
     o->keep_fp64 = 0;
     o->device = 0;
-    o->x_hot_kb = 0;
+    o->x_hot_kb = -1;  // automatic
     o->x_lds_kb = -1;
 }
 

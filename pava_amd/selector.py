@@ -33,11 +33,12 @@ class Config:
     fmt: str        # "cvr" | "csr5" | "csr"
     prec: str       # "f64" | "f32rc" | "f32rc16"
     tile: int       # S for cvr (0: the library's automatic size), sigma for csr5, L for csr
-    xs: int = 0     # x staging: hot-x copy size in KiB (0 = off)
+    xs: int = 0     # x staging: hot-x copy size in KiB (0 = off, -1 = the library's automatic choice)
 
     @property
     def name(self) -> str:
-        return f"{self.fmt}-{self.prec}-{self.tile}" + (f"-x{self.xs}" if self.xs else "")
+        xs = "" if self.xs == 0 else "-xauto" if self.xs < 0 else f"-x{self.xs}"
+        return f"{self.fmt}-{self.prec}-{self.tile}{xs}"
 
 
 CONFIGS = ([Config("cvr", "f64", s) for s in (32, 128, 512)]
@@ -47,8 +48,8 @@ CONFIGS = ([Config("cvr", "f64", s) for s in (32, 128, 512)]
            + [Config("cvr", "f64", s, 2048) for s in (128, 512)]
            + [Config("cvr", "f32rc", 128, 2048)]
            + [Config("csr5", "f64", s) for s in (8, 32)]
-           + [Config("cvr", "f64", 0)])
-DEFAULT = Config("cvr", "f64", 0)  # the fallback without a model: the library's default
+           + [Config("cvr", "f64", 0, -1)])
+DEFAULT = Config("cvr", "f64", 0, -1)  # the fallback without a model: the library's default
 
 
 def features(A: CSR) -> np.ndarray:

@@ -13,8 +13,9 @@ large matrices (1e6 to 5e6 rows). The training and first held-out sets use rows
   python -m pava_amd.selector_data retime --config NAME --first F --count C --out DIR
       re-times one config (after a change to it) for matrices [F, F+C):
       DIR/retime_<NAME>_<F>_<C>.npz
-  python -m pava_amd.selector_data patch --data FILE.npz --config NAME RETIME.npz ...
+  python -m pava_amd.selector_data patch --data FILE.npz --config NAME [--replaces OLD] RETIME.npz ...
       replaces that config's column of the dataset with the re-timed values
+      (with --replaces: the column of config OLD, which becomes NAME)
   python -m pava_amd.selector_data merge --out FILE.npz SHARD.npz ...
   python -m pava_amd.selector_data eval --data FILE.npz --out REPORT.json
       trains the MLP on the GPU (the library's own kernels) on the training
@@ -93,10 +94,12 @@ def retime(name: str, first: int, count: int, out: Path, iters: int = 5) -> Path
     return path
 
 
-def patch(data: Path, name: str, parts: list[Path]) -> None:
+def patch(data: Path, name: str, parts: list[Path], replaces: str | None = None) -> None:
     d = dict(np.load(data, allow_pickle=False))
     names = [str(n) for n in d["configs"]]
-    j = names.index(name)
+    j = names.index(replaces or name)
+    names[j] = name
+    d["configs"] = np.array(names)
     pos = {int(i): k for k, i in enumerate(d["index"])}
     ms = d["ms"].copy()
     n = 0
@@ -221,6 +224,7 @@ def main(argv=None) -> int:
     pt = sub.add_parser("patch")
     pt.add_argument("--data", type=Path, required=True)
     pt.add_argument("--config", required=True)
+    pt.add_argument("--replaces", default=None)
     pt.add_argument("parts", type=Path, nargs="+")
     m = sub.add_parser("merge")
     m.add_argument("--out", type=Path, required=True)
@@ -236,7 +240,7 @@ def main(argv=None) -> int:
     elif a.cmd == "retime":
         print(retime(a.config, a.first, a.count, a.out))
     elif a.cmd == "patch":
-        patch(a.data, a.config, a.parts)
+        patch(a.data, a.config, a.parts, a.replaces)
     elif a.cmd == "merge":
         merge(a.out, a.shards)
     else:

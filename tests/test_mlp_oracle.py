@@ -63,7 +63,7 @@ def test_config_space():
     # every axis the autotuner searches is present: format, precision, tile, x staging
     assert {c.fmt for c in S.CONFIGS} == {"cvr", "csr5", "csr"}
     assert {c.prec for c in S.CONFIGS} == {"f64", "f32rc", "f32rc16"}
-    assert {c.xs for c in S.CONFIGS} == {0, 2048}
+    assert {c.xs for c in S.CONFIGS} == {-1, 0, 2048}
     assert len({c.tile for c in S.CONFIGS if c.fmt == "cvr"}) >= 3
 
 
