@@ -46,6 +46,12 @@ extern "C" {
 #define PAVA_HIP_ERANGE (-6)   /* index out of range or size beyond int32 */
 #define PAVA_HIP_ESTATE (-7)   /* handle used with the wrong precision / API */
 
+/* Largest numRows, numCols and nnz the GPU formats accept: 2^31 - 2^16, so that
+ * element positions plus a chunk's padding or a 512-element stride stay in
+ * int32 (larger matrices: PAVA_HIP_ERANGE). The reader and the CSR cache take
+ * up to INT32_MAX. */
+#define PAVA_HIP_MAX_SIZE 2147418112
+
 /* Reader return codes follow readCOOMatrix (utils/basicMatrix.cpp:84-132):
  * -1 bad banner, -2 not a matrix, -3 not sparse (coordinate), -4 bad size line,
  * -5 allocation failure; this build adds -6 (entry out of range / truncated

@@ -154,8 +154,9 @@ extern "C" int coo_hip_destroy(coo_hip_handle *h) {
 extern "C" int coo_hip_create(const pava_csr_matrix *A, int device, coo_hip_handle **out) {
     if (!out) return set_error(PAVA_HIP_EINVAL, "null output");
     *out = nullptr;
-    int rc = pava_csr_validate(A);
-    if (rc) return rc;
+    int rc = 0;
+    if (A && (rc = check_gpu_sizes(A->numRows, A->numCols, A->nnz))) return rc;
+    if ((rc = pava_csr_validate(A))) return rc;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(PAVA_HIP_EDEVICE, "no HIP device");
     if (hipSetDevice(device) != hipSuccess) return set_error(PAVA_HIP_EDEVICE, "hipSetDevice failed");

@@ -35,6 +35,7 @@
 #include "pava_hip.h"
 #include "pava_internal.h"
 
+using pava::check_gpu_sizes;
 using pava::set_error;
 
 namespace csr5 {
@@ -378,8 +379,9 @@ static int csr5_build(const pava_csr_matrix *A, csr5_hip_handle *h) {
 extern "C" int csr5_hip_create(const pava_csr_matrix *A, int sigma, int device, csr5_hip_handle **out) {
     if (!out) return set_error(PAVA_HIP_EINVAL, "null output handle");
     *out = nullptr;
-    int rc = pava_csr_validate(A);
-    if (rc) return rc;
+    int rc = 0;
+    if (A && (rc = check_gpu_sizes(A->numRows, A->numCols, A->nnz))) return rc;
+    if ((rc = pava_csr_validate(A))) return rc;
     if (sigma == 0) sigma = 16;
     if (sigma != 4 && sigma != 8 && sigma != 16 && sigma != 32)
         return set_error(PAVA_HIP_EINVAL, "sigma must be 4, 8, 16 or 32, got %d", sigma);

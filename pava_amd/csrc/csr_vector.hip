@@ -72,8 +72,9 @@ extern "C" int csr_hip_create(const pava_csr_matrix *A, int lanes_per_row, int p
                               csr_hip_handle **out) {
     if (!out) return set_error(PAVA_HIP_EINVAL, "null output");
     *out = nullptr;
-    int rc = pava_csr_validate(A);
-    if (rc) return rc;
+    int rc = 0;
+    if (A && (rc = check_gpu_sizes(A->numRows, A->numCols, A->nnz))) return rc;
+    if ((rc = pava_csr_validate(A))) return rc;
     if (lanes_per_row < 1 || lanes_per_row > 64 || (lanes_per_row & (lanes_per_row - 1)))
         return set_error(PAVA_HIP_EINVAL, "lanes_per_row must be a power of two in [1, 64]");
     if (precision != CVR_HIP_PREC_F64 && precision != CVR_HIP_PREC_F32)

@@ -142,9 +142,13 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
         steps_last = (steps_last + cvr::kStepGroup - 1) / cvr::kStepGroup * cvr::kStepGroup;
         h->layout_elems = (long long)(h->nchunks - 1) * C + 64 * steps_last;
     }
-    if (h->layout_elems > (long long)INT32_MAX + 4096)
-        return set_error(PAVA_HIP_ERANGE, "layout beyond int32 elements");
+    // layout positions (padding included) and segment-list offsets are int32
+    if (h->layout_elems > PAVA_HIP_MAX_SIZE)
+        return set_error(PAVA_HIP_ERANGE, "layout of %lld elements beyond %d", h->layout_elems, PAVA_HIP_MAX_SIZE);
     h->seg_cap = (long long)M + (long long)cvr::kSegExtra * h->nchunks + 512;
+    if (h->seg_cap > INT32_MAX)
+        return set_error(PAVA_HIP_ERANGE, "segment list of %lld entries beyond int32 (use more steps per chunk)",
+                         h->seg_cap);
     int rc;
     const int vs = value_size(h->prec);
     if ((rc = dev_alloc(h, &h->d_vals, (size_t)h->layout_elems * vs))) return rc;
@@ -330,8 +334,9 @@ static int begin_handle(const cvr_hip_opts *opts, long long nnz, cvr_hip_opts *o
 }
 
 extern "C" int cvr_hip_create(const pava_csr_matrix *A, const cvr_hip_opts *opts, cvr_hip_handle **out) {
-    int rc = pava_csr_validate(A);
-    if (rc) return rc;
+    int rc = 0;
+    if (A && (rc = check_gpu_sizes(A->numRows, A->numCols, A->nnz))) return rc;
+    if ((rc = pava_csr_validate(A))) return rc;
     cvr_hip_opts o;
     cvr_hip_handle *h = nullptr;
     if ((rc = begin_handle(opts, A->nnz, &o, out, &h))) return rc;
@@ -380,10 +385,11 @@ extern "C" int cvr_hip_create_device(int numRows, int numCols, int nnz, const in
                                      const double *d_vals, const cvr_hip_opts *opts, cvr_hip_handle **out) {
     if (numRows < 0 || numCols < 0 || nnz < 0) return set_error(PAVA_HIP_EINVAL, "negative size");
     if (!d_rowOffsets || (nnz > 0 && (!d_cols || !d_vals))) return set_error(PAVA_HIP_EINVAL, "null device CSR");
+    int rc = check_gpu_sizes(numRows, numCols, nnz);
+    if (rc) return rc;
     cvr_hip_opts o;
     cvr_hip_handle *h = nullptr;
-    int rc = begin_handle(opts, nnz, &o, out, &h);
-    if (rc) return rc;
+    if ((rc = begin_handle(opts, nnz, &o, out, &h))) return rc;
     if (o.recompute || o.keep_fp64) {
         // keep a private copy of the FP64 CSR for the recompute pass
         if ((rc = dev_alloc(h, (void **)&h->d_rp, (size_t)(numRows + 1) * 4)) ||

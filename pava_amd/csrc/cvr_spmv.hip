@@ -566,7 +566,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void long_piece_kernel(LongArg
     if (!a.rflag[r]) return;
     const int b = a.rp[r] + (p - a.poff[li]) * kPiece;
     const int re = a.rp[r + 1];
-    const double s = range_fp64(a.ci, a.va, a.x64, b, b + kPiece < re ? b + kPiece : re, lane);
+    const double s = range_fp64(a.ci, a.va, a.x64, b, re - b > kPiece ? b + kPiece : re, lane);
     if (lane == 0) a.partial[p] = s;
 }
 

@@ -525,6 +525,15 @@ extern "C" int pava_write_matrix_market(const char *path, const pava_csr_matrix 
     return 0;
 }
 
+namespace pava {
+int check_gpu_sizes(long long M, long long N, long long nnz) {
+    if (M > PAVA_HIP_MAX_SIZE || N > PAVA_HIP_MAX_SIZE || nnz > PAVA_HIP_MAX_SIZE)
+        return set_error(PAVA_HIP_ERANGE, "matrix %lld x %lld with %lld nonzeros beyond the GPU formats' limit %d", M,
+                         N, nnz, PAVA_HIP_MAX_SIZE);
+    return 0;
+}
+}  // namespace pava
+
 extern "C" int pava_csr_validate(const pava_csr_matrix *A) {
     if (!A) return set_error(PAVA_HIP_EINVAL, "null matrix");
     if (A->numRows < 0 || A->numCols < 0 || A->nnz < 0)

@@ -109,3 +109,153 @@ def test_stencil128_fp32_recompute_within_1e5(P, O, stencil128):
     _say(f"stencil128 f32+recompute: recomputed {n}, max rel {rel.max():.3e}, median {np.median(rel):.3e}")
     assert n > 0
     assert rel.max() <= 1e-5
+
+
+@pytest.mark.parametrize("k", [8, 32])
+def test_batch_of_config_shape_matrices(P, O, k):
+    """The first 64 matrices of the batching workload (rows log-uniform in
+    [1e3, 2e5], Poisson row lengths with mean U[5, 60], 70% of columns within
+    +-1000 of the diagonal, FP32 N(0,1) values; the generator and seeds of the
+    benchmark) in one persistent launch: every output within one FP32 rounding
+    plus the FP64 summation bound of k oracle SpMVs, and a graph replay equal
+    to the eager launch."""
+    import torch
+    rng = np.random.default_rng(4)
+    mats = []
+    for i in range(64):
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+        A = P.synthetic(n, P.RL_POISSON, float(rng.uniform(5, 60)), col_struct=P.COL_MIXED, half_band=1000,
+                        band_frac=0.7, value_dist=0, seed=10_000 + i)
+        A.vals = A.vals.astype(np.float32).astype(np.float64)
+        mats.append(A)
+    hs = [P.CvrHip(A, steps_per_chunk=64, precision=P.PREC_F32) for A in mats]
+    b = P.Batch(hs, k)
+    xr = np.random.default_rng(k)
+    Xs = [torch.from_numpy(xr.uniform(-1, 1, (A.numCols, k)).astype(np.float32)).cuda() for A in mats]
+    Ys = [torch.full((A.numRows, k), float("nan"), dtype=torch.float32, device="cuda") for A in mats]
+    b.set_operands([x.data_ptr() for x in Xs], [y.data_ptr() for y in Ys])
+    _say(f"batch k={k}: {len(mats)} matrices, nnz {sum(A.nnz for A in mats)}, {b.info()}")
+    s = torch.cuda.Stream()
+    b.spmm(s.cuda_stream)
+    s.synchronize()
+    first = [y.cpu().numpy() for y in Ys]
+    b.capture(s.cuda_stream)
+    for y in Ys:
+        y.fill_(float("nan"))
+    torch.cuda.synchronize()
+    b.replay(s.cuda_stream)
+    s.synchronize()
+    for Y0, y in zip(first, Ys):
+        assert np.array_equal(Y0, y.cpu().numpy())
+    b.close()
+    for h in hs:
+        h.close()
+    for A, X, Y in zip(mats, Xs, first):
+        Xh = X.cpu().numpy().astype(np.float64)
+        yref = np.stack([O.spmv_fma(A, Xh[:, j]) for j in range(k)], axis=1)
+        lens = np.diff(A.rowOffsets)[:, None]
+        sabs = np.stack([O.row_abs(A, np.abs(Xh[:, j])) for j in range(k)], axis=1)
+        bound = 2.0 ** -24 * np.abs(yref) + 2 * (lens + 64) * U * sabs + 1e-38
+        assert np.all(np.abs(Y.astype(np.float64) - yref) <= bound)
+
+
+def _create_device(P, M, N, nnz, rp, ci, va, **opts):
+    import ctypes as C
+    L = P.lib()
+    o = P._Opts()
+    L.cvr_hip_default_opts(C.byref(o))
+    for k, v in opts.items():
+        setattr(o, k, v)
+    h = C.c_void_p()
+    rc = L.cvr_hip_create_device(M, N, nnz, rp, ci, va, C.byref(o), C.byref(h))
+    if rc < 0:
+        raise P.PavaError(rc, L.pava_hip_last_error().decode())
+    return h
+
+
+def _spmv_device(P, h, dx, dy):
+    import ctypes as C
+    import torch
+    L = P.lib()
+    torch.cuda.synchronize()
+    assert L.cvr_hip_spmv(h, C.c_void_p(dx.data_ptr()), C.c_void_p(dy.data_ptr()), None) == 0
+    torch.cuda.synchronize()
+
+
+def _info(P, h):
+    import ctypes as C
+    inf = P._Info()
+    assert P.lib().cvr_hip_get_info(h, C.byref(inf)) == 0
+    return inf
+
+
+def test_create_from_device_csr_equals_host_create(P):
+    """cvr_hip_create_device on a CSR already in device memory gives the same
+    handle layout size and the same y, bit for bit, as the host create."""
+    import torch
+    for A in (P.rmat(14, 16, seed=5), P.stencil27(20, 0.01, 1)):
+        x = P.fill_uniform(A.numCols, 2)
+        with P.CvrHip(A) as hh:
+            y_host = hh.spmv_host(x)
+            n_layout = hh.info()["layout_elems"]
+        rp = torch.from_numpy(A.rowOffsets.astype(np.int32)).cuda()
+        ci = torch.from_numpy(A.cols.astype(np.int32)).cuda()
+        va = torch.from_numpy(A.vals.astype(np.float64)).cuda()
+        torch.cuda.synchronize()
+        h = _create_device(P, A.numRows, A.numCols, A.nnz, rp.data_ptr(), ci.data_ptr(), va.data_ptr())
+        try:
+            assert _info(P, h).layout_elems == n_layout
+            del rp, ci, va  # the handle keeps no reference to the caller's arrays
+            torch.cuda.empty_cache()
+            dx = torch.from_numpy(x).cuda()
+            dy = torch.full((A.numRows,), float("nan"), dtype=torch.float64, device="cuda")
+            _spmv_device(P, h, dx, dy)
+            assert np.array_equal(dy.cpu().numpy(), y_host)
+        finally:
+            P.lib().cvr_hip_destroy(h)
+
+
+def test_cvr_at_the_size_limit(P):
+    """nnz = PAVA_HIP_MAX_SIZE = 2^31 - 2^16: 4,194,176 rows of 512 elements,
+    element j of row r in column (r + 8191 j) mod N with value 1 + (r + j) mod 4,
+    and x[i] = 1 + i mod 3 (no zero products, so a dropped or repeated element
+    changes its row). Built in device memory and converted from device
+    pointers; integer products and sums far below 2^53 are exact in any order,
+    so y must equal the integer row sums exactly."""
+    import torch
+    limit = 2 ** 31 - 2 ** 16
+    M = N = limit // 512
+    dev = torch.device("cuda:0")
+    _say(f"size limit: building {M} x {N}, nnz {limit} in device memory")
+    rp = (torch.arange(M + 1, device=dev, dtype=torch.int64) * 512).to(torch.int32)
+    ci = torch.empty(limit, dtype=torch.int32, device=dev)
+    va = torch.empty(limit, dtype=torch.float64, device=dev)
+    j = torch.arange(512, device=dev, dtype=torch.int64)[None, :]
+    B = 1 << 16
+    for r0 in range(0, M, B):
+        r = torch.arange(r0, min(r0 + B, M), device=dev, dtype=torch.int64)[:, None]
+        e0, e1 = r0 * 512, (r0 + r.shape[0]) * 512
+        ci[e0:e1] = ((r + 8191 * j) % N).to(torch.int32).reshape(-1)
+        va[e0:e1] = (1 + (r + j) % 4).to(torch.float64).reshape(-1)
+    torch.cuda.synchronize()
+    _say("size limit: converting")
+    h = _create_device(P, M, N, limit, rp.data_ptr(), ci.data_ptr(), va.data_ptr())
+    try:
+        inf = _info(P, h)
+        _say(f"size limit: S={inf.steps_per_chunk} chunks={inf.nchunks} layout={inf.layout_elems}")
+        assert inf.nnz == limit and inf.layout_elems <= limit
+        del ci, va
+        torch.cuda.empty_cache()
+        x = (1 + torch.arange(N, device=dev, dtype=torch.int64) % 3).to(torch.float64)
+        dy = torch.full((M,), float("nan"), dtype=torch.float64, device=dev)
+        _spmv_device(P, h, x, dy)
+    finally:
+        P.lib().cvr_hip_destroy(h)
+    _say("size limit: checking")
+    bad = 0
+    for r0 in range(0, M, B):
+        r = torch.arange(r0, min(r0 + B, M), device=dev, dtype=torch.int64)[:, None]
+        want = ((1 + (r + j) % 4) * (1 + ((r + 8191 * j) % N) % 3)).sum(1)
+        got = dy[r0:r0 + r.shape[0]]
+        bad += int((got != want.to(torch.float64)).sum())
+    assert bad == 0

@@ -68,3 +68,41 @@ def test_print_performance_line(P, capfd):
 def test_init_vectors_pattern(P):
     x, y, yr = P.init_vectors(5, 7)
     assert x.tolist() == [0, 1, 2, 0, 1, 2, 0] and not y.any() and not yr.any()
+
+
+def test_sizes_beyond_gpu_limit_rejected_before_device(P):
+    """Every GPU format rejects a dimension or nnz above PAVA_HIP_MAX_SIZE
+    (2^31 - 2^16) with PAVA_HIP_ERANGE before it reads the arrays or touches a
+    device; the limit keeps element positions plus padding inside int32."""
+    text = (ROOT / "include" / "pava_hip.h").read_text()
+    limit = int(re.search(r"#define PAVA_HIP_MAX_SIZE (\d+)", text).group(1))
+    assert limit == 2 ** 31 - 2 ** 16
+    L = P.lib()
+    V = C.c_void_p
+    L.csr5_hip_create.argtypes = [C.POINTER(P._CSR), C.c_int, C.c_int, C.POINTER(V)]
+    L.coo_hip_create.argtypes = [C.POINTER(P._CSR), C.c_int, C.POINTER(V)]
+    L.csr_hip_create.argtypes = [C.POINTER(P._CSR), C.c_int, C.c_int, C.c_int, C.POINTER(V)]
+    rp = np.zeros(2, np.int32)
+    one = np.zeros(1, np.int32)
+    val = np.zeros(1)
+    for M, N, nnz in [(limit + 1, 4, 0), (1, limit + 1, 0), (1, 4, limit + 1)]:
+        s = P._CSR()
+        s.numRows, s.numCols, s.nnz = M, N, nnz
+        s.rowOffsets = rp.ctypes.data_as(C.POINTER(C.c_int))
+        s.cols = one.ctypes.data_as(C.POINTER(C.c_int))
+        s.vals = val.ctypes.data_as(C.POINTER(C.c_double))
+        o = P._Opts()
+        L.cvr_hip_default_opts(C.byref(o))
+        h = V()
+        calls = {
+            "cvr_hip_create": lambda: L.cvr_hip_create(C.byref(s), C.byref(o), C.byref(h)),
+            "cvr_hip_create_device": lambda: L.cvr_hip_create_device(M, N, nnz, rp.ctypes.data, one.ctypes.data,
+                                                                     val.ctypes.data, C.byref(o), C.byref(h)),
+            "csr5_hip_create": lambda: L.csr5_hip_create(C.byref(s), 16, 0, C.byref(h)),
+            "coo_hip_create": lambda: L.coo_hip_create(C.byref(s), 0, C.byref(h)),
+            "csr_hip_create": lambda: L.csr_hip_create(C.byref(s), 4, 0, 0, C.byref(h)),
+        }
+        for name, call in calls.items():
+            assert call() == -6, (name, M, N, nnz)
+            assert "limit" in L.pava_hip_last_error().decode(), name
+            assert not h.value, name
