@@ -241,6 +241,12 @@ int cvr_hip_batch_spmm(cvr_hip_batch *b, void *stream);
 int cvr_hip_batch_graph(cvr_hip_batch *b, void *stream);
 int cvr_hip_batch_replay(cvr_hip_batch *b, void *stream);
 int cvr_hip_batch_info(const cvr_hip_batch *b, int *nchunks_total, int *grid, long long *nnz);
+/* Queue check of the last completed batched SpMM (synchronises the device).
+ * The queue is cut into *nranges = min(grid, 8) ranges; range q holds work
+ * items [total*q/n, total*(q+1)/n). counts[q] (8 entries) = pops from range q:
+ * each workgroup pops until it draws an index past the range's end, so every
+ * item handed out once gives counts[q] = range size + grid (0 beyond nranges). */
+int cvr_hip_batch_queue_counts(const cvr_hip_batch *b, int *counts, int *nranges);
 int cvr_hip_batch_destroy(cvr_hip_batch *b);
 
 /* "csr_hip": plain CSR SpMV with lanes_per_row (power of two, 1..64) lanes per

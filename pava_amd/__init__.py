@@ -85,7 +85,7 @@ EXPORTED = [
     "pava_mlp_loss_grad", "pava_mlp_train", "pava_mlp_destroy", "cvr_hip_export_hot", "pava_write_csr_binary",
     "pava_read_csr_binary", "conduct_csr_hip", "csr5_hip_create", "csr5_hip_spmv", "csr5_hip_spmv_host",
     "csr5_hip_get_info", "csr5_hip_export_layout", "csr5_hip_destroy", "conduct_csr5_hip", "coo_hip_create",
-    "coo_hip_spmv", "coo_hip_get_info", "coo_hip_destroy", "conduct_coo_hip",
+    "coo_hip_spmv", "coo_hip_get_info", "coo_hip_destroy", "conduct_coo_hip", "cvr_hip_batch_queue_counts",
 ]
 
 
@@ -149,6 +149,7 @@ def lib() -> C.CDLL:
     L.cvr_hip_batch_replay.argtypes = [P, P]
     L.cvr_hip_batch_info.argtypes = [P, C.POINTER(i), C.POINTER(i), C.POINTER(C.c_longlong)]
     L.cvr_hip_batch_destroy.argtypes = [P]
+    L.cvr_hip_batch_queue_counts.argtypes = [P, C.POINTER(i), C.POINTER(i)]
     _lib = L
     return L
 
@@ -434,6 +435,13 @@ class Batch:
         t, g, n = C.c_int(), C.c_int(), C.c_longlong()
         _check(lib().cvr_hip_batch_info(self._b, C.byref(t), C.byref(g), C.byref(n)))
         return {"chunks": t.value, "grid": g.value, "nnz": n.value}
+
+    def queue_counts(self) -> tuple:
+        """(pops per queue range in the last completed launch, number of ranges)."""
+        counts = (C.c_int * 8)()
+        n = C.c_int()
+        _check(lib().cvr_hip_batch_queue_counts(self._b, counts, C.byref(n)))
+        return list(counts), n.value
 
     def close(self) -> None:
         if self._b:

@@ -733,6 +733,15 @@ extern "C" int cvr_hip_batch_info(const cvr_hip_batch *b, int *nchunks_total, in
     return 0;
 }
 
+extern "C" int cvr_hip_batch_queue_counts(const cvr_hip_batch *b, int *counts, int *nranges) {
+    if (!b || !counts) return set_error(PAVA_HIP_EINVAL, "null argument");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(counts, b->d_counter + cvr::kBatchQueues, cvr::kBatchQueues * sizeof(int), hipMemcpyDeviceToHost));
+    if (nranges) *nranges = std::min(b->grid, cvr::kBatchQueues);
+    return 0;
+}
+
 extern "C" int cvr_hip_batch_destroy(cvr_hip_batch *b) {
     if (!b) return 0;
     hipSetDevice(b->device);

@@ -15,6 +15,10 @@ constexpr int kStepGroup = 8;       // steps per chunk are a multiple of this
 constexpr int kSplitCap = 72;       // split-table entries per chunk (groups + sentinel)
 constexpr int kSegExtra = 66;       // segment-list slack per chunk (steals + padding piece)
 constexpr int kSlotCap = 160;       // split-segment slots per chunk (bound: 64 thieves + 67 owners)
+// Queue ranges of the batched SpMM (one per XCD). Its counter array holds the
+// live pop counters in [0, kBatchQueues) and, in [kBatchQueues, 2 kBatchQueues),
+// their final values in the last completed launch (saved by the fixup kernel).
+constexpr int kBatchQueues = 8;
 constexpr int kEndFlag = (int)0x80000000u;  // sign bit of a column: last element of a segment
 constexpr int kHotFlag = 0x40000000;        // bit 30: column renumbered into the hot-x buffer
 constexpr int kColMask = 0x3fffffff;

@@ -34,7 +34,6 @@ __device__ __forceinline__ void wave_sync_lds_m() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kBatchQueues = 8;  // queue ranges of the batched launch (one per XCD); counters in use
 
 // The layout, X and Y live in global memory; pointers read from a descriptor
 // in memory (the batched path) go through to_global.
@@ -344,7 +343,10 @@ __global__ void spmm_fixup_kernel(SpmmDesc a) {
 __global__ void spmm_batch_fixup_kernel(const SpmmDesc *__restrict__ descs, const int2 *__restrict__ queue, int total,
                                         int *__restrict__ counter) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < kBatchQueues) counter[t] = 0;  // the queue kernel has finished (stream order): rearm it
+    if (t < kBatchQueues) {  // the queue kernel has finished (stream order): keep its pop count, rearm it
+        counter[kBatchQueues + t] = counter[t];
+        counter[t] = 0;
+    }
     if (t < total) {
         const int2 w = queue[t];
         spmm_fix_chunk(descs[w.x], w.y);

@@ -20,6 +20,8 @@ import sys
 import numpy as np
 import pytest
 
+from test_gpu_spmm import check_queue_counts
+
 pytestmark = pytest.mark.gpu
 
 U = 2.0 ** -53
@@ -138,6 +140,7 @@ def test_batch_of_config_shape_matrices(P, O, k):
     s = torch.cuda.Stream()
     b.spmm(s.cuda_stream)
     s.synchronize()
+    check_queue_counts(b)
     first = [y.cpu().numpy() for y in Ys]
     b.capture(s.cuda_stream)
     for y in Ys:
@@ -145,6 +148,7 @@ def test_batch_of_config_shape_matrices(P, O, k):
     torch.cuda.synchronize()
     b.replay(s.cuda_stream)
     s.synchronize()
+    check_queue_counts(b)
     for Y0, y in zip(first, Ys):
         assert np.array_equal(Y0, y.cpu().numpy())
     b.close()

@@ -28,6 +28,18 @@ def _oracle_mm(O, A, X):
     return np.stack([O.spmv_fma(A, X[:, j].astype(np.float64)) for j in range(X.shape[1])], axis=1)
 
 
+def check_queue_counts(b):
+    """Every work item of the last batched launch was handed out exactly once:
+    pops per queue range = range size + one empty pop per workgroup."""
+    counts, nq = b.queue_counts()
+    inf = b.info()
+    total, grid = inf["chunks"], inf["grid"]
+    assert nq == min(grid, 8)
+    for q in range(8):
+        want = total * (q + 1) // nq - total * q // nq + grid if q < nq else 0
+        assert counts[q] == want, (q, counts, total, grid)
+
+
 def _bound(O, A, X, yref):
     lens = np.diff(A.rowOffsets)[:, None]
     sabs = np.stack([O.row_abs(A, np.abs(X[:, j]).astype(np.float64)) for j in range(X.shape[1])], axis=1)
@@ -88,6 +100,7 @@ def test_batched_persistent_and_graph_replay(P, O, k):
     s = torch.cuda.Stream()
     b.spmm(s.cuda_stream)
     s.synchronize()
+    check_queue_counts(b)
     first = [y.cpu().numpy().astype(np.float64) for y in Ys]
     for A, X, Y in zip(mats, Xs, first):
         yref = _oracle_mm(O, A, X.cpu().numpy())
@@ -100,6 +113,7 @@ def test_batched_persistent_and_graph_replay(P, O, k):
         torch.cuda.synchronize()
         b.replay(s.cuda_stream)
         s.synchronize()
+        check_queue_counts(b)
         for Y0, y in zip(first, Ys):
             assert np.array_equal(Y0, y.cpu().numpy().astype(np.float64)), rep
     b.close()
