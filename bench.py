@@ -157,11 +157,12 @@ def run_precision(P, A, x, steps: int, S: int = 0) -> dict:
     yref = P.reference_spmv(A, x)
     x64 = torch.from_numpy(x).to(dev)
     x32 = x64.float()
-    variants = [("f64", dict(precision=0), 8, 4, 8),
-                ("f32", dict(precision=1), 4, 4, 4),
-                ("f32_recompute", dict(precision=1, recompute=True), 4, 4, 4),
+    variants = [("f64", dict(precision=0, col_index_bits=32), 8, 4, 8),
+                ("f64_col16", dict(precision=0, col_index_bits=16), 8, 2.125, 8),
+                ("f32", dict(precision=1, col_index_bits=32), 4, 4, 4),
+                ("f32_recompute", dict(precision=1, recompute=True, col_index_bits=32), 4, 4, 4),
                 ("f32_col16_recompute", dict(precision=1, recompute=True, col_index_bits=16), 4, 2.125, 4),
-                ("bf16", dict(precision=2), 2, 4, 4),
+                ("bf16", dict(precision=2, col_index_bits=32), 2, 4, 4),
                 ("bf16_col16", dict(precision=2, col_index_bits=16), 2, 2.125, 4)]
     out = {}
     for name, kw, sv, si, sx in variants:
@@ -286,6 +287,9 @@ def main():
     ap.add_argument("--x-hot-kb", type=int, default=-1,
                     help="hot-x staging buffer (KiB; 0 = off; -1 = automatic: 2048 KiB when column use is skewed): "
                          "the most used columns read from a dense copy")
+    ap.add_argument("--col-index-bits", type=int, default=0, choices=[0, 16, 32],
+                    help="CVR column width: 32, 16 (chunk-relative offsets + end masks), 0 = automatic (16 when "
+                         "every chunk spans fewer than 65,536 columns)")
     ap.add_argument("--x-lds-kb", type=int, default=-1,
                     help="of the hot-x buffer, KiB held in LDS per workgroup (-1 = as much as fits, 0 = none)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -329,7 +333,7 @@ def main():
           f"gen {t_gen:.1f}s", file=sys.stderr, flush=True)
 
     h = P.CvrHip(A, steps_per_chunk=args.steps_per_chunk, precision=w["prec"], x_hot_kb=args.x_hot_kb,
-                 x_lds_kb=args.x_lds_kb)
+                 x_lds_kb=args.x_lds_kb, col_index_bits=args.col_index_bits)
     info = h.info()
     S = info["steps_per_chunk"]
     dev = torch.device("cuda:0")
@@ -356,7 +360,9 @@ def main():
 
     B = algorithmic_bytes(A.numRows, A.numCols, A.nnz)
     value = B / (ms_step * 1e-3) / 1e9
-    kernel = "spmv_kernel<double,double,double,false,false,false,%s>" % ("true" if info["n_lds_cols"] else "false")
+    col16 = info["col_index_bits"] == 16
+    kernel = "spmv_kernel<double,double,double,false,%s,false,%s>" % ("true" if col16 else "false",
+                                                                      "true" if info["n_lds_cols"] else "false")
     achieved = B / (ms_main * 1e-3) / 1e9
 
     result = {
@@ -376,7 +382,8 @@ def main():
         "config": {"workload": args.workload, "rows": A.numRows, "cols": A.numCols, "nnz": A.nnz,
                    "max_row_len": int(np.diff(A.rowOffsets).max()), "steps_per_chunk": S,
                    "chunks": info["nchunks"], "algorithmic_bytes": B, "x_hot_kb": args.x_hot_kb,
-                   "hot_columns": info["n_hot_cols"], "lds_columns": info["n_lds_cols"]},
+                   "hot_columns": info["n_hot_cols"], "lds_columns": info["n_lds_cols"],
+                   "col_index_bits": info["col_index_bits"]},
         "nnz_per_s": A.nnz / (ms_step * 1e-3),
         "gflops": 2.0 * A.nnz / (ms_step * 1e-3) / 1e9,
         "hbm_frac": value / HBM_PEAK_GBS,

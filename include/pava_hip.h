@@ -170,7 +170,10 @@ typedef struct cvr_hip_opts {
                                 (default): about four chunks per resident wave slot of the device,
                                 at most 256 (cvr_hip_get_info reports the value used) */
     int precision;           /* CVR_HIP_PREC_* */
-    int col_index_bits;      /* 32, or 16 (chunk-relative offsets; mixed precision only) */
+    int col_index_bits;      /* 32; 16: chunk-relative offsets plus a 64-bit end mask per step
+                                (2.125 B/nnz instead of 4; PAVA_HIP_ERANGE if a chunk spans
+                                65,536 columns or more; no hot-x staging); 0 = automatic
+                                (default): 16 when every chunk fits and x_hot_kb <= 0, else 32 */
     int recompute;           /* mixed precision: recompute rows with kappa > kappa_threshold in FP64 */
     double kappa_threshold;  /* sum|a*x| / |sum a*x| above which a row is recomputed (default 64:
                                 FP32 storage then stays within ~1e-5 relative of FP64) */

@@ -288,7 +288,7 @@ def check_results(y: np.ndarray, y_ref: np.ndarray) -> int:
 class CvrHip:
     """A CVR-64 matrix resident on one GPU (the cvr_hip format plugin)."""
 
-    def __init__(self, A: CSR, steps_per_chunk: int = 0, precision: int = PREC_F64, col_index_bits: int = 32,
+    def __init__(self, A: CSR, steps_per_chunk: int = 0, precision: int = PREC_F64, col_index_bits: int = 0,
                  recompute: bool = False, kappa_threshold: float = 64.0, keep_fp64: bool = False, device: int = 0,
                  x_hot_kb: int = -1, x_lds_kb: int = -1):
         L = lib()

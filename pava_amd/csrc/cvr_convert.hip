@@ -793,7 +793,40 @@ hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, 
     return finish_hot(hc, N, colmap, hot_cols, cols, L, n_hot, st);
 }
 
+// Automatic column width: *flag becomes nonzero if the nonzeros of some chunk
+// (the CSR elements [C c, C (c+1)), one workgroup per chunk) span 65,536
+// columns or more, so that 16-bit chunk-relative offsets cannot encode them.
+__global__ __launch_bounds__(256) void chunk_span_kernel(const int *__restrict__ ci, long long nnz, long long C,
+                                                         int *__restrict__ flag) {
+    __shared__ int smn[4], smx[4];
+    const long long b = C * blockIdx.x, e = b + C < nnz ? b + C : nnz;
+    int mn = 0x7fffffff, mx = -1;
+    for (long long k = b + threadIdx.x; k < e; k += blockDim.x) {
+        const int v = ci[k];
+        mn = min(mn, v);
+        mx = max(mx, v);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        mn = min(mn, __shfl_xor(mn, d));
+        mx = max(mx, __shfl_xor(mx, d));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) smn[w] = mn, smx[w] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < 4; ++i) mn = min(mn, smn[i]), mx = max(mx, smx[i]);
+        if (mx - mn >= 65536) atomicOr(flag, 1);
+    }
+}
+
 // Host-side launchers --------------------------------------------------------
+hipError_t launch_chunk_span(const int *ci, long long nnz, long long C, int nchunks, int *flag, hipStream_t st) {
+    if (nchunks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(chunk_span_kernel, dim3((unsigned)nchunks), dim3(256), 0, st, ci, nnz, C, flag);
+    return hipGetLastError();
+}
+
 hipError_t launch_empty_rows(const int *rp, int M, unsigned *bits, int *status, hipStream_t st) {
     int nw = (M + 31) >> 5;
     if (nw == 0) return hipSuccess;

@@ -80,7 +80,7 @@ extern "C" void cvr_hip_default_opts(cvr_hip_opts *o) {
     memset(o, 0, sizeof *o);
     o->steps_per_chunk = 0;  // automatic
     o->precision = CVR_HIP_PREC_F64;
-    o->col_index_bits = 32;
+    o->col_index_bits = 0;  // automatic
     o->recompute = 0;
     o->kappa_threshold = 64.0;
     o->keep_fp64 = 0;
@@ -115,10 +115,8 @@ static int validate_opts(const cvr_hip_opts *o) {
         return set_error(PAVA_HIP_EINVAL, "steps_per_chunk must be a multiple of 8 in [8, 4096], got %d",
                          o->steps_per_chunk);
     if (o->precision < 0 || o->precision > 2) return set_error(PAVA_HIP_EINVAL, "bad precision %d", o->precision);
-    if (o->col_index_bits != 32 && o->col_index_bits != 16)
-        return set_error(PAVA_HIP_EINVAL, "col_index_bits must be 32 or 16");
-    if (o->col_index_bits == 16 && o->precision == CVR_HIP_PREC_F64)
-        return set_error(PAVA_HIP_EINVAL, "16-bit columns are offered for FP32/BF16 storage only");
+    if (o->col_index_bits != 0 && o->col_index_bits != 32 && o->col_index_bits != 16)
+        return set_error(PAVA_HIP_EINVAL, "col_index_bits must be 0 (automatic), 16 or 32, got %d", o->col_index_bits);
     if (o->x_hot_kb < -1) return set_error(PAVA_HIP_EINVAL, "x_hot_kb must be >= -1, got %d", o->x_hot_kb);
     if (o->x_lds_kb < -1) return set_error(PAVA_HIP_EINVAL, "x_lds_kb must be >= -1, got %d", o->x_lds_kb);
     return 0;
@@ -133,9 +131,26 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     h->nnz = nnz;
     h->S = o->steps_per_chunk;
     h->prec = o->precision;
-    h->col16 = (o->col_index_bits == 16);
     const long long C = 64LL * h->S;
     h->nchunks = nnz > 0 ? (int)((nnz + C - 1) / C) : 0;
+    h->col16 = (o->col_index_bits == 16);
+    if (o->col_index_bits == 0 && h->nchunks > 0 && o->x_hot_kb <= 0) {
+        // automatic: 16-bit chunk-relative columns when every chunk's nonzeros
+        // span fewer than 65,536 columns (an explicit hot-x request keeps 32)
+        if (N <= 65536) {
+            h->col16 = 1;
+        } else {
+            int *d_flag = nullptr, flag = 1;
+            hipError_t e = hipMalloc(&d_flag, 4);
+            if (e == hipSuccess) e = hipMemsetAsync(d_flag, 0, 4, h->stream);
+            if (e == hipSuccess) e = cvr::launch_chunk_span(d_ci, nnz, C, h->nchunks, d_flag, h->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            (void)hipFree(d_flag);
+            if (e != hipSuccess) return set_error(PAVA_HIP_EDEVICE, "column span check: %s", hipGetErrorString(e));
+            h->col16 = flag == 0;
+        }
+    }
     if (h->nchunks > 0) {
         long long nlast = nnz - (long long)(h->nchunks - 1) * C;
         long long steps_last = (nlast + 63) / 64;
@@ -564,7 +579,6 @@ extern "C" int cvr_hip_spmv_host(cvr_hip_handle *h, const double *x, double *y) 
 
 static int spmm_desc(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, cvr::SpmmDesc *a) {
     if (k < 1 || k > 32) return set_error(PAVA_HIP_EINVAL, "k must be in [1, 32], got %d", k);
-    if (h->col16) return set_error(PAVA_HIP_ESTATE, "SpMM needs a 32-bit column layout");
     if ((!d_X && h->N > 0) || (!d_Y && h->M > 0)) return set_error(PAVA_HIP_EINVAL, "null X or Y");
     if (k == cvr::spmm_kp(k) && (((uintptr_t)d_X | (uintptr_t)d_Y) & 15))
         return set_error(PAVA_HIP_EINVAL, "X and Y must be 16-byte aligned");
@@ -582,7 +596,10 @@ static int spmm_desc(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, cvr
     a->split_tab = h->d_split_tab;
     a->empty_bits = h->d_empty;
     a->fix = h->d_fix;
-    a->hot_cols = h->d_hot_cols;
+    a->hot_cols = h->n_hot > 0 ? h->d_hot_cols : nullptr;
+    a->col_base = h->d_col_base;
+    a->end_mask = h->d_end_mask;
+    a->col16 = h->col16;
     a->X = d_X;
     a->Y = d_Y;
     a->carry = h->d_carry_mm;
@@ -640,10 +657,9 @@ extern "C" int cvr_hip_batch_create(int nmat, cvr_hip_handle *const *handles, in
     std::vector<int2> queue;
     for (int i = 0; i < nmat; ++i) {
         cvr_hip_handle *h = handles[i];
-        if (!h || h->device != b->device || h->prec != b->prec || h->col16 || h->nchunks == 0) {
+        if (!h || h->device != b->device || h->prec != b->prec || h->nchunks == 0) {
             delete b;
-            return set_error(PAVA_HIP_EINVAL,
-                             "batch matrices must be non-empty and share device and precision, 32-bit columns");
+            return set_error(PAVA_HIP_EINVAL, "batch matrices must be non-empty and share device and precision");
         }
         b->max_slots = std::max(b->max_slots, h->max_slots);
         b->nnz += h->nnz;

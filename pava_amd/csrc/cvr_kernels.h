@@ -143,11 +143,13 @@ struct SpmmDesc {
     const unsigned *empty_bits;
     const int *fix;
     const int *hot_cols;  // column of each hot index (layouts with hot-x renumbering)
+    const int *col_base;          // 16-bit layouts: smallest column of each chunk
+    const uint64_t *end_mask;     // 16-bit layouts: segment-end mask of each step
     const float *X;   // numCols x k, row-major
     float *Y;         // numRows x k, row-major
     double *carry;    // 2 * nchunks * k
     long long layout_elems;
-    int nchunks, S, M, k, has_empty, pad_;
+    int nchunks, S, M, k, has_empty, col16;
 };
 
 int spmm_kp(int k);
@@ -158,6 +160,8 @@ hipError_t launch_spmm_batch(const SpmmDesc *d_descs, const int2 *d_queue, int t
                              size_t lds, int grid, hipStream_t st);
 
 hipError_t launch_empty_rows(const int *rp, int M, unsigned *bits, int *status, hipStream_t st);
+// *flag |= 1 if a chunk of C elements of ci spans 65,536 columns or more (flag cleared by the caller)
+hipError_t launch_chunk_span(const int *ci, long long nnz, long long C, int nchunks, int *flag, hipStream_t st);
 hipError_t launch_convert(const ConvertArgs &a, hipStream_t st);
 hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st);
 // LDS table size for a handle: n_lds (0 = no LDS variant) and its grid.

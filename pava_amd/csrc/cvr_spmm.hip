@@ -70,7 +70,17 @@ __device__ __noinline__ float4 xpart_partial(gptr<const float> p, int j0, int k)
     return q;
 }
 
-template <typename VT, int KP>
+// Segment-end mask of step i of a group: the group's masks arrive in one
+// vector load, mask i in lane i, read out with readlane.
+__device__ __forceinline__ unsigned long long group_step_mask(unsigned long long em, int i) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)em, i);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(em >> 32), i);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// C16: 16-bit layouts (chunk-relative column offsets, end flags in a mask per
+// step) instead of 32-bit columns whose sign bit flags a segment end.
+template <typename VT, int KP, bool C16>
 __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const int lane, double *slots,
                            int *segbuf) {
     constexpr int TPL = KP / 4;   // threads per CVR lane
@@ -89,15 +99,31 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
     const gptr<const int> segs = to_global(a.seg + zr + kSegExtra * c);
     const gptr<const VT> vp = to_global(reinterpret_cast<const VT *>(a.vals) + base + cl);
     const gptr<const int> cpa = to_global(a.cols + base + lane);  // the step's 64 columns, one per thread
+    const gptr<const unsigned short> cp16 =
+        to_global(reinterpret_cast<const unsigned short *>(a.cols) + base + lane);
+    const gptr<const uint64_t> emp = to_global(a.end_mask + (base >> 6));
+    const int cbase = C16 ? a.col_base[c] : 0;
     const gptr<const float> X = to_global(a.X);
     const gptr<const int> hot_cols = to_global(a.hot_cols);
     const gptr<float> Y = to_global(a.Y);
     const int k = a.k;
     const bool full = (k == KP);
 
-    auto load_cols = [&](int g0, int (&cc)[G]) {
+    auto load_cols = [&](int g0, int (&cc)[G], unsigned long long &em) {
+        if constexpr (C16) {
 #pragma unroll
-        for (int i = 0; i < G; ++i) cc[i] = __builtin_nontemporal_load(&cpa[(size_t)(g0 + i) * 64]);
+            for (int i = 0; i < G; ++i) cc[i] = (int)cp16[(size_t)(g0 + i) * 64];
+            em = emp[g0 + (lane & (G - 1))];
+        } else {
+#pragma unroll
+            for (int i = 0; i < G; ++i) cc[i] = __builtin_nontemporal_load(&cpa[(size_t)(g0 + i) * 64]);
+        }
+    };
+    auto step_ends = [&](int cv, unsigned long long em, int i) -> unsigned long long {
+        if constexpr (C16)
+            return group_step_mask(em, i);
+        else
+            return __ballot(cv < 0);
     };
     // this thread's 4 values of the X rows of a group's stored columns: the
     // hot-column lookups (layouts with hot-x renumbering) are issued first, so
@@ -107,7 +133,10 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
         int col[G];
 #pragma unroll
         for (int i = 0; i < G; ++i) col[i] = __shfl(cc[i], cl);
-        if (has_hot) {
+        if constexpr (C16) {
+#pragma unroll
+            for (int i = 0; i < G; ++i) col[i] += cbase;
+        } else if (has_hot) {
 #pragma unroll
             for (int i = 0; i < G; ++i)
                 col[i] = (col[i] & kHotFlag) ? hot_cols[col[i] & kColMask] : (col[i] & kColMask);
@@ -126,10 +155,10 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
             }
         }
     };
-    auto group_ends = [&](const int (&cc)[G]) -> int {
+    auto group_ends = [&](const int (&cc)[G], unsigned long long em) -> int {
         int e = 0;
 #pragma unroll
-        for (int i = 0; i < G; ++i) e += __popcll(__ballot(cc[i] < 0));
+        for (int i = 0; i < G; ++i) e += __popcll(step_ends(cc[i], em, i));
         return e;
     };
     auto load_segs = [&](int sb, int ne, int (&r)[G]) {
@@ -141,17 +170,18 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     VT vb[G], vn[G];
     int cb[G], cn[G], cnn[G];
+    unsigned long long eb = 0, en = 0, enn = 0;
     float4 xb[G], xn[G];
     int sc[G], sn[G];
 
     // prologue: group 0 issued, columns of group 1 in flight
-    load_cols(0, cb);
+    load_cols(0, cb, eb);
 #pragma unroll
     for (int i = 0; i < G; ++i) vb[i] = __builtin_nontemporal_load(&vp[(size_t)i * 64]);
     xgroup(cb, xb);
-    int ecur = group_ends(cb);
+    int ecur = group_ends(cb, eb);
     load_segs(0, ecur, sc);
-    if (G < steps) load_cols(G, cn);
+    if (G < steps) load_cols(G, cn, en);
     // the loop is entered with nothing in flight (its wait counts then follow
     // its own loads only)
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
@@ -163,11 +193,11 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
         int enext = 0;
         if (has1) {
             xgroup(cn, xn);
-            enext = group_ends(cn);
+            enext = group_ends(cn, en);
             load_segs(sbase_next, enext, sn);
 #pragma unroll
             for (int i = 0; i < G; ++i) vn[i] = __builtin_nontemporal_load(&vp[(size_t)(g + G + i) * 64]);
-            if (has2) load_cols(g + 2 * G, cnn);
+            if (has2) load_cols(g + 2 * G, cnn, enn);
         }
         // this group's destinations -> this wave's LDS copy
 #pragma unroll
@@ -181,7 +211,7 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
             acc[1] = __builtin_fma(v, (double)xb[i].y, acc[1]);
             acc[2] = __builtin_fma(v, (double)xb[i].z, acc[2]);
             acc[3] = __builtin_fma(v, (double)xb[i].w, acc[3]);
-            const unsigned long long m = __ballot(cb[i] < 0);
+            const unsigned long long m = step_ends(cb[i], eb, i);
             if (m) {
                 if ((m >> cl) & 1ull) {
                     const int rank = __popcll(m & ((1ull << cl) - 1ull));
@@ -215,6 +245,8 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
                 cn[i] = cnn[i];
                 sc[i] = sn[i];
             }
+            eb = en;
+            en = enn;
             sbase = sbase_next;
             ecur = enext;
         }
@@ -271,7 +303,10 @@ static inline int spmm_block(int kp) { return 16 * kp; }
 template <typename VT, int KP>
 __device__ __forceinline__ void spmm_any_chunk(const SpmmDesc &a, int c, double *smem, int *segbuf) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    spmm_chunk<VT, KP>(a, c, wv, lane, smem, segbuf + wv * 64 * kStepGroup);
+    if (a.col16)
+        spmm_chunk<VT, KP, true>(a, c, wv, lane, smem, segbuf + wv * 64 * kStepGroup);
+    else
+        spmm_chunk<VT, KP, false>(a, c, wv, lane, smem, segbuf + wv * 64 * kStepGroup);
 }
 
 template <typename VT, int KP>

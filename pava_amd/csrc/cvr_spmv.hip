@@ -103,12 +103,12 @@ __device__ __forceinline__ void store_y(void *y, int r, double v) {
 // each wave walks chunks of the workgroup's contiguous chunk range; a gather
 // of such a column is an LDS read instead of a divergent global load.
 template <typename VT, typename XT, typename YT, bool COMP, bool COL16, bool KAPPA, bool LDSX>
-// 32-bit layouts: >= 4 waves per SIMD (128 VGPRs, no spills); the LDS variant
-// runs one workgroup of kWavesLds waves per CU (its table fills the LDS), so
-// 2 waves per SIMD and up to 256 VGPRs; the 16-bit layouts are left
-// unconstrained
+// 32-bit layouts and FP64 16-bit layouts: >= 4 waves per SIMD (128 VGPRs, no
+// spills); the LDS variant runs one workgroup of kWavesLds waves per CU (its
+// table fills the LDS), so 2 waves per SIMD and up to 256 VGPRs; the
+// compensated 16-bit layouts are left unconstrained
 __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
-                             LDSX ? kWavesLds / 4 : (COL16 ? 1 : 4)) void spmv_kernel(
+                             LDSX ? kWavesLds / 4 : (COL16 && COMP ? 1 : 4)) void spmv_kernel(
     SpmvArgs a) {
     constexpr int WPB = LDSX ? kWavesLds : kWavesPerBlock;
     const int lane = threadIdx.x & 63;
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
         const uint64_t ax = reinterpret_cast<uint64_t>(x), ah = reinterpret_cast<uint64_t>(xh);
         auto xval = [&](int cv) -> XT {
             if constexpr (COL16) {
-                return x[cbase + cv];
+                return load_global<XT>(ax + (uint64_t)(uint32_t)(cbase + cv) * sizeof(XT));
             } else {
                 const uint32_t idx = (uint32_t)(cv & kColMask);
                 if constexpr (LDSX) {
@@ -644,7 +644,11 @@ hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st) {
     if (a.n_lds > 0 && (col16 || a.lds_blocks <= 0)) return hipErrorInvalidValue;
     dim3 grid((a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock);
     if (prec == 0) {
-        launch_one<double, double, double, false, false>(a, grid, st);
+        if (col16)
+            hipLaunchKernelGGL((spmv_kernel<double, double, double, false, true, false, false>), grid,
+                               dim3(64 * kWavesPerBlock), 0, st, a);
+        else
+            launch_one<double, double, double, false, false>(a, grid, st);
     } else if (prec == 1) {
         if (col16)
             launch_mixed<float, true>(a, grid, st);

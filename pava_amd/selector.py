@@ -80,10 +80,13 @@ def build(A: CSR, cfg: Config):
         return CsrHip(A, lanes_per_row=cfg.tile, precision=PREC_F64)
     if cfg.fmt == "csr5":
         return Csr5Hip(A, sigma=cfg.tile)
+    # fixed configs fix the column width; the library default (xs < 0) keeps
+    # its automatic choices of column width and hot-x staging
+    bits = 16 if cfg.prec == "f32rc16" else 0 if cfg.xs < 0 else 32
     if cfg.prec == "f64":
-        return CvrHip(A, steps_per_chunk=cfg.tile, x_hot_kb=cfg.xs)
+        return CvrHip(A, steps_per_chunk=cfg.tile, x_hot_kb=cfg.xs, col_index_bits=bits)
     return CvrHip(A, steps_per_chunk=cfg.tile, precision=PREC_F32, recompute=True, kappa_threshold=64.0,
-                  col_index_bits=16 if cfg.prec == "f32rc16" else 32, x_hot_kb=cfg.xs)
+                  col_index_bits=bits, x_hot_kb=cfg.xs)
 
 
 def autotune(A: CSR, x: np.ndarray, iters: int = 5, configs=CONFIGS) -> np.ndarray:

@@ -56,7 +56,7 @@ def _compare_layout(A, gpu, emu, S):
 def test_gpu_layout_equals_emulation(P, O, mats, S):
     for name, A in mats.items():
         _say(f"layout {name} S={S} m={A.numRows} nnz={A.nnz}")
-        with P.CvrHip(A, steps_per_chunk=S, x_hot_kb=0) as h:
+        with P.CvrHip(A, steps_per_chunk=S, x_hot_kb=0, col_index_bits=32) as h:
             gpu = h.export_layout()
         emu = O.cvr64_build(A, S, 0)
         _compare_layout(A, gpu, emu, S)
@@ -158,7 +158,7 @@ def test_automatic_hot_x_only_on_skewed_column_use(P, O):
              "laplacian": (P.laplacian2d(200, 150), False)}
     for name, (A, skewed) in cases.items():
         x = P.fill_uniform(A.numCols, 3)
-        with P.CvrHip(A, steps_per_chunk=64, x_hot_kb=-1) as h:
+        with P.CvrHip(A, steps_per_chunk=64, x_hot_kb=-1, col_index_bits=32) as h:
             n_hot = h.info()["n_hot_cols"]
             y = h.spmv_host(x)
         counts = np.bincount(A.cols, minlength=A.numCols)
@@ -176,7 +176,7 @@ def test_lds_hot_table_bit_identical(P, O, prec, recompute):
     A = P.rmat(16, 16, seed=3)
     x = P.fill_uniform(A.numCols, 9)
     _say(f"lds-hot prec={prec} recompute={recompute} nnz={A.nnz}")
-    kw = dict(steps_per_chunk=64, precision=prec, recompute=recompute, keep_fp64=recompute)
+    kw = dict(steps_per_chunk=64, precision=prec, recompute=recompute, keep_fp64=recompute, col_index_bits=32)
     with P.CvrHip(A, x_hot_kb=256, **kw) as h, P.CvrHip(A, x_hot_kb=0, **kw) as h0:
         assert h.info()["n_lds_cols"] >= 1024
         y, y0 = h.spmv_host(x), h0.spmv_host(x)
@@ -296,7 +296,7 @@ def test_automatic_chunk_size(P, O):
 def test_invalid_options_rejected(P):
     A = P.laplacian2d(10, 10)
     for kw in [dict(steps_per_chunk=7), dict(steps_per_chunk=8192), dict(precision=5),
-               dict(col_index_bits=16)]:
+               dict(col_index_bits=12)]:
         with pytest.raises(P.PavaError):
             P.CvrHip(A, **kw)
 
@@ -400,6 +400,67 @@ def test_col16_layout_and_spmv(P, O):
     real = ~((emu["vals"] == 0) & ((emu["cols"] & 0x7FFFFFFF) == 0))
     assert np.array_equal(full[real], (emu["cols"] & 0x7FFFFFFF)[real])
     assert np.array_equal(ends, emu["cols"] < 0)
+
+
+def test_fp64_col16_equals_32bit_layout(P, O, mats):
+    """FP64 storage with 16-bit chunk-relative columns: same layout order, so
+    the same y bit for bit as the 32-bit layout; a chunk spanning 65,536
+    columns or more is refused with PAVA_HIP_ERANGE."""
+    cases = dict(mats)
+    cases["stencil"] = P.stencil27(24, 0.01, 3)
+    cases["laplacian"] = P.laplacian2d(300, 200)
+    done = 0
+    for name, A in cases.items():
+        x = P.fill_uniform(A.numCols, 6)
+        for S in (8, 64):
+            with P.CvrHip(A, steps_per_chunk=S, x_hot_kb=0, col_index_bits=32) as h32:
+                y32 = h32.spmv_host(x)
+            try:
+                h16 = P.CvrHip(A, steps_per_chunk=S, col_index_bits=16)
+            except P.PavaError as e:
+                assert e.code == -6, (name, S)
+                continue
+            with h16:
+                assert h16.info()["col_index_bits"] == 16
+                y16 = h16.spmv_host(x)
+            assert np.array_equal(y16, y32), (name, S)
+            done += 1
+    wide = P.CSR(2, 200000, np.array([0, 2, 3]), np.array([0, 150000, 7]), np.ones(3))
+    with pytest.raises(P.PavaError) as e:
+        P.CvrHip(wide, steps_per_chunk=8, col_index_bits=16)
+    assert e.value.code == -6
+    assert done >= 4
+
+
+def test_automatic_column_width(P, O):
+    """col_index_bits = 0 (the default) takes 16-bit columns when every chunk's
+    nonzeros span fewer than 65,536 columns (always when numCols <= 65,536),
+    32-bit otherwise or when hot-x staging is requested explicitly; y is the
+    32-bit layout's either way."""
+    rng = np.random.default_rng(12)
+    n = 300000
+    rows = np.arange(n)
+    def csr(other):  # row r: columns r and other[r]
+        c = np.sort(np.stack([rows, other], 1), 1)
+        two = c[:, 1] != c[:, 0]
+        cols = np.stack([c[:, 0], np.where(two, c[:, 1], -1)], 1).reshape(-1)
+        rp = np.concatenate([[0], np.cumsum(1 + two)]).astype(np.int32)
+        return P.CSR(n, n, rp, cols[cols >= 0].astype(np.int32), rng.uniform(-1, 1, int(rp[-1])))
+    band = csr(np.minimum(rows + 1000, n - 1))     # a chunk spans its rows plus 1,000 columns
+    far = csr((rows * 7919 + 12345) % n)           # a chunk spans the whole matrix
+    cases = [("banded", band, 16, {}), ("scattered", far, 32, {}),
+             ("small rmat", P.rmat(14, 16, seed=2), 16, {}), ("explicit hot-x", P.rmat(14, 16, seed=2), 32,
+                                                              dict(x_hot_kb=64))]
+    for name, A, bits, kw in cases:
+        x = P.fill_uniform(A.numCols, 1)
+        with P.CvrHip(A, steps_per_chunk=64, **kw) as h:
+            got = h.info()["col_index_bits"]
+            y = h.spmv_host(x)
+        with P.CvrHip(A, steps_per_chunk=64, x_hot_kb=0, col_index_bits=32) as h:
+            y32 = h.spmv_host(x)
+        _say(f"auto column width {name}: {got} bits")
+        assert got == bits, name
+        assert np.array_equal(y, y32), name
 
 
 def test_recompute_needs_fp64_x(P):

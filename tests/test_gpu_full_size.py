@@ -81,6 +81,15 @@ def test_stencil128_fp64_defaults(P, O, stencil128):
     assert inf["n_hot_cols"] == 0  # even column use: no staging
 
 
+def test_stencil128_fp64_col16_equals_emulation(P, O, stencil128):
+    A = stencil128
+    x = P.fill_uniform(A.numCols, 0)
+    with P.CvrHip(A, col_index_bits=16) as h:
+        S = h.info()["steps_per_chunk"]
+        y = h.spmv_host(x)
+    assert np.array_equal(y, O.cvr64_spmv(A, O.cvr64_build(A, S, 0), x))
+
+
 @pytest.mark.parametrize("prec", [1, 2])
 def test_stencil128_low_precision_equals_emulation(P, O, stencil128, prec):
     A = stencil128

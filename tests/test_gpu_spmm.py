@@ -91,7 +91,9 @@ def test_batched_persistent_and_graph_replay(P, O, k):
         A = P.synthetic(n, P.RL_POISSON, float(rng.uniform(5, 60)), col_struct=P.COL_MIXED, half_band=1000,
                         band_frac=0.7, seed=1000 + i)
         mats.append(_f32_matrix(A))
-    hs = [P.CvrHip(A, steps_per_chunk=32, precision=P.PREC_F32) for A in mats]
+    # 16- and 32-bit column layouts mixed in one batch
+    hs = [P.CvrHip(A, steps_per_chunk=32, precision=P.PREC_F32, col_index_bits=16 if i % 2 else 32, x_hot_kb=0)
+          for i, A in enumerate(mats)]
     b = P.Batch(hs, k)
     Xs = [torch.from_numpy(rng.uniform(-1, 1, (A.numCols, k)).astype(np.float32)).cuda() for A in mats]
     Ys = [torch.full((A.numRows, k), float("nan"), dtype=torch.float32, device="cuda") for A in mats]
@@ -121,8 +123,9 @@ def test_batched_persistent_and_graph_replay(P, O, k):
         h.close()
 
 
+@pytest.mark.parametrize("bits", [16, 32])
 @pytest.mark.parametrize("k", [3, 8, 12, 16, 32])
-def test_spmm_bit_exact_vs_cvr_order(P, O, k):
+def test_spmm_bit_exact_vs_cvr_order(P, O, k, bits):
     """FP32 products are exact in FP64, so every output is the FP32 rounding of
     the FP64 CVR-order sum (the SpMV emulation at the same chunk size, column by
     column), including split rows and rows spanning chunks."""
@@ -136,7 +139,8 @@ def test_spmm_bit_exact_vs_cvr_order(P, O, k):
         A = _f32_matrix(A)
         S = 16
         X = rng.uniform(-1, 1, (A.numCols, k)).astype(np.float32)
-        with P.CvrHip(A, steps_per_chunk=S, precision=P.PREC_F32) as h:
+        with P.CvrHip(A, steps_per_chunk=S, precision=P.PREC_F32, col_index_bits=bits, x_hot_kb=0) as h:
+            assert h.info()["col_index_bits"] == bits
             dX = torch.from_numpy(X).cuda()
             dY = torch.full((A.numRows, k), float("nan"), dtype=torch.float32, device="cuda")
             h.spmm(k, dX.data_ptr(), dY.data_ptr())

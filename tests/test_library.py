@@ -39,7 +39,7 @@ def test_version(P):
 
 
 @pytest.mark.parametrize("kw", [dict(steps_per_chunk=7), dict(steps_per_chunk=8192), dict(precision=5),
-                                dict(col_index_bits=16), dict(col_index_bits=12)])
+                                dict(col_index_bits=12), dict(x_hot_kb=-2)])
 def test_invalid_options_rejected_before_device(P, kw):
     A = P.laplacian2d(10, 10)
     with pytest.raises(P.PavaError) as e:
