@@ -463,6 +463,30 @@ def test_automatic_column_width(P, O):
         assert np.array_equal(y, y32), name
 
 
+@pytest.mark.parametrize("span,bits", [(65535, 16), (65536, 32)])
+def test_column_width_boundary(P, span, bits):
+    """A chunk whose columns span exactly 65,535 (largest 16-bit offset)
+    takes 16-bit columns automatically; one more column forces 32 bits, and an
+    explicit 16-bit request is then refused."""
+    n = 200000
+    rows = 600
+    lens = np.full(rows, 2)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    cols = np.stack([np.arange(rows) + 10, np.arange(rows) + 20], 1).reshape(-1)
+    cols[1] = 10 + span  # row 0: columns 10 and 10 + span, in the first chunk
+    A = P.CSR(rows, n, rp, cols.astype(np.int32), np.random.default_rng(1).uniform(-1, 1, rp[-1]))
+    x = P.fill_uniform(n, 2)
+    with P.CvrHip(A, steps_per_chunk=8) as h:
+        assert h.info()["col_index_bits"] == bits
+        y = h.spmv_host(x)
+    with P.CvrHip(A, steps_per_chunk=8, col_index_bits=32, x_hot_kb=0) as h:
+        assert np.array_equal(y, h.spmv_host(x))
+    if bits == 32:
+        with pytest.raises(P.PavaError) as e:
+            P.CvrHip(A, steps_per_chunk=8, col_index_bits=16)
+        assert e.value.code == -6
+
+
 def test_recompute_needs_fp64_x(P):
     import torch
     A = P.laplacian2d(30, 30)
