@@ -229,7 +229,7 @@ int cvr_hip_spmv_mixed(cvr_hip_handle *h, const float *d_x32, const double *d_x6
  * FP32, device pointers; 1 <= k <= 32. Any precision of handle. */
 int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *stream);
 /* Batched multi-RHS SpMM: the chunks of many matrices (same device and
- * precision, 32-bit column layouts, nnz > 0) form one work queue consumed by a
+ * precision, 16- or 32-bit columns, nnz > 0) form one work queue consumed by a
  * persistent grid sized to the GPU's resident capacity; a fixup kernel closes
  * rows spanning chunks and rearms the queue, so the launch pair can be captured
  * once in a hipGraph and replayed. The handles must outlive the batch.
