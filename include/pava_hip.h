@@ -226,7 +226,10 @@ int cvr_hip_spmv_host(cvr_hip_handle *h, const double *x, double *y);
 int cvr_hip_spmv_mixed(cvr_hip_handle *h, const float *d_x32, const double *d_x64, float *d_y,
                        void *stream);
 /* Y = A*X with k right-hand sides, X (numCols x k) and Y (numRows x k) row-major
- * FP32, device pointers; 1 <= k <= 32. Any precision of handle. */
+ * FP32, device pointers; 1 <= k <= 32. Any precision of handle. The first call
+ * on a handle with nonzeros allocates its cross-chunk buffer (nchunks * 512 B):
+ * make one call before capturing the call in a graph. cvr_hip_spmv and
+ * cvr_hip_spmv_mixed enqueue kernels only and can be captured at any time. */
 int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *stream);
 /* Batched multi-RHS SpMM: the chunks of many matrices (same device and
  * precision, 16- or 32-bit columns, nnz > 0) form one work queue consumed by a

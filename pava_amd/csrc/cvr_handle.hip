@@ -583,7 +583,7 @@ static int spmm_desc(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, cvr
     if (k == cvr::spmm_kp(k) && (((uintptr_t)d_X | (uintptr_t)d_Y) & 15))
         return set_error(PAVA_HIP_EINVAL, "X and Y must be 16-byte aligned");
     const size_t need = (size_t)h->nchunks * 2 * 32 * 8 + 64;
-    if (!h->d_carry_mm) {
+    if (!h->d_carry_mm && h->nchunks > 0) {  // allocated by the first call (none for an empty matrix)
         int rc = dev_alloc(h, (void **)&h->d_carry_mm, need);
         if (rc) return rc;
     }
@@ -612,9 +612,6 @@ static int spmm_desc(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, cvr
     return 0;
 }
 
-static hipError_t zero_rows_k(float *d_Y, long long n, hipStream_t st) {
-    return n > 0 ? hipMemsetAsync(d_Y, 0, (size_t)n * 4, st) : hipSuccess;
-}
 
 extern "C" int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *stream) {
     if (!h) return set_error(PAVA_HIP_EINVAL, "null handle");
@@ -623,7 +620,7 @@ extern "C" int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d
     int rc = spmm_desc(h, k, d_X, d_Y, &a);
     if (rc) return rc;
     if (h->nchunks == 0) {
-        HIP_TRY(zero_rows_k(d_Y, (long long)h->M * k, (hipStream_t)stream));
+        HIP_TRY(cvr::launch_zero_f32(d_Y, (long long)h->M * k, (hipStream_t)stream));
         return 0;
     }
     HIP_TRY(cvr::launch_spmm(a, h->prec, h->max_slots, (hipStream_t)stream));

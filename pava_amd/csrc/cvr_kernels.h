@@ -153,6 +153,7 @@ struct SpmmDesc {
 };
 
 int spmm_kp(int k);
+hipError_t launch_zero_f32(float *y, long long n, hipStream_t st);
 size_t spmm_lds_bytes(int max_slots, int k);
 hipError_t launch_spmm(const SpmmDesc &a, int prec, int max_slots, hipStream_t st);
 hipError_t spmm_batch_grid(int prec, int k, size_t lds, int *grid);

@@ -20,6 +20,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cvr_kernels.h"
 
 namespace cvr {
@@ -401,6 +403,20 @@ static hipError_t launch_spmm_vt(const SpmmDesc &a, int kp, size_t lds, hipStrea
 }
 
 int spmm_kp(int k) { return k <= 8 ? 8 : k <= 16 ? 16 : 32; }
+
+// Y = 0 for a matrix without nonzeros (a kernel, not a memset node, so that
+// the call can be captured in a graph)
+__global__ void zero_f32_kernel(float *__restrict__ y, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[i] = 0.f;
+}
+
+hipError_t launch_zero_f32(float *y, long long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const long long blocks = std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(zero_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, y, n);
+    return hipGetLastError();
+}
 
 size_t spmm_lds_bytes(int max_slots, int k) { return (size_t)(max_slots > 0 ? max_slots : 1) * spmm_kp(k) * 8; }
 

@@ -487,6 +487,83 @@ def test_column_width_boundary(P, span, bits):
         assert e.value.code == -6
 
 
+def test_spmv_captured_in_a_graph(P):
+    """cvr_hip_spmv and cvr_hip_spmv_mixed enqueue kernels only (no memset, copy,
+    allocation or synchronisation), so a caller can capture them in a graph.
+    Replays with new x values in the same buffers, with eager work and
+    synchronisations between them, equal eager calls bit for bit. Covers hot-x
+    staging with the LDS table, 16-bit columns, FP32 with the recompute pass
+    (long rows included) and SpMM of a matrix without nonzeros."""
+    import torch
+    rng = np.random.default_rng(5)
+    lens = np.concatenate([rng.integers(1, 30, 4000), [300, 900]])
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    cols = np.concatenate([np.sort(rng.choice(6000, l, replace=False)) for l in lens]).astype(np.int32)
+    cancel = P.CSR(lens.size, 6000, rp, cols, rng.choice([-1.0, 1.0], rp[-1]) * (1 + 0.01 * rng.random(rp[-1])))
+    cases = [("rmat hot-x + LDS", P.rmat(16, 16, seed=3), dict(x_hot_kb=256, col_index_bits=32)),
+             ("stencil 16-bit", P.stencil27(32, 0.01, 1), dict(col_index_bits=16)),
+             ("f32 recompute", cancel, dict(precision=1, recompute=True, kappa_threshold=4.0))]
+    for name, A, kw in cases:
+        prec = kw.get("precision", 0)
+        with P.CvrHip(A, steps_per_chunk=64, **kw) as h:
+            if name.startswith("rmat"):
+                assert h.info()["n_lds_cols"] > 0
+            dx64 = torch.empty(A.numCols, dtype=torch.float64, device="cuda")
+            dx32 = torch.empty(A.numCols, dtype=torch.float32, device="cuda")
+            dy = torch.empty(A.numRows, dtype=torch.float64 if prec == 0 else torch.float32, device="cuda")
+
+            def call(stream):
+                if prec == 0:
+                    h.spmv(dx64.data_ptr(), dy.data_ptr(), stream)
+                else:
+                    h.spmv_mixed(dx32.data_ptr(), dy.data_ptr(), dx64.data_ptr(), stream)
+
+            def set_x(seed):
+                dx64.copy_(torch.from_numpy(P.fill_uniform(A.numCols, seed)))
+                dx32.copy_(dx64)
+                torch.cuda.synchronize()
+
+            set_x(0)
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                call(s.cuda_stream)
+            s.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                call(torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+            for rep in range(4):
+                set_x(10 + rep)
+                call(0)
+                torch.cuda.synchronize()
+                want = dy.clone()
+                if prec:
+                    assert h.last_recompute_count() > 0
+                dy.fill_(float("nan"))
+                torch.cuda.synchronize()
+                g.replay()
+                torch.cuda.synchronize()
+                _say(f"graph {name}: replay {rep}")
+                assert torch.equal(dy, want), (name, rep)
+            del g
+    # SpMM of a matrix without nonzeros: a fill kernel, capturable as well
+    E = P.CSR(5, 5, np.zeros(6, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    with P.CvrHip(E, precision=1) as h:
+        X = torch.ones((5, 8), dtype=torch.float32, device="cuda")
+        Y = torch.full((5, 8), float("nan"), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            h.spmm(8, X.data_ptr(), Y.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        for rep in range(3):
+            Y.fill_(float("nan"))
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(Y, torch.zeros_like(Y))
+        del g
+
+
 def test_recompute_needs_fp64_x(P):
     import torch
     A = P.laplacian2d(30, 30)
