@@ -272,3 +272,37 @@ def test_cvr_at_the_size_limit(P):
         got = dy[r0:r0 + r.shape[0]]
         bad += int((got != want.to(torch.float64)).sum())
     assert bad == 0
+
+
+@pytest.mark.parametrize("fmt", ["csr5", "coo", "csr"])
+def test_other_formats_at_headline_sizes(P, O, rmat22, stencil128, fmt):
+    """CSR5-HIP (sigma 16), COO-HIP and CSR-vector (8 lanes per row) on R-MAT
+    scale 22 and the 27-point stencil 128^3: every row within the rounding
+    bound of a reordered FP64 sum of its n terms, 2 (n + 64) u sum |a x|, of
+    the serial oracle; the product of a power of two scales y exactly."""
+    import torch
+    for name, A in (("rmat22", rmat22), ("stencil128", stencil128)):
+        x = P.fill_uniform(A.numCols, 0)
+        dx = torch.from_numpy(x).cuda()
+        dy = torch.full((A.numRows,), float("nan"), dtype=torch.float64, device="cuda")
+        if fmt == "csr5":
+            h = P.Csr5Hip(A, sigma=16)
+        elif fmt == "coo":
+            h = P.CooHip(A)
+        else:
+            h = P.CsrHip(A, lanes_per_row=8)
+        with h:
+            torch.cuda.synchronize()
+            h.spmv(dx.data_ptr(), dy.data_ptr(), 0)
+            torch.cuda.synchronize()
+            y = dy.cpu().numpy()
+            dx.mul_(-4.0)
+            h.spmv(dx.data_ptr(), dy.data_ptr(), 0)
+            torch.cuda.synchronize()
+            assert np.array_equal(dy.cpu().numpy(), -4.0 * y), (fmt, name)
+        yref = O.spmv_fma(A, x)
+        lens = np.diff(A.rowOffsets)
+        bound = 2 * (lens + 64) * U * O.row_abs(A, x)
+        _say(f"{fmt} {name}: bit-identical rows {np.mean(y == yref):.4f}")
+        assert not np.isnan(y).any(), (fmt, name)
+        assert np.all(np.abs(y - yref) <= bound), (fmt, name)
