@@ -52,7 +52,7 @@ def test_mlp_training_reduces_loss():
 def test_autotune_best_config_reproducible():
     """Exhaustive autotuning of seeded matrices of the selector family, run
     twice: where the first run's best config beats the runner-up by more than
-    15% the second run picks it again, and everywhere the first choice is
+    30% the second run picks it again, and everywhere the first choice is
     within 10% of the second run's best. Configs that cannot apply (16-bit
     columns on chunks spanning more than 65,536 columns) are inf in both."""
     import pava_amd as P
@@ -70,7 +70,7 @@ def test_autotune_best_config_reproducible():
         _say(f"matrix {i}: rows {A.numRows} nnz {A.nnz}: best {S.CONFIGS[b1].name} then {S.CONFIGS[b2].name}, "
              f"margin {runner_up / t1[b1]:.2f}")
         assert t2[b1] <= 1.10 * t2[b2]
-        if runner_up > 1.15 * t1[b1]:
+        if runner_up > 1.30 * t1[b1]:
             clear += 1
             assert b1 == b2
     assert clear >= 1
