@@ -92,6 +92,23 @@ def test_conduct_coo_hip_drop_in(P, O, capfd):
     assert np.array_equal(y, O.spmv_fma(A, x))  # integer-valued products: exact
 
 
+def test_bench_file_cvr_size_gate(P, tmp_path, capfd):
+    """As in pava's CVR block (main.cpp:558), the CVR_HIP block runs only for
+    more than 4,096 rows and columns: a 3,000-row matrix gets the skipped-block
+    line (-1 -1) and the other formats still run and pass the 1% check."""
+    A = P.laplacian2d(60, 50)
+    path = tmp_path / "lap3000.mtx"
+    P.write_mtx(path, A)
+    errors = P.lib().pava_bench_file(str(path).encode(), 2)
+    out = capfd.readouterr().out
+    assert errors == 0
+    line = [l for l in out.splitlines() if " CVR_HIP " in l and "Performance" in l]
+    assert len(line) == 1 and line[0].split()[-2:] == ["-1.000000", "-1.000000"], line
+    for fmt in ("CSR5_HIP", "CSR_HIP", "COO_HIP"):
+        got = [l for l in out.splitlines() if f" {fmt} " in l and "Performance" in l]
+        assert len(got) == 1 and not got[0].split()[-1].startswith("-1"), (fmt, got)
+
+
 def test_bench_file_runs_every_gpu_format(P, tmp_path, capfd):
     """pava_bench_file (benchFile with the GPU registry): every block reports a
     line and passes pava's 1% check."""
