@@ -27,6 +27,15 @@
  *                        last step, record counts) and by its SpMV equalling
  *                        orc_csr_spmv_fma bit for bit on rows that are not
  *                        split and within the rounding bound on split rows.
+ *                        The FP32/BF16 lane accumulation (Dot2: TwoProduct by
+ *                        fmaf, TwoSum) is pinned against exact rational row
+ *                        sums on strongly cancelling rows, within Dot2's
+ *                        published bound, which an uncompensated sum and Dot2
+ *                        without TwoProduct both break (tests/test_compensated.py).
+ *   orc_csr5_build/spmv  CPU emulation of the GPU CSR5-64 tiles (csr5/); pinned
+ *                        by tile invariants and by its SpMV against
+ *                        orc_csr_spmv_fma within the rounding bound
+ *                        (tests/test_csr5_layout.py).
  *
  * Build with -ffp-contract=off: every fused operation below is an explicit fma.
  */
