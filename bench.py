@@ -296,8 +296,8 @@ def main():
                     help="also run 3 plain device copies of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures the single-GPU SpMV path only; this build has no multi-GPU path "
-              "(see DESIGN.md for how the path would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures the single-GPU SpMV path only; the library has no multi-GPU path. "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch

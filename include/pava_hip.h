@@ -54,7 +54,7 @@ extern "C" {
 
 /* Reader return codes follow readCOOMatrix (utils/basicMatrix.cpp:84-132):
  * -1 bad banner, -2 not a matrix, -3 not sparse (coordinate), -4 bad size line,
- * -5 allocation failure; this build adds -6 (entry out of range / truncated
+ * -5 allocation failure; this library adds -6 (entry out of range / truncated
  * body) and -7 (file cannot be opened; the reference calls exit(1) there). */
 #define PAVA_MM_EBANNER (-1)
 #define PAVA_MM_ENOTMATRIX (-2)

@@ -16,7 +16,7 @@
 // rows wholly inside the chunk are written, the partial of a row shared with
 // a neighbouring chunk goes to a per-chunk carry, which the fixup kernel adds
 // in chunk order (pava uses omp atomic +=, cvr/cvr.h:1413-1421, 1717, 1855;
-// this build is deterministic: the same inputs give bit-identical y).
+// here the SpMV is deterministic: the same inputs give bit-identical y).
 // Finally the wave zeroes the empty rows of its row range (pava leaves them
 // unwritten and relies on the caller's zeroing, main.cpp:561).
 
