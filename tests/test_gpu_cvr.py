@@ -564,6 +564,55 @@ def test_spmv_captured_in_a_graph(P):
         del g
 
 
+def test_seeded_sweep_of_shapes_and_options(P, O):
+    """40 seeded random matrices (empty rows, single-element rows, rows longer
+    than a chunk, ragged lengths, banded or scattered columns) under random
+    chunk sizes, column widths and hot-x settings: y equals the emulation of the
+    same chunk size bit for bit, and the FP32 layout its FP32 emulation."""
+    rng = np.random.default_rng(2024)
+    for case in range(40):
+        M = int(rng.integers(1, 20000))
+        N = int(rng.choice([M, int(rng.integers(1, 200000))]))
+        kind = case % 4
+        if kind == 0:
+            lens = rng.integers(0, 40, M)
+        elif kind == 1:
+            lens = rng.choice([0, 1, 2, 3, 500], M, p=[0.2, 0.3, 0.2, 0.29, 0.01])
+        elif kind == 2:
+            lens = np.minimum(rng.zipf(1.8, M), 3000)
+        else:
+            lens = np.full(M, int(rng.integers(1, 30)))
+        lens = np.minimum(lens, N)
+        rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        banded = rng.random() < 0.5
+        cols = []
+        for r, l in enumerate(lens):
+            if l == 0:
+                continue
+            if banded:
+                lo = max(0, min(N - l, r * N // max(M, 1) - 200))
+                c = lo + np.sort(rng.choice(min(N - lo, max(l, 400)), l, replace=False))
+            else:
+                c = np.sort(rng.choice(N, l, replace=False))
+            cols.append(c)
+        ci = (np.concatenate(cols) if cols else np.zeros(0)).astype(np.int32)
+        A = P.CSR(M, N, rp, ci, rng.uniform(-1, 1, int(rp[-1])))
+        S = int(rng.integers(1, 33)) * 8
+        bits = int(rng.choice([0, 32]))
+        hot = int(rng.choice([-1, 0, 64]))
+        prec = int(rng.choice([0, 0, 1]))
+        _say(f"sweep {case}: M={M} N={N} nnz={A.nnz} S={S} bits={bits} hot={hot} prec={prec}")
+        x = P.fill_uniform(N, case)
+        with P.CvrHip(A, steps_per_chunk=S, col_index_bits=bits, x_hot_kb=hot, precision=prec) as h:
+            y = h.spmv_host(x)
+        emu = O.cvr64_build(A, S, prec)
+        if prec == 0:
+            want = O.cvr64_spmv(A, emu, x)
+        else:
+            want = O.cvr64_spmv(A, emu, x.astype(np.float32)).astype(np.float64)
+        assert np.array_equal(y, want), case
+
+
 def test_recompute_needs_fp64_x(P):
     import torch
     A = P.laplacian2d(30, 30)
