@@ -180,3 +180,61 @@ def test_spmm_with_hot_columns_bit_identical(P, O, k):
     assert np.array_equal(y_hot, y_plain)
     assert np.array_equal(yb_hot, yb_plain)
     assert np.array_equal(y_hot, yb_hot)
+
+
+def test_seeded_sweep_spmm_and_batch(P, O):
+    """16 seeded random matrices (ragged, empty and long rows; banded or
+    scattered columns) with random chunk sizes and column widths, one random
+    k: each single-matrix SpMM output is the FP32 rounding of the FP64
+    CVR-order sum, and one batch of all 16 gives the same Y bit for bit."""
+    import torch
+    rng = np.random.default_rng(77)
+    k = int(rng.integers(1, 33))
+    mats, hs, Xs, Ys = [], [], [], []
+    for i in range(16):
+        M = int(rng.integers(1, 8000))
+        N = int(rng.choice([M, int(rng.integers(1, 150000))]))
+        lens = np.minimum(rng.choice([0, 1, 5, 20, 300], M, p=[0.1, 0.2, 0.4, 0.29, 0.01]), N)
+        rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        banded = i % 2 == 0
+        cols = []
+        for r, l in enumerate(lens):
+            if l == 0:
+                continue
+            if banded:
+                lo = max(0, min(N - l, r * N // M - 100))
+                cols.append(lo + np.sort(rng.choice(min(N - lo, max(l, 300)), l, replace=False)))
+            else:
+                cols.append(np.sort(rng.choice(N, l, replace=False)))
+        ci = (np.concatenate(cols) if cols else np.zeros(0)).astype(np.int32)
+        if ci.size == 0:
+            continue
+        A = _f32_matrix(P.CSR(M, N, rp, ci, rng.uniform(-1, 1, int(rp[-1]))))
+        S = int(rng.integers(1, 17)) * 8
+        bits = int(rng.choice([0, 32]))
+        h = P.CvrHip(A, steps_per_chunk=S, precision=P.PREC_F32, col_index_bits=bits)
+        X = rng.uniform(-1, 1, (N, k)).astype(np.float32)
+        dX = torch.from_numpy(X).cuda()
+        dY = torch.full((M, k), float("nan"), dtype=torch.float32, device="cuda")
+        h.spmm(k, dX.data_ptr(), dY.data_ptr())
+        torch.cuda.synchronize()
+        Y = dY.cpu().numpy()
+        lay = O.cvr64_build(A, S, 0)
+        want = np.stack([O.cvr64_spmv(A, lay, X[:, j].astype(np.float64)) for j in range(k)], 1).astype(np.float32)
+        _say(f"spmm sweep {i}: M={M} N={N} nnz={A.nnz} S={S} bits={h.info()['col_index_bits']} k={k}")
+        assert np.array_equal(Y, want), i
+        mats.append(A)
+        hs.append(h)
+        Xs.append(dX)
+        Ys.append((dY, Y))
+    b = P.Batch(hs, k)
+    outs = [torch.full_like(dY, float("nan")) for dY, _ in Ys]
+    b.set_operands([x.data_ptr() for x in Xs], [o.data_ptr() for o in outs])
+    b.spmm(0)
+    torch.cuda.synchronize()
+    check_queue_counts(b)
+    for o, (_, Y) in zip(outs, Ys):
+        assert np.array_equal(o.cpu().numpy(), Y)
+    b.close()
+    for h in hs:
+        h.close()
