@@ -50,3 +50,24 @@ def split_rows(A, lay, S):
             elif code == -2:
                 out.add(bisect.bisect_right(rp, e - 1) - 1)
     return out
+
+
+def random_csr(P, rng, max_rows=20000):
+    """A seeded random CSR: ragged / empty / single-element / long rows,
+    banded or scattered sorted columns, values U[-1, 1]."""
+    M = int(rng.integers(1, max_rows))
+    N = int(rng.choice([M, int(rng.integers(1, 200000))]))
+    lens = np.minimum(rng.choice([0, 1, 3, 17, 60, 900], M, p=[0.1, 0.2, 0.3, 0.3, 0.095, 0.005]), N)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    banded = rng.random() < 0.5
+    cols = []
+    for r, l in enumerate(lens):
+        if l == 0:
+            continue
+        if banded:
+            lo = max(0, min(N - l, r * N // M - 150))
+            cols.append(lo + np.sort(rng.choice(min(N - lo, max(l, 400)), l, replace=False)))
+        else:
+            cols.append(np.sort(rng.choice(N, l, replace=False)))
+    ci = (np.concatenate(cols) if cols else np.zeros(0)).astype(np.int32)
+    return P.CSR(M, N, rp, ci, rng.uniform(-1, 1, int(rp[-1])))

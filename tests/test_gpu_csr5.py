@@ -85,3 +85,33 @@ def test_conduct_csr5_hip_drop_in(P, O, capfd):
     out = capfd.readouterr().out
     assert " Performance McRae lap90x70 CSR5_HIP " in out
     assert np.array_equal(y, O.spmv_fma(A, x))  # integer-valued products: exact
+
+
+def test_seeded_sweep_csr5_coo_csr(P, O):
+    """24 seeded random shapes: CSR5-HIP (random sigma) equals its emulation bit
+    for bit; COO-HIP and CSR-vector (random lanes per row) stay within the
+    rounding bound 2 (n + 64) u sum |a x| of the serial oracle."""
+    import torch
+    from matrices import random_csr
+    rng = np.random.default_rng(91)
+    for case in range(24):
+        A = random_csr(P, rng)
+        x = P.fill_uniform(A.numCols, case)
+        sigma = int(rng.choice([4, 8, 16, 32]))
+        L = int(rng.choice([1, 2, 4, 8, 16, 32, 64]))
+        _say(f"sweep {case}: M={A.numRows} N={A.numCols} nnz={A.nnz} sigma={sigma} L={L}")
+        with P.Csr5Hip(A, sigma=sigma) as h:
+            y5 = h.spmv_host(x)
+        assert np.array_equal(y5, O.csr5_spmv(A, O.csr5_build(A, sigma), x)), case
+        yref = O.spmv_fma(A, x)
+        bound = 2 * (np.diff(A.rowOffsets) + 64) * 2.0 ** -53 * O.row_abs(A, x)
+        dx = torch.from_numpy(x).cuda()
+        for h in (P.CooHip(A), P.CsrHip(A, lanes_per_row=L)):
+            dy = torch.full((A.numRows,), float("nan"), dtype=torch.float64, device="cuda")
+            with h:
+                torch.cuda.synchronize()
+                h.spmv(dx.data_ptr(), dy.data_ptr(), 0)
+                torch.cuda.synchronize()
+            y = dy.cpu().numpy()
+            assert not np.isnan(y).any(), (case, type(h).__name__)
+            assert np.all(np.abs(y - yref) <= bound), (case, type(h).__name__)
