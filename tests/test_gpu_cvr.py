@@ -613,6 +613,37 @@ def test_seeded_sweep_of_shapes_and_options(P, O):
         assert np.array_equal(y, want), case
 
 
+def test_unsorted_and_repeated_columns(P, O):
+    """A CSR whose rows list columns out of order and repeat some: the layout
+    keeps each row's element order, so y equals the emulation bit for bit
+    (both column widths) and the serial oracle within the bound."""
+    from matrices import random_csr
+    rng = np.random.default_rng(31)
+    A = random_csr(P, rng, max_rows=6000)
+    cols = A.cols.copy()
+    for r in range(A.numRows):
+        b, e = A.rowOffsets[r], A.rowOffsets[r + 1]
+        if e - b > 1:
+            seg = cols[b:e]
+            rng.shuffle(seg)
+            if e - b > 3:
+                seg[-1] = seg[0]  # a repeated column
+    B = P.CSR(A.numRows, A.numCols, A.rowOffsets, cols, A.vals)
+    x = P.fill_uniform(B.numCols, 4)
+    want = O.cvr64_spmv(B, O.cvr64_build(B, 16, 0), x)
+    for bits in (16, 32):
+        try:
+            h = P.CvrHip(B, steps_per_chunk=16, col_index_bits=bits, x_hot_kb=0)
+        except P.PavaError as e:
+            assert bits == 16 and e.code == -6
+            continue
+        with h:
+            assert np.array_equal(h.spmv_host(x), want), bits
+    yref = O.spmv_fma(B, x)
+    lens = np.diff(B.rowOffsets)
+    assert np.all(np.abs(want - yref) <= 2 * (lens + 64) * 2.0 ** -53 * O.row_abs(B, x))
+
+
 def test_recompute_needs_fp64_x(P):
     import torch
     A = P.laplacian2d(30, 30)
