@@ -268,7 +268,7 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     h->max_slots = status[0];
     if (status[1]) return set_error(PAVA_HIP_EDEVICE, "CVR conversion overflowed its per-chunk tables (flags %d)", status[1]);
     if (status[2])
-        return set_error(PAVA_HIP_ERANGE, "a chunk spans more than 65536 columns: 16-bit column layout not possible");
+        return set_error(PAVA_HIP_ERANGE, "a chunk spans 65536 columns or more: 16-bit column layout not possible");
     h->n_fix = status[3];
     h->has_empty = status[4] ? 1 : 0;
 
