@@ -644,6 +644,44 @@ def test_unsorted_and_repeated_columns(P, O):
     assert np.all(np.abs(want - yref) <= 2 * (lens + 64) * 2.0 ** -53 * O.row_abs(B, x))
 
 
+def test_extreme_shapes(P, O):
+    """One row of 3,000,000 elements (hundreds of chunks, thousands of
+    recompute pieces), a single column of 100,000 rows, a million rows with
+    only the last one non-empty, and 1 x 1: FP64 equals the emulation bit for
+    bit (both column widths). With FP32 storage and a cancellation threshold
+    of 1, every row stays within FP32 input rounding of the oracle, and the long
+    row (random signs, so certainly flagged) is recomputed in FP64: the serial
+    sum rounded to FP32, within FP64 rounding of it."""
+    rng = np.random.default_rng(8)
+    n = 3_000_000
+    shapes = {
+        "one_long_row": P.CSR(1, n, np.array([0, n], np.int32), np.arange(n, dtype=np.int32),
+                              rng.uniform(-1, 1, n)),
+        "one_column": P.CSR(100000, 1, np.arange(100001, dtype=np.int32), np.zeros(100000, np.int32),
+                            rng.uniform(-1, 1, 100000)),
+        "last_row_only": P.CSR(1_000_000, 5000, np.concatenate([np.zeros(1_000_000, np.int32), [100]]).astype(np.int32),
+                               np.sort(rng.choice(5000, 100, replace=False)).astype(np.int32), rng.uniform(-1, 1, 100)),
+        "one_by_one": P.CSR(1, 1, np.array([0, 1], np.int32), np.array([0], np.int32), np.array([-2.5])),
+    }
+    for name, A in shapes.items():
+        x = P.fill_uniform(A.numCols, 3)
+        for bits in (0, 32):
+            with P.CvrHip(A, col_index_bits=bits) as h:
+                S = h.info()["steps_per_chunk"]
+                y = h.spmv_host(x)
+            _say(f"extreme {name}: S={S} bits={bits}")
+            assert np.array_equal(y, O.cvr64_spmv(A, O.cvr64_build(A, S, 0), x)), (name, bits)
+        yref = O.spmv_fma(A, x)
+        sabs = O.row_abs(A, x)
+        with P.CvrHip(A, precision=1, recompute=True, kappa_threshold=1.0) as h:
+            y32 = h.spmv_host(x)
+            n_rc = h.last_recompute_count()
+        assert np.all(np.abs(y32 - yref) <= 4 * 2.0 ** -24 * sabs), name
+        if name == "one_long_row":
+            assert n_rc == 1
+            assert abs(y32[0] - yref[0]) <= 2.0 ** -24 * abs(yref[0]) + 2 * (n + 64) * 2.0 ** -53 * sabs[0]
+
+
 def test_recompute_needs_fp64_x(P):
     import torch
     A = P.laplacian2d(30, 30)
