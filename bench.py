@@ -280,8 +280,9 @@ def main():
     ap.add_argument("--steps-per-chunk", type=int, default=0, help="CVR chunk steps (0 = automatic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--variant", choices=["none", "precision", "batch"], default="none",
-                    help="precision: FP64/FP32/BF16 storage on the workload's matrix; batch: 512-matrix SpMM")
+    ap.add_argument("--variant", choices=["none", "precision", "batch", "create"], default="none",
+                    help="precision: FP64/FP32/BF16 storage on the workload's matrix; batch: 512-matrix SpMM; "
+                         "create: --steps handle creations from the host CSR")
     ap.add_argument("--k", type=int, default=8, help="batch variant: right-hand sides")
     ap.add_argument("--nmat", type=int, default=512, help="batch variant: matrices")
     ap.add_argument("--x-hot-kb", type=int, default=-1,
@@ -318,6 +319,24 @@ def main():
         return
 
     w = WORKLOADS[args.workload]
+    if args.variant == "create":
+        # handle creation from a host CSR, repeated: upload, conversion and
+        # hot-column staging (the first creation of a process also loads code)
+        A = make_matrix(P, w)
+        reps = []
+        for _ in range(max(args.steps, 1)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            h = P.CvrHip(A, steps_per_chunk=args.steps_per_chunk, col_index_bits=args.col_index_bits,
+                         x_hot_kb=args.x_hot_kb)
+            torch.cuda.synchronize()
+            wall = (time.perf_counter() - t0) * 1e3
+            inf = h.info()
+            reps.append({"wall_ms": round(wall, 3), "convert_ms": round(inf["convert_ms"], 3),
+                         "hot_setup_ms": round(inf["hot_setup_ms"], 3)})
+            h.close()
+        print(json.dumps({"variant": "create", "workload": args.workload, "nnz": A.nnz, "reps": reps}), flush=True)
+        return
     if args.variant == "precision":
         A = make_matrix(P, w)
         x = P.fill_uniform(A.numCols, 0)

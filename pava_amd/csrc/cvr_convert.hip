@@ -798,7 +798,11 @@ hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, 
 // columns or more, so that 16-bit chunk-relative offsets cannot encode them.
 __global__ __launch_bounds__(256) void chunk_span_kernel(const int *__restrict__ ci, long long nnz, long long C,
                                                          int *__restrict__ flag) {
-    __shared__ int smn[4], smx[4];
+    __shared__ int smn[4], smx[4], done;
+    // a chunk already found too wide settles the answer: later workgroups stop
+    if (threadIdx.x == 0) done = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (done) return;
     const long long b = C * blockIdx.x, e = b + C < nnz ? b + C : nnz;
     int mn = 0x7fffffff, mx = -1;
     for (long long k = b + threadIdx.x; k < e; k += blockDim.x) {
