@@ -39,7 +39,7 @@ WORKLOADS = {
 
 
 def algorithmic_bytes(m: int, n: int, nnz: int, s_val=8, s_idx=4, s_x=8, s_y=8) -> int:
-    return nnz * (s_val + s_idx) + (m + 1) * 4 + n * s_x + m * s_y
+    return int(round(nnz * (s_val + s_idx) + (m + 1) * 4 + n * s_x + m * s_y))
 
 
 def make_matrix(P, w):
@@ -377,11 +377,13 @@ def main():
     tst = h.time_spmv_stats(dx.data_ptr(), dy.data_ptr(), 5, max(100, min(args.steps, 500)))
     ms_main = tst["main_mean"]
 
-    B = algorithmic_bytes(A.numRows, A.numCols, A.nnz)
-    value = B / (ms_step * 1e-3) / 1e9
     col16 = info["col_index_bits"] == 16
-    kernel = "spmv_kernel<double,double,double,false,%s,false,%s>" % ("true" if col16 else "false",
-                                                                      "true" if info["n_lds_cols"] else "false")
+    B = algorithmic_bytes(A.numRows, A.numCols, A.nnz, s_idx=2.125 if col16 else 4)
+    value = B / (ms_step * 1e-3) / 1e9
+    par = info["parity_tol"] > 0
+    kernel = "spmv_kernel<double,double,double,false,%s,%s,%s>" % ("true" if col16 else "false",
+                                                                   "true" if par else "false",
+                                                                   "true" if info["n_lds_cols"] else "false")
     achieved = B / (ms_main * 1e-3) / 1e9
 
     result = {
@@ -421,6 +423,18 @@ def main():
         result["roofline"]["traffic"] = traffic
         result["roofline"]["traffic_GB_s_at_this_kernel_time"] = round(traffic / (ms_main * 1e-3) / 1e9, 1)
 
+    if par:
+        # what the FP64 parity pass costs: the same matrix with parity_tol = 0
+        # (plain CVR order), timed the same way
+        result["parity"] = {"parity_tol": info["parity_tol"], "rows_recomputed": h.last_recompute_count()}
+        with P.CvrHip(A, steps_per_chunk=args.steps_per_chunk, precision=w["prec"], x_hot_kb=args.x_hot_kb,
+                      x_lds_kb=args.x_lds_kb, col_index_bits=args.col_index_bits, parity_tol=0.0) as h0:
+            dy0 = torch.empty_like(dy)
+            t0s = h0.time_spmv_stats(dx.data_ptr(), dy0.data_ptr(), 5, 200)
+        result["parity"].update({"ms_per_spmv": round(tst["ms_total"], 5),
+                                 "ms_per_spmv_parity_off": round(t0s["ms_total"], 5),
+                                 "cost": round(tst["ms_total"] / t0s["ms_total"] - 1.0, 4)})
+        del dy0
     if not args.no_check:
         # FP64 reference: the driver's verification SpMV (pava's referenceSpMV,
         # serial FMA per row)

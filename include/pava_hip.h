@@ -187,6 +187,14 @@ typedef struct cvr_hip_opts {
                                 used column (32-bit layouts only) */
     int x_lds_kb;            /* of the hot-x copy, the first x_lds_kb KiB are held in LDS by each
                                 workgroup of the SpMV; -1 = as much as fits (default), 0 = none */
+    double parity_tol;       /* FP64 handles: a row that the CVR order splits is recomputed as one
+                                serial FMA chain (the order of the reference's referenceSpMV,
+                                utils/basicMatrix.cpp:538-556) when its running error bound cannot
+                                show |y_cvr - y_serial| <= parity_tol * |y_serial|. Rows of at most
+                                4,096 elements use the rigorous bound; longer rows also accept a
+                                probabilistic bound (lambda = 10). Unsplit rows equal the serial sum
+                                bit for bit. Default 1e-12; 0 = off (pure CVR order). Keeps the FP64
+                                CSR resident (12 B per nonzero + 4 B per row) */
 } cvr_hip_opts;
 
 typedef struct cvr_hip_info {
@@ -202,6 +210,7 @@ typedef struct cvr_hip_info {
     int n_hot_cols;          /* columns staged in the hot-x copy */
     int n_lds_cols;          /* of those, columns the SpMV reads from LDS */
     double hot_setup_ms;     /* wall time of the hot-column staging (counts, selection, remap) */
+    double parity_tol;       /* in effect (0: off) */
 } cvr_hip_info;
 
 typedef struct cvr_hip_handle cvr_hip_handle;
@@ -317,7 +326,8 @@ int pava_mlp_train(pava_mlp *m, int n, const float *X, const float *T, const flo
                    float lr, unsigned long long seed, float *loss_hist);
 int pava_mlp_destroy(pava_mlp *m);
 
-/* Number of rows recomputed in FP64 by the last mixed SpMV (synchronises). */
+/* Number of rows recomputed in FP64 by the last mixed SpMV, or in serial order
+ * by the last FP64 SpMV with parity_tol > 0 (synchronises). */
 int cvr_hip_last_recompute_count(cvr_hip_handle *h, int *count);
 
 /* Timing of `iters` back-to-back SpMVs after `warmup`, with hipEvents on the

@@ -37,6 +37,8 @@ def lib() -> C.CDLL:
         L.orc_cvr64_build.argtypes = [i, i, i, P, P, P, i, i] + [P] * 9
         L.orc_cvr64_spmv.argtypes = [i, i, i, i] + [P] * 10
         L.orc_cvr64_spmv.restype = None
+        L.orc_cvr64_spmv_par.argtypes = [i, i, i, i] + [P] * 12 + [C.c_double, P, P]
+        L.orc_cvr64_spmv_par.restype = None
         L.orc_csr5_ntiles.argtypes = [i, i]
         L.orc_csr5_build.argtypes = [i, i, P, P, P, i] + [P] * 8
         L.orc_csr5_spmv.argtypes = [i, i, i] + [P] * 10
@@ -118,9 +120,17 @@ def cvr64_build(A, S: int = 256, prec: int = 0) -> dict:
     return out
 
 
-def cvr64_spmv(A, lay: dict, x):
-    """y over an emulated layout, in the GPU kernel's summation order."""
-    rp = np.ascontiguousarray(A.rowOffsets, np.int32)
+PARITY_TOL = 1e-12  # the library's default parity_tol for FP64 handles
+
+
+def cvr64_spmv(A, lay: dict, x, parity_tol: float = PARITY_TOL, details: bool = False):
+    """y over an emulated layout, in the GPU kernel's summation order. For an
+    FP64 layout, split rows whose running error bound cannot show agreement
+    with the serial sum within parity_tol are recomputed serially, as the GPU
+    does with its default parity_tol (0: the plain CVR order). details=True
+    returns (y, bound, n_recomputed), bound[r] = u * A of split row r (0 on
+    unsplit rows)."""
+    rp, ci, va = _csr(A)
     prec = lay["prec"]
     if prec == 0:
         xx = np.ascontiguousarray(x, np.float64)
@@ -128,9 +138,15 @@ def cvr64_spmv(A, lay: dict, x):
     else:
         xx = np.ascontiguousarray(x, np.float32)
         y = np.empty(A.numRows, np.float32)
-    lib().orc_cvr64_spmv(A.numRows, A.nnz, lay["S"], prec, *[_p(lay[k]) for k in
-                                                              ["vals", "cols", "chunk_rows", "seg", "split_n",
-                                                               "split_tab", "fix"]], _p(rp), _p(xx), _p(y))
+        parity_tol = 0.0
+    bound = np.zeros(A.numRows) if details else None
+    nrc = C.c_int(0)
+    lib().orc_cvr64_spmv_par(A.numRows, A.nnz, lay["S"], prec, *[_p(lay[k]) for k in
+                                                                  ["vals", "cols", "chunk_rows", "seg", "split_n",
+                                                                   "split_tab", "fix"]], _p(rp), _p(ci), _p(va),
+                             _p(xx), _p(y), float(parity_tol), _p(bound) if details else None, C.byref(nrc))
+    if details:
+        return y, bound, nrc.value
     return y
 
 

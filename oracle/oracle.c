@@ -337,14 +337,68 @@ int orc_cvr64_build(int m, int n, int nnz, const int *rp, const int *ci, const d
     return 0;
 }
 
+/* FP64 serial parity (the GPU's rule, restated): bound terms of a split row,
+ * packing of (q, n) into one double, and the decision. */
+static const double PAR_LAMBDA = 10.0;
+enum { PAR_RIGOROUS_LEN = 4096 };
+static const int64_t PAR_NMASK = 0xFFFFF;
+
+static double par_pack(double q, int n) {
+    int64_t b;
+    memcpy(&b, &q, 8);
+    b = (b & ~PAR_NMASK) | (int64_t)n;
+    double r;
+    memcpy(&r, &b, 8);
+    return r;
+}
+static void par_unpack(double p, double *q, int *n) {
+    int64_t b;
+    memcpy(&b, &p, 8);
+    *n = (int)(b & PAR_NMASK);
+    b &= ~PAR_NMASK;
+    memcpy(q, &b, 8);
+}
+/* rigorous running-error bound (rows <= PAR_RIGOROUS_LEN) or, for longer
+ * rows, also the probabilistic one; 0 -> recompute in serial order */
+static int par_ok(double s, double A, double Q, int N, double tol) {
+    const double u = 0x1p-53;
+    const double lim = tol * fabs(s);
+    const double Ar = A + (double)N * 0x1p-1021;
+    if (u * Ar * (1.0 + 2e-6) <= lim) return 1;
+    if (N > PAR_RIGOROUS_LEN && Q > 0x1p-900) {
+        const double p = PAR_LAMBDA * u;
+        if (p * p * Q * (1.0 + 4e-6) <= lim * lim) return 1;
+    }
+    return 0;
+}
+static double serial_row(const int *rp, const int *ci, const double *va, const double *x, int r) {
+    double s = 0.0;
+    for (int j = rp[r]; j < rp[r + 1]; ++j) s = fma(va[j], x[ci[j]], s);
+    return s;
+}
+
 void orc_cvr64_spmv(int m, int nnz, int S, int prec, const void *vals, const int *cols, const int *chunk_rows,
                     const int *seg, const int *split_n, const int *split_tab, const int *fix, const int *rp,
                     const void *x, void *y) {
+    orc_cvr64_spmv_par(m, nnz, S, prec, vals, cols, chunk_rows, seg, split_n, split_tab, fix, rp, NULL, NULL, x, y,
+                       0.0, NULL, NULL);
+}
+
+void orc_cvr64_spmv_par(int m, int nnz, int S, int prec, const void *vals, const int *cols, const int *chunk_rows,
+                        const int *seg, const int *split_n, const int *split_tab, const int *fix, const int *rp,
+                        const int *ci, const double *va, const void *x, void *y, double parity_tol, double *bound,
+                        int *n_recomputed) {
     int nchunks;
     long long L, segcap;
     orc_cvr64_sizes(m, nnz, S, &nchunks, &L, &segcap);
     const long long C = 64LL * S;
+    const int par = prec == 0 && (parity_tol > 0.0 || bound != NULL);
+    int nrc = 0;
+    if (bound)
+        for (int r = 0; r < m; ++r) bound[r] = 0.0;
     double *carry = (double *)calloc((size_t)(2 * nchunks + 2), sizeof(double));
+    double *cbA = (double *)calloc((size_t)(2 * nchunks + 2), sizeof(double));
+    double *cbQ = (double *)calloc((size_t)(2 * nchunks + 2), sizeof(double));
 #define STORE_Y(r, v)                               \
     do {                                            \
         if (prec == 0)                              \
@@ -358,10 +412,11 @@ void orc_cvr64_spmv(int m, int nnz, int S, int prec, const void *vals, const int
         const long long base = C * c;
         const int steps = (c == nchunks - 1) ? (int)((L - base) / 64) : S;
         const int *segs = seg + chunk_rows[c] + SEG_EXTRA * c;
-        double slots[SLOT_CAP];
-        double accd[LANES];
+        double slots[SLOT_CAP], slotq[SLOT_CAP];
+        double accd[LANES], accq[LANES];
         float accs[LANES], accc[LANES];
-        for (int l = 0; l < LANES; ++l) accd[l] = 0.0, accs[l] = 0.f, accc[l] = 0.f;
+        int st0[LANES];
+        for (int l = 0; l < LANES; ++l) accd[l] = 0.0, accq[l] = 0.0, accs[l] = 0.f, accc[l] = 0.f, st0[l] = 0;
         int segp = 0;
         for (int s = 0; s < steps; ++s) {
             for (int l = 0; l < LANES; ++l) {
@@ -369,6 +424,7 @@ void orc_cvr64_spmv(int m, int nnz, int S, int prec, const void *vals, const int
                 const int col = cols[o] & 0x7fffffff;
                 if (prec == 0) {
                     accd[l] = fma(((const double *)vals)[o], ((const double *)x)[col], accd[l]);
+                    accq[l] = fma(accd[l], accd[l], accq[l]);
                 } else {
                     float v = prec == 1 ? ((const float *)vals)[o] : bf16_to_f32(((const uint16_t *)vals)[o]);
                     float xv = ((const float *)x)[col];
@@ -388,11 +444,14 @@ void orc_cvr64_spmv(int m, int nnz, int S, int prec, const void *vals, const int
                 const int d = segs[segp + k];
                 ++k;
                 double v = prec == 0 ? accd[l] : (double)accs[l] + (double)accc[l];
-                if (d >= 0)
+                if (d >= 0) {
                     STORE_Y(d, v);
-                else
+                } else {
                     slots[~d] = v;
-                accd[l] = 0.0, accs[l] = 0.f, accc[l] = 0.f;
+                    slotq[~d] = par_pack(accq[l], s + 1 - st0[l]);
+                }
+                st0[l] = s + 1;
+                accd[l] = 0.0, accq[l] = 0.0, accs[l] = 0.f, accc[l] = 0.f;
             }
             segp += k;
         }
@@ -400,6 +459,37 @@ void orc_cvr64_spmv(int m, int nnz, int S, int prec, const void *vals, const int
         for (int g = 0; g < split_n[c]; ++g) {
             const int code = tab[2 * g], sb = tab[2 * g + 1], se = tab[2 * g + 3];
             double s = slots[sb];
+            if (par) {
+                double q0;
+                int n0;
+                par_unpack(slotq[sb], &q0, &n0);
+                double A = 2.0 * sqrt((double)n0 * q0), Q = 2.0 * q0;
+                int N = n0;
+                for (int j = sb + 1; j < se; ++j) {
+                    double qj;
+                    int nj;
+                    par_unpack(slotq[j], &qj, &nj);
+                    A += 2.0 * sqrt((double)nj * qj) + (double)nj * fabs(s);
+                    Q += 2.0 * (double)nj * s * s + 3.0 * qj;
+                    s += slots[j];
+                    A += fabs(s);
+                    Q += s * s;
+                    N += nj;
+                }
+                if (code >= 0) {
+                    if (bound) bound[code] = 0x1p-53 * (A + (double)N * 0x1p-1021);
+                    if (parity_tol > 0.0 && !par_ok(s, A, Q, N, parity_tol)) {
+                        s = serial_row(rp, ci, va, (const double *)x, code);
+                        ++nrc;
+                    }
+                    STORE_Y(code, s);
+                } else if (code == C_HEAD) {
+                    carry[2 * c] = s, cbA[2 * c] = A, cbQ[2 * c] = par_pack(Q, N);
+                } else if (code == C_TAIL) {
+                    carry[2 * c + 1] = s, cbA[2 * c + 1] = A, cbQ[2 * c + 1] = par_pack(Q, N);
+                }
+                continue;
+            }
             for (int j = sb + 1; j < se; ++j) s += slots[j];
             if (code >= 0)
                 STORE_Y(code, s);
@@ -416,12 +506,39 @@ void orc_cvr64_spmv(int m, int nnz, int S, int prec, const void *vals, const int
         if (row < 0) continue;
         const int c0 = fix[2 * c + 1];
         double s = carry[2 * c0 + 1];
+        if (par) {
+            double A = cbA[2 * c0 + 1], Q;
+            int N;
+            par_unpack(cbQ[2 * c0 + 1], &Q, &N);
+            for (int k = c0 + 1; k <= c; ++k) {
+                const int idx = k < c ? 2 * k + 1 : 2 * c;
+                double Qk;
+                int Nk;
+                par_unpack(cbQ[idx], &Qk, &Nk);
+                A += cbA[idx] + (double)Nk * fabs(s);
+                Q += 2.0 * Qk + 2.0 * (double)Nk * s * s;
+                s += carry[idx];
+                A += fabs(s);
+                Q += s * s;
+                N += Nk;
+            }
+            if (bound) bound[row] = 0x1p-53 * (A + (double)N * 0x1p-1021);
+            if (parity_tol > 0.0 && !par_ok(s, A, Q, N, parity_tol)) {
+                s = serial_row(rp, ci, va, (const double *)x, row);
+                ++nrc;
+            }
+            STORE_Y(row, s);
+            continue;
+        }
         for (int k = c0 + 1; k < c; ++k) s += carry[2 * k + 1];
         s += carry[2 * c];
         STORE_Y(row, s);
     }
 #undef STORE_Y
     free(carry);
+    free(cbA);
+    free(cbQ);
+    if (n_recomputed) *n_recomputed = nrc;
 }
 
 /* ---------------------------------------------------------------- CSR5-64 */

@@ -41,6 +41,18 @@ void orc_cvr64_spmv(int m, int nnz, int S, int prec, const void *vals, const int
                     const int *seg, const int *split_n, const int *split_tab, const int *fix, const int *rp,
                     const void *x, void *y);
 
+/* The same with the FP64 serial-parity pass of the GPU (prec 0): split rows
+ * whose running error bound cannot show |y_cvr - y_serial| <= parity_tol *
+ * |y_serial| (rigorous for rows <= 4096 elements, also probabilistic with
+ * lambda = 10 beyond) are recomputed as the serial FMA sum from the CSR (ci,
+ * va). bound (may be NULL, m entries): u * A of each split row (the rigorous
+ * bound on |y_cvr - y_serial|), 0 on unsplit rows. parity_tol = 0 with bound
+ * != NULL computes the bounds of the plain CVR order. */
+void orc_cvr64_spmv_par(int m, int nnz, int S, int prec, const void *vals, const int *cols, const int *chunk_rows,
+                        const int *seg, const int *split_n, const int *split_tab, const int *fix, const int *rp,
+                        const int *ci, const double *va, const void *x, void *y, double parity_tol, double *bound,
+                        int *n_recomputed);
+
 /* ---------------------------------------------------------------- CSR5-64
  * CPU emulation of the GPU csr5_hip format (CSR5 re-derived for 64 lanes:
  * csr5/detail/avx512/format_avx512.h, csr5_spmv_avx512.h) and of its

@@ -49,7 +49,7 @@ class _CSR(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("steps_per_chunk", C.c_int), ("precision", C.c_int), ("col_index_bits", C.c_int),
                 ("recompute", C.c_int), ("kappa_threshold", C.c_double), ("keep_fp64", C.c_int),
-                ("device", C.c_int), ("x_hot_kb", C.c_int), ("x_lds_kb", C.c_int)]
+                ("device", C.c_int), ("x_hot_kb", C.c_int), ("x_lds_kb", C.c_int), ("parity_tol", C.c_double)]
 
 
 class _Info(C.Structure):
@@ -58,7 +58,7 @@ class _Info(C.Structure):
                 ("layout_elems", C.c_longlong), ("seg_capacity", C.c_longlong), ("max_slots", C.c_int),
                 ("has_empty_rows", C.c_int), ("n_fixup_rows", C.c_int), ("device_bytes", C.c_longlong),
                 ("convert_ms", C.c_double), ("n_hot_cols", C.c_int), ("n_lds_cols", C.c_int),
-                ("hot_setup_ms", C.c_double)]
+                ("hot_setup_ms", C.c_double), ("parity_tol", C.c_double)]
 
 
 class _GenParams(C.Structure):
@@ -290,18 +290,41 @@ class CvrHip:
 
     def __init__(self, A: CSR, steps_per_chunk: int = 0, precision: int = PREC_F64, col_index_bits: int = 0,
                  recompute: bool = False, kappa_threshold: float = 64.0, keep_fp64: bool = False, device: int = 0,
-                 x_hot_kb: int = -1, x_lds_kb: int = -1):
+                 x_hot_kb: int = -1, x_lds_kb: int = -1, parity_tol: Optional[float] = None):
         L = lib()
-        o = _Opts()
-        L.cvr_hip_default_opts(C.byref(o))
-        o.steps_per_chunk, o.precision, o.col_index_bits = steps_per_chunk, precision, col_index_bits
-        o.recompute, o.kappa_threshold, o.keep_fp64, o.device = int(recompute), kappa_threshold, int(keep_fp64), device
-        o.x_hot_kb, o.x_lds_kb = x_hot_kb, x_lds_kb
+        o = self._opts(steps_per_chunk, precision, col_index_bits, recompute, kappa_threshold, keep_fp64, device,
+                       x_hot_kb, x_lds_kb, parity_tol)
         self._h = C.c_void_p()
         s = A.c_struct()
         _check(L.cvr_hip_create(C.byref(s), C.byref(o), C.byref(self._h)))
         self.A_shape = (A.numRows, A.numCols)
         self.precision = precision
+
+    @staticmethod
+    def _opts(steps_per_chunk=0, precision=PREC_F64, col_index_bits=0, recompute=False, kappa_threshold=64.0,
+              keep_fp64=False, device=0, x_hot_kb=-1, x_lds_kb=-1, parity_tol=None) -> _Opts:
+        o = _Opts()
+        lib().cvr_hip_default_opts(C.byref(o))
+        if parity_tol is not None:
+            o.parity_tol = parity_tol
+        o.steps_per_chunk, o.precision, o.col_index_bits = steps_per_chunk, precision, col_index_bits
+        o.recompute, o.kappa_threshold, o.keep_fp64, o.device = int(recompute), kappa_threshold, int(keep_fp64), device
+        o.x_hot_kb, o.x_lds_kb = x_hot_kb, x_lds_kb
+        return o
+
+    @classmethod
+    def from_device(cls, numRows: int, numCols: int, nnz: int, rp_ptr: int, ci_ptr: int, va_ptr: int,
+                    **kw) -> "CvrHip":
+        """cvr_hip_create_device: a CSR already in device memory (complete and
+        valid when called; not retained)."""
+        self = cls.__new__(cls)
+        self._h = C.c_void_p()
+        o = cls._opts(**kw)
+        _check(lib().cvr_hip_create_device(numRows, numCols, nnz, rp_ptr, ci_ptr, va_ptr, C.byref(o),
+                                           C.byref(self._h)))
+        self.A_shape = (numRows, numCols)
+        self.precision = o.precision
+        return self
 
     def close(self) -> None:
         if self._h:

@@ -32,6 +32,12 @@ struct cvr_hip_handle {
     int *d_fix = nullptr;
     double *d_carry = nullptr, *d_carry_abs = nullptr;
     double *d_carry_mm = nullptr;  // SpMM carries: 2 * nchunks * 32
+    // FP64 serial parity (opts.parity_tol > 0): bound terms of the carries
+    int par = 0;
+    double2 *d_carry_bnd = nullptr;
+    int *d_close_of = nullptr, *d_close_cnt = nullptr;  // rows spanning chunks closed inside the SpMV
+    int *d_row_intra = nullptr;    // per closing chunk: the row's chunks all run in one workgroup
+    int *d_wg_list = nullptr, *d_wg_start = nullptr;  // LDS variant: chunks of each workgroup
     unsigned char *d_rflag = nullptr;
     int *d_rc_count = nullptr;
     // recompute of long rows: the rows longer than kLongRow, cut in pieces
@@ -87,6 +93,7 @@ extern "C" void cvr_hip_default_opts(cvr_hip_opts *o) {
     o->device = 0;
     o->x_hot_kb = -1;  // automatic
     o->x_lds_kb = -1;
+    o->parity_tol = 1e-12;
 }
 
 static int value_size(int prec) { return prec == CVR_HIP_PREC_F64 ? 8 : prec == CVR_HIP_PREC_F32 ? 4 : 2; }
@@ -119,8 +126,11 @@ static int validate_opts(const cvr_hip_opts *o) {
         return set_error(PAVA_HIP_EINVAL, "col_index_bits must be 0 (automatic), 16 or 32, got %d", o->col_index_bits);
     if (o->x_hot_kb < -1) return set_error(PAVA_HIP_EINVAL, "x_hot_kb must be >= -1, got %d", o->x_hot_kb);
     if (o->x_lds_kb < -1) return set_error(PAVA_HIP_EINVAL, "x_lds_kb must be >= -1, got %d", o->x_lds_kb);
+    if (!(o->parity_tol >= 0.0)) return set_error(PAVA_HIP_EINVAL, "parity_tol must be >= 0, got %g", o->parity_tol);
     return 0;
 }
+
+static int plan_parity(cvr_hip_handle *h);
 
 // Core: conversion from a CSR in device memory.
 static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, const double *d_va,
@@ -134,6 +144,7 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     const long long C = 64LL * h->S;
     h->nchunks = nnz > 0 ? (int)((nnz + C - 1) / C) : 0;
     h->col16 = (o->col_index_bits == 16);
+    h->par = (h->prec == CVR_HIP_PREC_F64 && o->parity_tol > 0.0 && h->d_rp && nnz > 0) ? 1 : 0;
     if (o->col_index_bits == 0 && h->nchunks > 0 && o->x_hot_kb <= 0) {
         // automatic: 16-bit chunk-relative columns when every chunk's nonzeros
         // span fewer than 65,536 columns (an explicit hot-x request keeps 32)
@@ -183,6 +194,18 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
         if ((rc = dev_alloc(h, (void **)&h->d_carry_abs, (size_t)h->nchunks * 16 + 16))) return rc;
         if ((rc = dev_alloc(h, (void **)&h->d_rflag, cvr::rflag_bytes(M)))) return rc;
         if ((rc = dev_alloc(h, (void **)&h->d_rc_count, cvr::rc_count_len(M, h->nchunks) * 4 + 4))) return rc;
+    }
+    if (h->par) {
+        if ((rc = dev_alloc(h, (void **)&h->d_carry_bnd, (size_t)h->nchunks * 2 * sizeof(double2) + 32))) return rc;
+        if ((rc = dev_alloc(h, (void **)&h->d_rc_count, (size_t)h->nchunks * 8 + 4)))
+            return rc;
+        HIP_TRY(hipMemsetAsync(h->d_carry_bnd, 0, (size_t)h->nchunks * 2 * sizeof(double2) + 32, h->stream));
+        HIP_TRY(hipMemsetAsync(h->d_rc_count, 0, (size_t)h->nchunks * 8 + 4,
+                               h->stream));
+        if ((rc = dev_alloc(h, (void **)&h->d_close_of, (size_t)h->nchunks * 4))) return rc;
+        if ((rc = dev_alloc(h, (void **)&h->d_close_cnt, (size_t)h->nchunks * 4))) return rc;
+        HIP_TRY(hipMemsetAsync(h->d_close_of, 0xff, (size_t)h->nchunks * 4, h->stream));
+        HIP_TRY(hipMemsetAsync(h->d_close_cnt, 0, (size_t)h->nchunks * 4, h->stream));
     }
     if ((rc = dev_alloc(h, (void **)&h->d_empty, (size_t)((M + 31) / 32) * 4))) return rc;
     if ((rc = dev_alloc(h, (void **)&h->d_status, 64))) return rc;
@@ -271,6 +294,10 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
         return set_error(PAVA_HIP_ERANGE, "a chunk spans 65536 columns or more: 16-bit column layout not possible");
     h->n_fix = status[3];
     h->has_empty = status[4] ? 1 : 0;
+    if (h->par) {
+        HIP_TRY(cvr::launch_close_of(h->d_fix, h->d_close_of, h->nchunks, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
 
     // hot-x staging: renumber the most used columns into a dense copy of x
     const int xsize = h->prec == CVR_HIP_PREC_F64 ? 8 : 4;
@@ -298,11 +325,106 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
             int rc2 = dev_alloc(h, &h->d_xhot, (size_t)nh * xsize);
             if (rc2) return rc2;
             const long long lds_bytes = o->x_lds_kb < 0 ? -1 : (long long)o->x_lds_kb * 1024;
-            HIP_TRY(cvr::plan_spmv_lds(h->prec, h->d_rflag != nullptr, nh, lds_bytes, h->nchunks, &h->n_lds,
+            HIP_TRY(cvr::plan_spmv_lds(h->prec, h->d_rflag != nullptr || h->par, nh, lds_bytes, h->nchunks, &h->n_lds,
                                        &h->lds_blocks));
         }
         h->hot_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
+    if (h->par && (rc = plan_parity(h))) return rc;
+    return 0;
+}
+
+// FP64 parity: which workgroup runs each chunk, and for each row spanning
+// chunks whether all its chunks run in one workgroup (its carries then pass
+// through ordinary memory with workgroup-scope ordering, else through
+// agent-scope atomics). The plain kernel runs chunks 4g..4g+3 in one
+// workgroup. The LDS variant's workgroup w runs a list: first the chunks of
+// the long rows (longer than kSpreadLen) assigned to it -- long rows are dealt
+// to the least loaded workgroups, longest first, so that their serial
+// recomputes, when the bound requires them, start early and on different
+// CUs -- then a contiguous run of the other chunks, filled up to its share
+// nchunks/G. A chunk holding the end of one long row and the start of
+// another goes with the first; the second then spans two workgroups.
+constexpr int kSpreadLen = 2048;
+static int plan_parity(cvr_hip_handle *h) {
+    const int nc = h->nchunks;
+    std::vector<int> fix((size_t)nc * 2), len((size_t)nc);
+    int *d_len = nullptr;
+    if (hipMalloc(&d_len, (size_t)nc * 4) != hipSuccess) return set_error(PAVA_HIP_ENOMEM, "parity plan: hipMalloc failed");
+    hipError_t e = cvr::launch_closer_len(h->d_fix, h->d_rp, d_len, nc, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(len.data(), d_len, (size_t)nc * 4, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(fix.data(), h->d_fix, (size_t)nc * 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d_len);
+    if (e != hipSuccess) return set_error(PAVA_HIP_EDEVICE, "parity plan: %s", hipGetErrorString(e));
+    std::vector<int> wg_of((size_t)nc);
+    int rc;
+    if (h->n_lds > 0) {
+        const int G = h->lds_blocks;
+        std::vector<int> owner((size_t)nc, -1);
+        std::vector<std::vector<int>> lst((size_t)G);
+        std::vector<long long> load((size_t)G, 0);
+        std::vector<int> longk;
+        for (int k = 0; k < nc; ++k)
+            if (fix[2 * (size_t)k] >= 0 && len[(size_t)k] > kSpreadLen) longk.push_back(k);
+        std::stable_sort(longk.begin(), longk.end(), [&](int p, int q) { return len[(size_t)p] > len[(size_t)q]; });
+        std::vector<int> row_wg(longk.size());
+        for (size_t i = 0; i < longk.size(); ++i) {  // chunks c0+1 .. k belong to the long row closed by k
+            const int k = longk[i];
+            const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            row_wg[i] = w;
+            for (int c = fix[2 * (size_t)k + 1] + 1; c <= k; ++c) {
+                owner[(size_t)c] = w;
+                lst[(size_t)w].push_back(c);
+            }
+            load[(size_t)w] += k - fix[2 * (size_t)k + 1];
+        }
+        // the first chunk too, unless it closes another long row (then it runs
+        // early in that row's workgroup)
+        for (size_t i = 0; i < longk.size(); ++i) {
+            const int c0 = fix[2 * (size_t)longk[i] + 1], w = row_wg[i];
+            if (owner[(size_t)c0] >= 0) continue;
+            owner[(size_t)c0] = w;
+            lst[(size_t)w].insert(lst[(size_t)w].begin(), c0);
+            ++load[(size_t)w];
+        }
+        for (int c = 0; c < nc; ++c)
+            if (owner[(size_t)c] >= 0) wg_of[(size_t)c] = owner[(size_t)c];
+        int w = 0;
+        long long filled = load[0];
+        for (int c = 0; c < nc; ++c) {
+            if (owner[(size_t)c] >= 0) continue;
+            while (w < G - 1 && filled >= (long long)nc * (w + 1) / G - (long long)nc * w / G) {
+                ++w;
+                filled = load[(size_t)w];
+            }
+            lst[(size_t)w].push_back(c);
+            wg_of[(size_t)c] = w;
+            ++filled;
+        }
+        std::vector<int> list, start((size_t)G + 1, 0);
+        list.reserve((size_t)nc);
+        for (int g = 0; g < G; ++g) {
+            start[(size_t)g] = (int)list.size();
+            list.insert(list.end(), lst[(size_t)g].begin(), lst[(size_t)g].end());
+        }
+        start[(size_t)G] = (int)list.size();
+        if ((rc = dev_alloc(h, (void **)&h->d_wg_list, (size_t)nc * 4))) return rc;
+        if ((rc = dev_alloc(h, (void **)&h->d_wg_start, ((size_t)G + 1) * 4))) return rc;
+        HIP_TRY(hipMemcpy(h->d_wg_list, list.data(), (size_t)nc * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(h->d_wg_start, start.data(), ((size_t)G + 1) * 4, hipMemcpyHostToDevice));
+    } else {
+        for (int c = 0; c < nc; ++c) wg_of[(size_t)c] = c / cvr::kWavesPerBlock;
+    }
+    std::vector<int> intra((size_t)nc, 0);
+    for (int k = 0; k < nc; ++k) {
+        if (fix[2 * (size_t)k] < 0) continue;
+        int same = 1;
+        for (int c = fix[2 * (size_t)k + 1]; c < k && same; ++c) same = wg_of[(size_t)c] == wg_of[(size_t)k];
+        intra[(size_t)k] = same;
+    }
+    if ((rc = dev_alloc(h, (void **)&h->d_row_intra, (size_t)nc * 4))) return rc;
+    HIP_TRY(hipMemcpy(h->d_row_intra, intra.data(), (size_t)nc * 4, hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -358,7 +480,7 @@ extern "C" int cvr_hip_create(const pava_csr_matrix *A, const cvr_hip_opts *opts
     const int M = A->numRows, nnz = A->nnz;
     int *d_rp = nullptr, *d_ci = nullptr;
     double *d_va = nullptr;
-    const bool keep = o.keep_fp64 || o.recompute;
+    const bool keep = o.keep_fp64 || o.recompute || (o.precision == CVR_HIP_PREC_F64 && o.parity_tol > 0.0);
     auto fail = [&](int code) {
         if (!keep) {
             hipFree(d_rp);
@@ -381,13 +503,14 @@ extern "C" int cvr_hip_create(const pava_csr_matrix *A, const cvr_hip_opts *opts
         (nnz && hipMemcpy(d_ci, A->cols, (size_t)nnz * 4, hipMemcpyHostToDevice) != hipSuccess) ||
         (nnz && hipMemcpy(d_va, A->vals, (size_t)nnz * 8, hipMemcpyHostToDevice) != hipSuccess))
         return fail(set_error(PAVA_HIP_EDEVICE, "CSR upload failed"));
-    rc = create_core(M, A->numCols, nnz, d_rp, d_ci, d_va, &o, h);
-    if (rc) return fail(rc);
     if (keep) {
         h->d_rp = d_rp;
         h->d_ci = d_ci;
         h->d_va = d_va;
-    } else {
+    }
+    rc = create_core(M, A->numCols, nnz, d_rp, d_ci, d_va, &o, h);
+    if (rc) return fail(rc);
+    if (!keep) {
         hipFree(d_rp);
         hipFree(d_ci);
         hipFree(d_va);
@@ -405,7 +528,7 @@ extern "C" int cvr_hip_create_device(int numRows, int numCols, int nnz, const in
     cvr_hip_opts o;
     cvr_hip_handle *h = nullptr;
     if ((rc = begin_handle(opts, nnz, &o, out, &h))) return rc;
-    if (o.recompute || o.keep_fp64) {
+    if (o.recompute || o.keep_fp64 || (o.precision == CVR_HIP_PREC_F64 && o.parity_tol > 0.0)) {
         // keep a private copy of the FP64 CSR for the recompute pass
         if ((rc = dev_alloc(h, (void **)&h->d_rp, (size_t)(numRows + 1) * 4)) ||
             (rc = dev_alloc(h, (void **)&h->d_ci, (size_t)nnz * 4)) || (rc = dev_alloc(h, (void **)&h->d_va, (size_t)nnz * 8))) {
@@ -446,6 +569,7 @@ extern "C" int cvr_hip_get_info(const cvr_hip_handle *h, cvr_hip_info *info) {
     info->n_hot_cols = h->n_hot;
     info->n_lds_cols = h->n_lds;
     info->hot_setup_ms = h->hot_setup_ms;
+    info->parity_tol = h->par ? h->opts.parity_tol : 0.0;
     return 0;
 }
 
@@ -475,12 +599,26 @@ static cvr::SpmvArgs spmv_args(const cvr_hip_handle *h, const void *d_x, void *d
     a.recompute = h->d_rflag != nullptr;
     a.n_lds = h->n_lds;
     a.lds_blocks = h->lds_blocks;
+    if (h->par) {
+        a.rp = h->d_rp;
+        a.ci = h->d_ci;
+        a.va = h->d_va;
+        a.carry_bnd = h->d_carry_bnd;
+        a.rc_count = h->d_rc_count;
+        a.fix = h->d_fix;
+        a.close_of = h->d_close_of;
+        a.close_cnt = h->d_close_cnt;
+        a.row_intra = h->d_row_intra;
+        a.wg_list = h->d_wg_list;
+        a.wg_start = h->d_wg_start;
+        a.parity_tol = h->opts.parity_tol;
+    }
     return a;
 }
 
 // Kernels after the main SpMV: cross-chunk rows, and for mixed precision with
 // recompute the FP64 pass over rows flagged for cancellation.
-static int enqueue_tail(cvr_hip_handle *h, void *d_y, const double *d_x64, hipStream_t st) {
+static int enqueue_tail(cvr_hip_handle *h, const void *d_x, void *d_y, const double *d_x64, hipStream_t st) {
     if (h->d_rflag) {
         cvr::FinishArgs f{};
         f.fix = h->d_fix;
@@ -515,7 +653,7 @@ static int enqueue_tail(cvr_hip_handle *h, void *d_y, const double *d_x64, hipSt
             l.n_pieces = h->n_pieces;
             if (d_x64) HIP_TRY(cvr::launch_long_rows(l, st));
         }
-    } else if (h->n_fix > 0) {
+    } else if (h->n_fix > 0 && !h->par) {  // FP64 parity closes these rows inside the SpMV
         HIP_TRY(cvr::launch_fixup(h->d_fix, h->d_carry, d_y, h->nchunks, h->prec, st));
     }
     return 0;
@@ -527,7 +665,7 @@ static int enqueue_spmv(cvr_hip_handle *h, const void *d_x, void *d_y, hipStream
         HIP_TRY(cvr::launch_hot_gather(d_x, h->d_hot_cols, h->d_xhot, h->n_hot,
                                        h->prec == CVR_HIP_PREC_F64 ? 8 : 4, st));
     HIP_TRY(cvr::launch_spmv(a, h->prec, h->col16, st));
-    return enqueue_tail(h, d_y, d_x64, st);
+    return enqueue_tail(h, d_x, d_y, d_x64, st);
 }
 
 extern "C" int cvr_hip_spmv(cvr_hip_handle *h, const double *d_x, double *d_y, void *stream) {
@@ -774,6 +912,14 @@ extern "C" int cvr_hip_last_recompute_count(cvr_hip_handle *h, int *count) {
     if (!h->d_rc_count || h->nchunks == 0) return 0;
     HIP_TRY(hipSetDevice(h->device));
     HIP_TRY(hipDeviceSynchronize());
+    if (h->par) {
+        std::vector<int> v((size_t)h->nchunks * 2);
+        HIP_TRY(hipMemcpy(v.data(), h->d_rc_count, v.size() * 4, hipMemcpyDeviceToHost));
+        long long t = 0;
+        for (int c : v) t += c;
+        *count = (int)t;
+        return 0;
+    }
     std::vector<int> v(cvr::rc_count_len(h->M, h->nchunks) + (size_t)h->n_long);
     HIP_TRY(hipMemcpy(v.data(), h->d_rc_count, cvr::rc_count_len(h->M, h->nchunks) * 4, hipMemcpyDeviceToHost));
     if (h->n_long > 0)
@@ -827,7 +973,7 @@ extern "C" int cvr_hip_time_spmv_stats(cvr_hip_handle *h, const void *d_x, const
         if (e == hipSuccess) e = cvr::launch_spmv(sa, h->prec, h->col16, h->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[3 + 2 * (size_t)i], h->stream);
         if (e != hipSuccess) return finish(set_error(PAVA_HIP_EDEVICE, "timed launch: %s", hipGetErrorString(e)));
-        if ((rc = enqueue_tail(h, d_y, d_x64, h->stream))) return finish(rc);
+        if ((rc = enqueue_tail(h, d_x, d_y, d_x64, h->stream))) return finish(rc);
     }
     if (hipStreamSynchronize(h->stream) != hipSuccess) return finish(set_error(PAVA_HIP_EDEVICE, "timed SpMVs failed"));
     float ms = 0.f;

@@ -96,7 +96,78 @@ struct SpmvArgs {
     int nchunks, S, M, has_empty, recompute;
     int n_lds;       // hot columns [0, n_lds) read from an LDS copy (0: no LDS table)
     int lds_blocks;  // grid of the LDS variant
+    // FP64 serial parity (parity_tol > 0): the resident FP64 CSR, the bound
+    // terms of each head / tail carry, and the rows recomputed per chunk
+    const int *rp;
+    const int *ci;
+    const double *va;
+    double2 *carry_bnd;  // 2 per chunk: (A, packed Q|N) of the head / tail carry
+    int *rc_count;       // [c]: rows of chunk c recomputed in serial order by the last SpMV;
+                         // [nchunks + c]: 1 if the row closed by chunk c was
+    const int *fix;      // per chunk (row closed by the chunk or -1, its first chunk)
+    const int *close_of; // per chunk: the chunk closing the row its tail carry belongs to, or -1
+    int *close_cnt;      // per chunk: contributing chunks arrived (0 between SpMVs)
+    const int *row_intra;    // per closing chunk: 1 if every chunk of its row runs in one workgroup
+    const int *wg_list;      // LDS variant: the chunks of each workgroup (or null: contiguous ranges)
+    const int *wg_start;     // LDS variant: workgroup w runs wg_list[wg_start[w] .. wg_start[w + 1])
+    double parity_tol;
 };
+
+// ---------------------------------------------------------- FP64 parity
+// A row that the CVR-64 order splits into segments (steals, chunk
+// boundaries) is summed as per-segment FMA chains added in element order;
+// the serial reference sums it as one FMA chain. Both are within a running
+// error bound of the exact sum (Wilkinson's running error analysis: an FMA
+// chain's error is at most u * sum_j |s_j| over its computed partial sums
+// s_j). For a row cut into segments k = 0..K-1 with per-segment partial sums
+// t_j, q_k = sum t_j^2 and lengths n_k, merged as S_k = S_(k-1) + seg_k:
+//   CVR order:    u * (sum_k sum|t_j| + sum_(k>=1) |S_k|)
+//   serial order: u * sum_k sum_j |S_(k-1) + t_j| <= u * sum_k (n_k |S_(k-1)| + sum|t_j|)
+// with sum|t_j| <= sqrt(n_k q_k) (Cauchy-Schwarz), so
+//   |y_cvr - y_serial| <= u * A,  A = sum_k 2 sqrt(n_k q_k) + sum_(k>=1) (n_k |S_(k-1)| + |S_k|).
+// For rows longer than kParRigorousLen the probabilistic bound of Higham and
+// Mary (SIAM J. Sci. Comput. 41(5), 2019: with independent mean-zero
+// rounding errors, |error| <= lambda u sqrt(sum_j s_j^2) with probability at
+// least 1 - 2 exp(-lambda^2 / 2)) is also accepted, with lambda = 10 and
+//   Q = sum_k 3 q_k + sum_(k>=1) (2 n_k S_(k-1)^2 + S_k^2)
+// covering both orders ((S + t)^2 <= 2 S^2 + 2 t^2). A row whose bound can
+// exceed parity_tol * |y| is recomputed as one serial FMA chain, which is
+// bit-identical to the reference. Underflow is covered by an absolute term of
+// 2^-1074 per element.
+constexpr int kParRigorousLen = 4096;
+constexpr double kParLambda = 10.0;
+constexpr long long kParNMask = 0xFFFFF;  // low 20 bits of a packed Q hold N (N <= 64 * 4096)
+
+__host__ __device__ inline double par_pack(double q, int n) {
+    long long b;
+    __builtin_memcpy(&b, &q, 8);
+    b = (b & ~kParNMask) | (long long)n;
+    double r;
+    __builtin_memcpy(&r, &b, 8);
+    return r;
+}
+__host__ __device__ inline void par_unpack(double p, double *q, int *n) {
+    long long b;
+    __builtin_memcpy(&b, &p, 8);
+    *n = (int)(b & kParNMask);
+    b &= ~kParNMask;
+    __builtin_memcpy(q, &b, 8);
+}
+// true when |y_cvr - y_serial| <= tol * |y_serial| is certain (rigorous
+// bound) or, for rows longer than kParRigorousLen, holds under the
+// probabilistic bound; false (including NaN) -> recompute the row serially
+__host__ __device__ inline bool par_ok(double s, double A, double Q, int N, double tol) {
+    const double u = 0x1p-53;
+    const double lim = tol * __builtin_fabs(s);
+    const double Ar = A + (double)N * 0x1p-1021;  // underflow: 2^-1074 = u * 2^-1021 per element
+    if (u * Ar * (1.0 + 2e-6) <= lim) return true;
+    if (N > kParRigorousLen && Q > 0x1p-900) {
+        const double p = kParLambda * u;
+        if (p * p * Q * (1.0 + 4e-6) <= lim * lim) return true;
+    }
+    return false;
+}
+
 
 // FP64 recompute after a mixed-precision SpMV: one wave per 64 chunks for the
 // cross-chunk rows, then one wave per block of kFinishRows rows for the rows
@@ -166,10 +237,15 @@ hipError_t launch_chunk_span(const int *ci, long long nnz, long long C, int nchu
 hipError_t launch_convert(const ConvertArgs &a, hipStream_t st);
 hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st);
 // LDS table size for a handle: n_lds (0 = no LDS variant) and its grid.
-// max_bytes < 0: as much as the workgroup can hold.
-hipError_t plan_spmv_lds(int prec, int recompute, int n_hot, long long max_bytes, int nchunks, int *n_lds,
+// max_bytes < 0: as much as the workgroup can hold. bound: the variant that
+// carries per-segment bound terms (mixed-precision recompute, FP64 parity).
+hipError_t plan_spmv_lds(int prec, int bound, int n_hot, long long max_bytes, int nchunks, int *n_lds,
                          int *lds_blocks);
 hipError_t launch_fixup(const int *fix, const double *carry, void *y, int nchunks, int prec, hipStream_t st);
+// close_of[k] = the chunk closing the row that continues past chunk k (-1 if none)
+hipError_t launch_close_of(const int *fix, int *close_of, int nchunks, hipStream_t st);
+// len[c] = length of the row chunk c closes, or -1
+hipError_t launch_closer_len(const int *fix, const int *rp, int *len, int nchunks, hipStream_t st);
 hipError_t launch_finish(const FinishArgs &a, hipStream_t st);
 
 // Long flagged rows (static list of the rows longer than kLongRow).

@@ -71,6 +71,98 @@ struct Acc<double, double, false, false> {
     __device__ __forceinline__ void reset() { s = 0.0; }
 };
 
+// FP64 with serial parity: the same FMA chain, plus q = sum of the squared
+// partial sums (the bound terms of the segment; see par_ok).
+template <>
+struct Acc<double, double, false, true> {
+    double s = 0.0, q = 0.0;
+    __device__ __forceinline__ void add(double v, double x) {
+        s = __builtin_fma(v, x, s);
+        q = __builtin_fma(s, s, q);
+    }
+    __device__ __forceinline__ double value() const { return s; }
+    __device__ __forceinline__ double absval() const { return q; }
+    __device__ __forceinline__ void reset() { s = 0.0, q = 0.0; }
+};
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The serial FMA sum of elements [b, e) of the FP64 CSR (the reference's
+// order), by one wave. A round covers 64 * K consecutive elements, K per lane
+// (lane l holds elements j0 + K l .. j0 + K l + K - 1). The chain runs lane
+// by lane: all lanes apply their K FMAs to the running sum s (wave-uniform,
+// an SGPR operand), and lane l's result becomes the next s, so the chain
+// costs K dependent FMAs and one readlane per lane -- no LDS. While round r
+// is chained, the x gathers of rounds up to r + D and the columns and values
+// of round r + D + 1 are in flight; nothing waits on a load before the
+// chain needs it. Elements past e are masked to (0, 0) when chained, which
+// leaves s unchanged. All lanes must call it with the same b < e; the
+// result is wave-uniform.
+template <int K, int D>
+__device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, const double *__restrict__ va,
+                                                 const double *__restrict__ x, int b, int e, int lane) {
+    constexpr int R = 64 * K;
+    double s = 0.0;
+    int cn[K];                             // columns of round r + D + 1
+    double vq[D + 2][K], xq[D + 1][K];     // values of rounds r .. r + D + 1, x of rounds r .. r + D
+    auto load_cv = [&](int j0, int (&c)[K], double (&v)[K]) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int j = j0 + lane * K + k;
+            const unsigned jj = (unsigned)(j < e ? j : e - 1);
+            c[k] = __builtin_nontemporal_load(&ci[jj]);
+            v[k] = __builtin_nontemporal_load(&va[jj]);
+        }
+    };
+    auto gather = [&](const int (&c)[K], double (&xo)[K]) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) xo[k] = x[(unsigned)c[k]];
+    };
+    // prologue: rounds 0 .. D gathered, round D + 1 columns loading
+#pragma unroll
+    for (int d = 0; d <= D; ++d) {
+        load_cv(b + d * R, cn, vq[d]);
+        gather(cn, xq[d]);
+    }
+    load_cv(b + (D + 1) * R, cn, vq[D + 1]);
+    for (int j0 = b; j0 < e; j0 += R) {
+        const int nl = (e - j0 + K - 1) / K;  // lanes holding elements of this round
+        const int nl64 = nl < 64 ? nl : 64;
+        double vv[K], xx[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const bool in = j0 + lane * K + k < e;
+            vv[k] = in ? vq[0][k] : 0.0;
+            xx[k] = in ? xq[0][k] : 0.0;
+        }
+#pragma unroll 4
+        for (int l = 0; l < nl64; ++l) {
+            double t = __builtin_fma(vv[0], xx[0], s);
+#pragma unroll
+            for (int k = 1; k < K; ++k) t = __builtin_fma(vv[k], xx[k], t);
+            s = readlane_f64(t, l);
+        }
+        // shift: gather round r + D + 1, load round r + D + 2
+        if (j0 + R < e) {
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+#pragma unroll
+                for (int k = 0; k < K; ++k) xq[d][k] = xq[d + 1][k];
+#pragma unroll
+            for (int d = 0; d <= D; ++d)
+#pragma unroll
+                for (int k = 0; k < K; ++k) vq[d][k] = vq[d + 1][k];
+            gather(cn, xq[D]);
+            load_cv(j0 + (D + 2) * R, cn, vq[D + 1]);
+        }
+    }
+    return s;
+}
+
 // KAPPA also accumulates sum|v x| (the cancellation scale) for the FP64
 // recompute decision.
 template <typename VT, bool KAPPA>
@@ -98,6 +190,36 @@ __device__ __forceinline__ void store_y(void *y, int r, double v) {
     reinterpret_cast<YT *>(y)[r] = (YT)v;
 }
 
+// FP64 parity carries (sum, bound terms). A row whose chunks are all run by
+// one workgroup passes them through ordinary stores and loads, ordered by
+// workgroup-scope fences; a row spanning workgroups passes them with
+// agent-scope atomic stores and loads, which go past the XCD's L2, so its
+// closer needs no L2 invalidate and its writers no L2 write-back.
+__device__ __forceinline__ void put_carry(const SpmvArgs &a, int idx, double s, double A, double QN, bool coherent) {
+    if (coherent) {
+        double *bb = reinterpret_cast<double *>(a.carry_bnd + idx);
+        __hip_atomic_store(&a.carry[idx], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&bb[0], A, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&bb[1], QN, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        a.carry[idx] = s;
+        a.carry_bnd[idx] = make_double2(A, QN);
+    }
+}
+__device__ __forceinline__ void get_carry(const SpmvArgs &a, int idx, double *s, double *A, double *QN,
+                                          bool coherent) {
+    if (coherent) {
+        const double *bb = reinterpret_cast<const double *>(a.carry_bnd + idx);
+        *s = __hip_atomic_load(&a.carry[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *A = __hip_atomic_load(&bb[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *QN = __hip_atomic_load(&bb[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        *s = a.carry[idx];
+        const double2 b = a.carry_bnd[idx];
+        *A = b.x, *QN = b.y;
+    }
+}
+
 // LDSX: kWavesLds-wave workgroups (one per CU) first copy the head of the hot-x
 // buffer -- the most used columns, hot numbers [0, n_lds) -- into LDS, then
 // each wave walks chunks of the workgroup's contiguous chunk range; a gather
@@ -114,10 +236,17 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     constexpr int G = kStepGroup;
+    // FP64 with serial parity: split rows carry bound terms, and the rows that
+    // fail the bound are recomputed in serial order
+    constexpr bool PAR = KAPPA && !COMP;
 
     __shared__ double slots[WPB][kSlotCap];
+    // KAPPA: sum|a x| of each slot (mixed precision), or q|n packed (PAR)
     __shared__ double slots_abs[KAPPA ? WPB : 1][KAPPA ? kSlotCap : 1];
-    __shared__ int segbuf[WPB][64 * kStepGroup];
+    // group segment entries in the main loop; then (PAR) the list of rows to
+    // recompute (ints [0, 128))
+    __shared__ __align__(16) int segbuf[WPB][64 * kStepGroup];
+    __shared__ int fcnt[PAR ? WPB : 1];
     extern __shared__ __align__(16) unsigned char lds_tab[];
     const XT *__restrict__ tab = reinterpret_cast<const XT *>(lds_tab);
 
@@ -201,6 +330,7 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
         uint64_t eb = 0, en = 0, enn = 0;
         XT xb[G], xn[G];
         int sc[G], sn[G];
+        int st0 = 0;  // PAR: first step of the lane's current segment
 
         // prologue: group 0 fully issued, columns of group 1 in flight
         load_cols(0, cb, eb);
@@ -254,11 +384,15 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                         const double v = acc.value();
                         if (d >= 0) {
                             store_y<YT>(a.y, d, v);
-                            if (KAPPA && acc.absval() > (double)a.kappa * __builtin_fabs(v)) a.rflag[d] = 1;
+                            if (KAPPA && COMP && acc.absval() > (double)a.kappa * __builtin_fabs(v)) a.rflag[d] = 1;
                         } else {
                             slots[wv][~d] = v;
-                            if (KAPPA) slots_abs[wv][~d] = acc.absval();
+                            if (PAR)
+                                slots_abs[wv][~d] = par_pack(acc.absval(), g + i + 1 - st0);
+                            else if (KAPPA)
+                                slots_abs[wv][~d] = acc.absval();
                         }
+                        if (PAR) st0 = g + i + 1;
                         acc.reset();
                     }
                     segp += __popcll(m);
@@ -282,12 +416,46 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
         }
 
         // split rows: sum slots in element order
+        int *flist = &segbuf[wv][0];
+        if (PAR && lane == 0) fcnt[wv] = 0;
         wave_sync_lds();
         const int ns = a.split_n[c];
         const int2 *tab = a.split_tab + (size_t)c * kSplitCap;
         for (int k = lane; k < ns; k += 64) {
             const int2 t = tab[k];
             const int se = tab[k + 1].y;
+            if constexpr (PAR) {
+                // merged sum and the bound terms of par_ok
+                double s = slots[wv][t.y], q0;
+                int n0;
+                par_unpack(slots_abs[wv][t.y], &q0, &n0);
+                double A = 2.0 * __builtin_sqrt((double)n0 * q0), Q = 2.0 * q0;
+                int N = n0;
+                for (int j = t.y + 1; j < se; ++j) {
+                    double qj;
+                    int nj;
+                    par_unpack(slots_abs[wv][j], &qj, &nj);
+                    A += 2.0 * __builtin_sqrt((double)nj * qj) + (double)nj * __builtin_fabs(s);
+                    Q += 2.0 * (double)nj * s * s + 3.0 * qj;
+                    s += slots[wv][j];
+                    A += __builtin_fabs(s);
+                    Q += s * s;
+                    N += nj;
+                }
+                if (t.x >= 0) {
+                    if (par_ok(s, A, Q, N, a.parity_tol))
+                        store_y<YT>(a.y, t.x, s);
+                    else
+                        flist[atomicAdd(&fcnt[wv], 1)] = t.x;
+                } else if (t.x == kCodeHead) {
+                    // the row closed by this chunk
+                    put_carry(a, 2 * c, s, A, par_pack(Q, N), !a.row_intra[c]);
+                } else if (t.x == kCodeTail) {
+                    const int k = a.close_of[c];
+                    put_carry(a, 2 * c + 1, s, A, par_pack(Q, N), !a.row_intra[k]);
+                }
+                continue;
+            }
             double s = slots[wv][t.y];
             for (int j = t.y + 1; j < se; ++j) s += slots[wv][j];
             double sa = 0.0;
@@ -302,6 +470,80 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
             } else if (t.x == kCodeTail) {
                 a.carry[2 * c + 1] = s;
                 if (KAPPA) a.carry_abs[2 * c + 1] = sa;
+            }
+        }
+        if constexpr (PAR) {
+            // rows of this chunk whose CVR-order sum may differ from the
+            // serial sum by more than parity_tol: one serial FMA chain each
+            wave_sync_lds();
+            const int nf = __builtin_amdgcn_readfirstlane(fcnt[wv]);
+            for (int f = 0; f < nf; ++f) {
+                const int row = __builtin_amdgcn_readfirstlane(flist[f]);
+                const int b = a.rp[row], e = a.rp[row + 1];
+                const double v = serial_fma_sum<LDSX ? 8 : 4, 1>(a.ci, a.va, reinterpret_cast<const double *>(a.x), b, e, lane);
+                if (lane == 0) store_y<YT>(a.y, row, v);
+            }
+            if (lane == 0) a.rc_count[c] = nf;
+
+            // rows spanning chunks: this chunk's head carry belongs to the row
+            // the chunk closes, its tail carry to the row closed by
+            // close_of[c]. Each contributing chunk counts itself in; the last
+            // one to arrive sums the carries in chunk order with their bound
+            // terms (carry: sum |S + w| <= N |S| + A, sum (S + w)^2 <=
+            // 2 N S^2 + 2 Q) and writes the row, recomputing it serially when
+            // par_ok fails. No wave waits for another; the counter is reset
+            // by its closer for the next SpMV.
+            const int targets[2] = {a.fix[2 * c] >= 0 ? c : -1, a.close_of[c]};
+            // the carries are in place before this chunk counts itself in:
+            // the workgroup-scope release for rows inside the range, and the
+            // completion of the agent-scope carry stores for rows spanning it
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            {
+                const int kt = targets[1];
+                const bool span_head = targets[0] >= 0 && !a.row_intra[c];
+                const bool span_tail = kt >= 0 && !a.row_intra[kt];
+                if (span_head || span_tail) __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int k = targets[t];
+                if (k < 0) continue;
+                const int c0 = a.fix[2 * k + 1];
+                const bool intra = a.row_intra[k] != 0;
+                int old = 0;
+                if (lane == 0)
+                    old = __hip_atomic_fetch_add(&a.close_cnt[k], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                old = __builtin_amdgcn_readfirstlane(old);
+                if (old != k - c0) continue;
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                const int row = a.fix[2 * k];
+                double s, A, Q, QN;
+                int N;
+                get_carry(a, 2 * c0 + 1, &s, &A, &QN, !intra);
+                par_unpack(QN, &Q, &N);
+                for (int j = c0 + 1; j <= k; ++j) {
+                    const int idx = j < k ? 2 * j + 1 : 2 * k;
+                    double sj, Aj, QNj;
+                    get_carry(a, idx, &sj, &Aj, &QNj, !intra);
+                    double Qj;
+                    int Nj;
+                    par_unpack(QNj, &Qj, &Nj);
+                    A += Aj + (double)Nj * __builtin_fabs(s);
+                    Q += 2.0 * Qj + 2.0 * (double)Nj * s * s;
+                    s += sj;
+                    A += __builtin_fabs(s);
+                    Q += s * s;
+                    N += Nj;
+                }
+                const bool ok = par_ok(s, A, Q, N, a.parity_tol);
+                if (!ok)
+                    s = serial_fma_sum<LDSX ? 8 : 4, 1>(a.ci, a.va, reinterpret_cast<const double *>(a.x), a.rp[row],
+                                                     a.rp[row + 1], lane);
+                if (lane == 0) {
+                    store_y<YT>(a.y, row, s);
+                    a.rc_count[a.nchunks + k] = ok ? 0 : 1;
+                    __hip_atomic_store(&a.close_cnt[k], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
         }
 
@@ -325,15 +567,34 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
     };
 
     if constexpr (LDSX) {
+        // the workgroup's chunks, handed to its waves from an LDS counter (a
+        // wave busy with a serial recompute leaves the rest to the others):
+        // a contiguous range, or with FP64 parity the list wg_list[wg_start[w]
+        // .. wg_start[w + 1]), which begins with the chunks of the long rows
+        // assigned to this workgroup (long rows are spread over the
+        // workgroups, so that their serial recomputes, when needed, start
+        // early and in parallel) followed by a contiguous run of the others
+        __shared__ int next_chunk;
+        const int lb = xcd_remap(blockIdx.x, gridDim.x);
+        const bool listed = PAR && a.wg_list;
+        const int c_lo = listed ? a.wg_start[lb] : (int)((long long)a.nchunks * lb / gridDim.x);
+        const int c_end = listed ? a.wg_start[lb + 1] : (int)((long long)a.nchunks * (lb + 1) / gridDim.x);
         XT *t = reinterpret_cast<XT *>(lds_tab);
         const XT *__restrict__ src = reinterpret_cast<const XT *>(a.xhot);
         for (int i = threadIdx.x; i < a.n_lds; i += blockDim.x) t[i] = src[i];
+        if (threadIdx.x == 0) next_chunk = c_lo;
         __syncthreads();
-        const int lb = xcd_remap(blockIdx.x, gridDim.x);
-        const int c_end = (int)((long long)a.nchunks * (lb + 1) / gridDim.x);
-        for (int c = (int)((long long)a.nchunks * lb / gridDim.x) + wv; c < c_end; c += WPB) run_chunk(c);
+        for (;;) {
+            int c = 0;
+            if (lane == 0) c = atomicAdd(&next_chunk, 1);
+            c = __builtin_amdgcn_readfirstlane(c);
+            if (c >= c_end) break;
+            if (listed) c = a.wg_list[c];
+            run_chunk(c);
+        }
     } else {
-        const int c = xcd_remap(blockIdx.x, gridDim.x) * WPB + wv;
+        const int g = xcd_remap(blockIdx.x, gridDim.x) * WPB;
+        const int c = g + wv;
         if (c < a.nchunks) run_chunk(c);
     }
 }
@@ -352,6 +613,23 @@ __global__ void fixup_kernel(const int *__restrict__ fix, const double *__restri
     for (int k = c0 + 1; k < c; ++k) s += carry[2 * k + 1];
     s += carry[2 * c];
     reinterpret_cast<YT *>(y)[row] = (YT)s;
+}
+
+// len[c] = length of the row chunk c closes (rows spanning chunks), or -1
+__global__ void closer_len_kernel(const int *__restrict__ fix, const int *__restrict__ rp, int *__restrict__ len,
+                                  int nchunks) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchunks) return;
+    const int r = fix[2 * c];
+    len[c] = r >= 0 ? rp[r + 1] - rp[r] : -1;
+}
+
+// close_of[k] = c for every chunk k in [c0, c) of a row that chunk c closes
+// (close_of is -1 beforehand)
+__global__ void close_of_kernel(const int *__restrict__ fix, int *__restrict__ close_of, int nchunks) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchunks || fix[2 * c] < 0) return;
+    for (int k = fix[2 * c + 1]; k < c; ++k) close_of[k] = c;
 }
 
 template <typename YT>
@@ -644,9 +922,15 @@ hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st) {
     if (a.n_lds > 0 && (col16 || a.lds_blocks <= 0)) return hipErrorInvalidValue;
     dim3 grid((a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock);
     if (prec == 0) {
-        if (col16)
+        const bool par = a.parity_tol > 0.0;
+        if (col16 && par)
+            hipLaunchKernelGGL((spmv_kernel<double, double, double, false, true, true, false>), grid,
+                               dim3(64 * kWavesPerBlock), 0, st, a);
+        else if (col16)
             hipLaunchKernelGGL((spmv_kernel<double, double, double, false, true, false, false>), grid,
                                dim3(64 * kWavesPerBlock), 0, st, a);
+        else if (par)
+            launch_one<double, double, double, false, true>(a, grid, st);
         else
             launch_one<double, double, double, false, false>(a, grid, st);
     } else if (prec == 1) {
@@ -668,12 +952,14 @@ static const void *lds_kernel() {
     return reinterpret_cast<const void *>(&spmv_kernel<VT, XT, XT, COMP, false, KAPPA, true>);
 }
 
-hipError_t plan_spmv_lds(int prec, int recompute, int n_hot, long long max_bytes, int nchunks, int *n_lds,
+hipError_t plan_spmv_lds(int prec, int bound, int n_hot, long long max_bytes, int nchunks, int *n_lds,
                          int *lds_blocks) {
     *n_lds = 0;
     *lds_blocks = 0;
     if (n_hot <= 0 || max_bytes == 0 || nchunks <= 0) return hipSuccess;
-    const void *fn = prec == 0   ? lds_kernel<double, double, false, false>()
+    const bool recompute = bound != 0;
+    const void *fn = prec == 0   ? (bound ? lds_kernel<double, double, false, true>()
+                                          : lds_kernel<double, double, false, false>())
                      : prec == 1 ? (recompute ? lds_kernel<float, float, true, true>()
                                               : lds_kernel<float, float, true, false>())
                                  : (recompute ? lds_kernel<unsigned short, float, true, true>()
@@ -720,6 +1006,18 @@ hipError_t launch_long_rows(const LongArgs &a, hipStream_t st) {
     hipLaunchKernelGGL(long_piece_kernel, dim3((a.n_pieces + kWavesPerBlock - 1) / kWavesPerBlock),
                        dim3(64 * kWavesPerBlock), 0, st, a);
     hipLaunchKernelGGL(long_sum_kernel, dim3((a.n_long + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_closer_len(const int *fix, const int *rp, int *len, int nchunks, hipStream_t st) {
+    if (nchunks == 0) return hipSuccess;
+    hipLaunchKernelGGL(closer_len_kernel, dim3((nchunks + 255) / 256), dim3(256), 0, st, fix, rp, len, nchunks);
+    return hipGetLastError();
+}
+
+hipError_t launch_close_of(const int *fix, int *close_of, int nchunks, hipStream_t st) {
+    if (nchunks == 0) return hipSuccess;
+    hipLaunchKernelGGL(close_of_kernel, dim3((nchunks + 255) / 256), dim3(256), 0, st, fix, close_of, nchunks);
     return hipGetLastError();
 }
 
