@@ -218,7 +218,13 @@ typedef struct cvr_hip_handle cvr_hip_handle;
 void cvr_hip_default_opts(cvr_hip_opts *o);
 /* Host CSR -> device upload -> GPU CSR->CVR conversion. Synchronous. */
 int cvr_hip_create(const pava_csr_matrix *A, const cvr_hip_opts *opts, cvr_hip_handle **out);
-/* Same from a CSR already in device memory (the arrays are not retained). */
+/* Same from a CSR already in device memory (the arrays are not retained). The
+ * call first waits for all work on the device (the CSR may still be written
+ * by the caller's streams), then checks the CSR on the device: offsets from 0
+ * to nnz, non-decreasing (PAVA_HIP_EINVAL otherwise) and every column in
+ * [0, numCols) (PAVA_HIP_ERANGE). 32-bit column layouts (col_index_bits 32, or
+ * automatic when a chunk spans 65,536 columns or more) take at most 2^30
+ * columns (PAVA_HIP_ERANGE); 16-bit layouts take up to PAVA_HIP_MAX_SIZE. */
 int cvr_hip_create_device(int numRows, int numCols, int nnz, const int *d_rowOffsets,
                           const int *d_cols, const double *d_vals, const cvr_hip_opts *opts,
                           cvr_hip_handle **out);
@@ -244,8 +250,11 @@ int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *s
  * precision, 16- or 32-bit columns, nnz > 0) form one work queue consumed by a
  * persistent grid sized to the GPU's resident capacity; a fixup kernel closes
  * rows spanning chunks and rearms the queue, so the launch pair can be captured
- * once in a hipGraph and replayed. The handles must outlive the batch.
- * Accumulation is FP64 over FP32 inputs (one rounding per output). */
+ * once in a hipGraph and replayed. The handles must outlive the batch; a
+ * handle may appear once per batch (EINVAL otherwise), and must not be used by
+ * cvr_hip_spmm or another batch while a batch holding it is in flight (they
+ * share its cross-chunk buffer). Accumulation is FP64 over FP32 inputs (one
+ * rounding per output). */
 typedef struct cvr_hip_batch cvr_hip_batch;
 int cvr_hip_batch_create(int nmat, cvr_hip_handle *const *handles, int k, cvr_hip_batch **out);
 /* Device pointers of each matrix's X (numCols x k) and Y (numRows x k), FP32
@@ -393,6 +402,15 @@ int conduct_csr5_hip(int m, int n, int nnz, int *csrRowPtr, int *csrColIdx, doub
  * with checkResults. Returns the total error count (>= 0), -1 if the matrix
  * was skipped, or a PAVA_HIP_E* code. */
 int pava_bench_file(const char *path, int numIterations);
+
+/* Test hooks (no counterpart in the reference; for tests of the failure
+ * paths). pava_hip_test_fail_alloc(h, d): counted from this call, the h-th
+ * host allocation of the matrix readers and converters and the d-th device
+ * allocation of the CVR-HIP handle and batch code fail as if memory had run
+ * out (0 = no failure). pava_hip_test_live_device_allocs: device allocations
+ * of that code currently held. */
+void pava_hip_test_fail_alloc(int host_nth, int device_nth);
+long long pava_hip_test_live_device_allocs(void);
 
 const char *pava_hip_last_error(void);
 /* Library version string. */

@@ -86,6 +86,7 @@ EXPORTED = [
     "pava_read_csr_binary", "conduct_csr_hip", "csr5_hip_create", "csr5_hip_spmv", "csr5_hip_spmv_host",
     "csr5_hip_get_info", "csr5_hip_export_layout", "csr5_hip_destroy", "conduct_csr5_hip", "coo_hip_create",
     "coo_hip_spmv", "coo_hip_get_info", "coo_hip_destroy", "conduct_coo_hip", "cvr_hip_batch_queue_counts",
+    "pava_hip_test_fail_alloc", "pava_hip_test_live_device_allocs",
 ]
 
 
@@ -150,6 +151,10 @@ def lib() -> C.CDLL:
     L.cvr_hip_batch_info.argtypes = [P, C.POINTER(i), C.POINTER(i), C.POINTER(C.c_longlong)]
     L.cvr_hip_batch_destroy.argtypes = [P]
     L.cvr_hip_batch_queue_counts.argtypes = [P, C.POINTER(i), C.POINTER(i)]
+    L.pava_hip_test_fail_alloc.argtypes = [i, i]
+    L.pava_hip_test_fail_alloc.restype = None
+    L.pava_hip_test_live_device_allocs.argtypes = []
+    L.pava_hip_test_live_device_allocs.restype = C.c_longlong
     _lib = L
     return L
 

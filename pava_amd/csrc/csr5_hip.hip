@@ -291,10 +291,10 @@ static int csr5_alloc(csr5_hip_handle *h, void **p, size_t bytes) {
 
 extern "C" int csr5_hip_destroy(csr5_hip_handle *h) {
     if (!h) return 0;
-    hipSetDevice(h->device);
-    if (h->stream) hipStreamSynchronize(h->stream);
-    for (void *p : h->allocs) hipFree(p);
-    if (h->stream) hipStreamDestroy(h->stream);
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *p : h->allocs) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return 0;
 }
@@ -322,12 +322,13 @@ static int csr5_build(const pava_csr_matrix *A, csr5_hip_handle *h) {
     if (hipMalloc(&d_rp, (size_t)(M + 1) * 4) != hipSuccess || hipMalloc(&d_ci, (size_t)nnz * 4 + 4) != hipSuccess ||
         hipMalloc(&d_va, (size_t)nnz * 8 + 8) != hipSuccess || hipMalloc(&d_status, 64) != hipSuccess ||
         hipMalloc(&d_count, (size_t)h->ntiles * 4 + 4) != hipSuccess) {
-        hipFree(d_rp), hipFree(d_ci), hipFree(d_va), hipFree(d_status), hipFree(d_count);
+        (void)hipFree(d_rp), (void)hipFree(d_ci), (void)hipFree(d_va), (void)hipFree(d_status), (void)hipFree(d_count);
         return set_error(PAVA_HIP_ENOMEM, "device allocation for the CSR5 conversion failed");
     }
     auto done = [&](int r) {
-        hipStreamSynchronize(h->stream);
-        hipFree(d_rp), hipFree(d_ci), hipFree(d_va), hipFree(d_status), hipFree(d_count);
+        if (hipStreamSynchronize(h->stream) != hipSuccess && r == 0)
+            r = set_error(PAVA_HIP_EDEVICE, "CSR5 conversion failed: %s", hipGetErrorString(hipGetLastError()));
+        (void)hipFree(d_rp), (void)hipFree(d_ci), (void)hipFree(d_va), (void)hipFree(d_status), (void)hipFree(d_count);
         return r;
     };
     hipEvent_t e0, e1;
@@ -364,9 +365,9 @@ static int csr5_build(const pava_csr_matrix *A, csr5_hip_handle *h) {
     }
     ok = ok && hipEventRecord(e1, h->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
     float ms = 0.f;
-    if (ok) hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (ok) ok = hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (!ok) return done(set_error(PAVA_HIP_EDEVICE, "CSR5 conversion failed"));
     int status[16] = {0};
     if (hipMemcpy(status, d_status, sizeof status, hipMemcpyDeviceToHost) != hipSuccess)
@@ -494,8 +495,8 @@ extern "C" int conduct_csr5_hip(int numRows, int numCols, int nnz, int *rowOffse
     if (rc) return rc;
     double *d_x = nullptr, *d_y = nullptr;
     auto fin = [&](int r) {
-        hipFree(d_x);
-        hipFree(d_y);
+        (void)hipFree(d_x);
+        (void)hipFree(d_y);
         csr5_hip_destroy(h);
         return r;
     };
@@ -506,7 +507,7 @@ extern "C" int conduct_csr5_hip(int numRows, int numCols, int nnz, int *rowOffse
     if ((rc = csr5_enqueue(h, d_x, d_y, nullptr))) return fin(rc);
     if (hipMemcpy(y, d_y, (size_t)numRows * 8, hipMemcpyDeviceToHost) != hipSuccess)
         return fin(set_error(PAVA_HIP_EDEVICE, "y download failed"));
-    hipDeviceSynchronize();
+    if (hipDeviceSynchronize() != hipSuccess) return fin(set_error(PAVA_HIP_EDEVICE, "verification SpMV failed"));
     const double t3 = pava_microtime();
     for (int it = 0; it < numIterations; ++it)
         if ((rc = csr5_enqueue(h, d_x, d_y, nullptr))) return fin(rc);

@@ -831,6 +831,34 @@ hipError_t launch_chunk_span(const int *ci, long long nnz, long long C, int nchu
     return hipGetLastError();
 }
 
+// A CSR handed over in device memory: rp[0] == 0, rp[M] == nnz, rp monotone,
+// every column in [0, N). *bad |= 1 (offsets) or 2 (columns); *bad is
+// cleared by the caller.
+__global__ void validate_csr_kernel(const int *__restrict__ rp, const int *__restrict__ ci, int M, int N, long long nnz,
+                                    int *bad) {
+    int f = 0;
+    const long long n = (long long)M > nnz ? (long long)M : nnz;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (i < M && rp[i + 1] < rp[i]) f |= 1;
+        if (i < nnz) {
+            const int c = ci[i];
+            if (c < 0 || c >= N) f |= 2;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (rp[0] != 0 || (long long)rp[M] != nnz)) f |= 1;
+    if (f) atomicOr(bad, f);
+}
+
+hipError_t launch_validate_csr(const int *rp, const int *ci, int M, int N, long long nnz, int *bad, hipStream_t st) {
+    long long n = (long long)M > nnz ? (long long)M : nnz;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(validate_csr_kernel, dim3((unsigned)blocks), dim3(256), 0, st, rp, ci, M, N, nnz, bad);
+    return hipGetLastError();
+}
+
 hipError_t launch_empty_rows(const int *rp, int M, unsigned *bits, int *status, hipStream_t st) {
     int nw = (M + 31) >> 5;
     if (nw == 0) return hipSuccess;

@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <memory>
+#include <new>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -70,15 +73,55 @@ struct cvr_hip_handle {
                              __FILE__, __LINE__);                                                       \
     } while (0)
 
+// Every device allocation of the CVR handle code goes through dmalloc/dfree:
+// they honour the failure injection of pava_hip_test_fail_alloc and count the
+// live allocations (pava_hip_test_live_device_allocs), so tests can check that
+// a failed creation releases everything it took.
+static std::atomic<long long> g_live_device_allocs{0};
+template <typename T>
+static hipError_t dmalloc(T **p, size_t bytes) {
+    *p = nullptr;
+    if (inject_device_failure()) return hipErrorOutOfMemory;
+    void *q = nullptr;
+    const hipError_t e = hipMalloc(&q, bytes);
+    if (e == hipSuccess) {
+        *p = (T *)q;
+        ++g_live_device_allocs;
+    }
+    return e;
+}
+static void dfree(void *p) {
+    if (!p) return;
+    (void)hipFree(p);
+    --g_live_device_allocs;
+}
+
+extern "C" long long pava_hip_test_live_device_allocs(void) { return g_live_device_allocs.load(); }
+
+// a failed HIP call: PAVA_HIP_ENOMEM when it ran out of memory, else EDEVICE
+static int hip_fail(hipError_t e, const char *what) {
+    return set_error(e == hipErrorOutOfMemory ? PAVA_HIP_ENOMEM : PAVA_HIP_EDEVICE, "%s: %s", what,
+                     hipGetErrorString(e));
+}
+
 static int dev_alloc(cvr_hip_handle *h, void **p, size_t bytes) {
     if (bytes == 0) bytes = 256;
-    hipError_t e = hipMalloc(p, bytes);
+    hipError_t e = dmalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
         return set_error(PAVA_HIP_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
     }
     h->allocs.push_back(*p);
     h->device_bytes += (long long)bytes;
+    return 0;
+}
+
+// 32-bit CVR layouts store a column with flags in bits 31 (segment end) and 30
+// (staged hot column): at most 2^30 columns.
+constexpr long long kCol32Limit = 1LL << 30;
+static int check_col_limit(const cvr_hip_opts *opts, long long numCols) {
+    if (opts && opts->col_index_bits == 32 && numCols > kCol32Limit)
+        return set_error(PAVA_HIP_ERANGE, "32-bit CVR column layouts take at most 2^30 columns, got %lld", numCols);
     return 0;
 }
 
@@ -108,10 +151,10 @@ constexpr double kAutoHotSkew = 2.0;
 
 extern "C" int cvr_hip_destroy(cvr_hip_handle *h) {
     if (!h) return 0;
-    hipSetDevice(h->device);
-    if (h->stream) hipStreamSynchronize(h->stream);
-    for (void *p : h->allocs) hipFree(p);
-    if (h->stream) hipStreamDestroy(h->stream);
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *p : h->allocs) dfree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return 0;
 }
@@ -133,8 +176,21 @@ static int validate_opts(const cvr_hip_opts *o) {
 static int plan_parity(cvr_hip_handle *h);
 
 // Core: conversion from a CSR in device memory.
+static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d_ci, const double *d_va,
+                            const cvr_hip_opts *o, cvr_hip_handle *h);
+// host containers may throw std::bad_alloc: it becomes PAVA_HIP_ENOMEM, and the
+// caller destroys the partly built handle as for any other failure
 static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, const double *d_va,
                        const cvr_hip_opts *o, cvr_hip_handle *h) {
+    try {
+        return create_core_body(M, N, nnz, d_rp, d_ci, d_va, o, h);
+    } catch (const std::bad_alloc &) {
+        return set_error(PAVA_HIP_ENOMEM, "host allocation failed while building the CVR layout");
+    }
+}
+
+static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d_ci, const double *d_va,
+                            const cvr_hip_opts *o, cvr_hip_handle *h) {
     h->opts = *o;
     h->M = M;
     h->N = N;
@@ -152,16 +208,19 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
             h->col16 = 1;
         } else {
             int *d_flag = nullptr, flag = 1;
-            hipError_t e = hipMalloc(&d_flag, 4);
+            hipError_t e = dmalloc(&d_flag, 4);
             if (e == hipSuccess) e = hipMemsetAsync(d_flag, 0, 4, h->stream);
             if (e == hipSuccess) e = cvr::launch_chunk_span(d_ci, nnz, C, h->nchunks, d_flag, h->stream);
             if (e == hipSuccess) e = hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, h->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-            (void)hipFree(d_flag);
-            if (e != hipSuccess) return set_error(PAVA_HIP_EDEVICE, "column span check: %s", hipGetErrorString(e));
+            dfree(d_flag);
+            if (e != hipSuccess) return hip_fail(e, "column span check");
             h->col16 = flag == 0;
         }
     }
+    if (!h->col16 && h->nchunks > 0 && (long long)N > kCol32Limit)
+        return set_error(PAVA_HIP_ERANGE, "a chunk spans 65,536 columns or more and there are more than 2^30 columns: "
+                                          "no CVR column layout fits");
     if (h->nchunks > 0) {
         long long nlast = nnz - (long long)(h->nchunks - 1) * C;
         long long steps_last = (nlast + 63) / 64;
@@ -284,8 +343,8 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     h->convert_ms = ms;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     int status[16];
     HIP_TRY(hipMemcpy(status, h->d_status, sizeof status, hipMemcpyDeviceToHost));
     h->max_slots = status[0];
@@ -309,7 +368,7 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
         int *cnt = nullptr, *colmap = nullptr;
         hipError_t e = hipSuccess;
         int nh = 0;
-        if (hipMalloc(&cnt, (size_t)N * 4 + 4) != hipSuccess || hipMalloc(&colmap, (size_t)N * 4 + 4) != hipSuccess ||
+        if (dmalloc(&cnt, (size_t)N * 4 + 4) != hipSuccess || dmalloc(&colmap, (size_t)N * 4 + 4) != hipSuccess ||
             dev_alloc(h, (void **)&h->d_hot_cols, (size_t)std::min<long long>(max_hot, N) * 4 + 4) != 0) {
             e = hipErrorOutOfMemory;
         } else {
@@ -317,9 +376,9 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
                                         cnt, colmap, h->d_hot_cols, (int *)h->d_cols, h->layout_elems, &nh, h->stream);
         }
         (void)hipStreamSynchronize(h->stream);
-        (void)hipFree(cnt);
-        (void)hipFree(colmap);
-        if (e != hipSuccess) return set_error(PAVA_HIP_EDEVICE, "hot-column staging failed: %s", hipGetErrorString(e));
+        dfree(cnt);
+        dfree(colmap);
+        if (e != hipSuccess) return hip_fail(e, "hot-column staging");
         h->n_hot = nh;
         if (nh > 0) {
             int rc2 = dev_alloc(h, &h->d_xhot, (size_t)nh * xsize);
@@ -350,13 +409,13 @@ static int plan_parity(cvr_hip_handle *h) {
     const int nc = h->nchunks;
     std::vector<int> fix((size_t)nc * 2), len((size_t)nc);
     int *d_len = nullptr;
-    if (hipMalloc(&d_len, (size_t)nc * 4) != hipSuccess) return set_error(PAVA_HIP_ENOMEM, "parity plan: hipMalloc failed");
+    if (dmalloc(&d_len, (size_t)nc * 4) != hipSuccess) return set_error(PAVA_HIP_ENOMEM, "parity plan: hipMalloc failed");
     hipError_t e = cvr::launch_closer_len(h->d_fix, h->d_rp, d_len, nc, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(len.data(), d_len, (size_t)nc * 4, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(fix.data(), h->d_fix, (size_t)nc * 8, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    (void)hipFree(d_len);
-    if (e != hipSuccess) return set_error(PAVA_HIP_EDEVICE, "parity plan: %s", hipGetErrorString(e));
+    dfree(d_len);
+    if (e != hipSuccess) return hip_fail(e, "parity plan");
     std::vector<int> wg_of((size_t)nc);
     int rc;
     if (h->n_lds > 0) {
@@ -460,7 +519,8 @@ static int begin_handle(const cvr_hip_opts *opts, long long nnz, cvr_hip_opts *o
     if (o->device < 0 || o->device >= ndev) return set_error(PAVA_HIP_EINVAL, "device %d out of range", o->device);
     HIP_TRY(hipSetDevice(o->device));
     if (o->steps_per_chunk == 0) o->steps_per_chunk = auto_steps(nnz, o->device);
-    cvr_hip_handle *h = new cvr_hip_handle();
+    cvr_hip_handle *h = new (std::nothrow) cvr_hip_handle();
+    if (!h) return set_error(PAVA_HIP_ENOMEM, "host allocation of the handle failed");
     h->device = o->device;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
@@ -473,6 +533,7 @@ static int begin_handle(const cvr_hip_opts *opts, long long nnz, cvr_hip_opts *o
 extern "C" int cvr_hip_create(const pava_csr_matrix *A, const cvr_hip_opts *opts, cvr_hip_handle **out) {
     int rc = 0;
     if (A && (rc = check_gpu_sizes(A->numRows, A->numCols, A->nnz))) return rc;
+    if (A && (rc = check_col_limit(opts, A->numCols))) return rc;
     if ((rc = pava_csr_validate(A))) return rc;
     cvr_hip_opts o;
     cvr_hip_handle *h = nullptr;
@@ -483,9 +544,9 @@ extern "C" int cvr_hip_create(const pava_csr_matrix *A, const cvr_hip_opts *opts
     const bool keep = o.keep_fp64 || o.recompute || (o.precision == CVR_HIP_PREC_F64 && o.parity_tol > 0.0);
     auto fail = [&](int code) {
         if (!keep) {
-            hipFree(d_rp);
-            hipFree(d_ci);
-            hipFree(d_va);
+            dfree(d_rp);
+            dfree(d_ci);
+            dfree(d_va);
         }
         cvr_hip_destroy(h);
         return code;
@@ -495,8 +556,8 @@ extern "C" int cvr_hip_create(const pava_csr_matrix *A, const cvr_hip_opts *opts
             (rc = dev_alloc(h, (void **)&d_va, (size_t)nnz * 8)))
             return fail(rc);
     } else {
-        if (hipMalloc(&d_rp, (size_t)(M + 1) * 4 + 4) != hipSuccess || hipMalloc(&d_ci, (size_t)nnz * 4 + 4) != hipSuccess ||
-            hipMalloc(&d_va, (size_t)nnz * 8 + 8) != hipSuccess)
+        if (dmalloc(&d_rp, (size_t)(M + 1) * 4 + 4) != hipSuccess || dmalloc(&d_ci, (size_t)nnz * 4 + 4) != hipSuccess ||
+            dmalloc(&d_va, (size_t)nnz * 8 + 8) != hipSuccess)
             return fail(set_error(PAVA_HIP_ENOMEM, "hipMalloc of the CSR upload failed"));
     }
     if (hipMemcpy(d_rp, A->rowOffsets, (size_t)(M + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
@@ -511,9 +572,9 @@ extern "C" int cvr_hip_create(const pava_csr_matrix *A, const cvr_hip_opts *opts
     rc = create_core(M, A->numCols, nnz, d_rp, d_ci, d_va, &o, h);
     if (rc) return fail(rc);
     if (!keep) {
-        hipFree(d_rp);
-        hipFree(d_ci);
-        hipFree(d_va);
+        dfree(d_rp);
+        dfree(d_ci);
+        dfree(d_va);
     }
     *out = h;
     return 0;
@@ -525,21 +586,40 @@ extern "C" int cvr_hip_create_device(int numRows, int numCols, int nnz, const in
     if (!d_rowOffsets || (nnz > 0 && (!d_cols || !d_vals))) return set_error(PAVA_HIP_EINVAL, "null device CSR");
     int rc = check_gpu_sizes(numRows, numCols, nnz);
     if (rc) return rc;
+    if ((rc = check_col_limit(opts, numCols))) return rc;
     cvr_hip_opts o;
     cvr_hip_handle *h = nullptr;
     if ((rc = begin_handle(opts, nnz, &o, out, &h))) return rc;
+    auto fail = [&](int code) {
+        cvr_hip_destroy(h);
+        return code;
+    };
+    // the CSR may still be written by work the caller enqueued on any stream:
+    // the conversion, on the handle's own stream, starts after all of it
+    if (hipDeviceSynchronize() != hipSuccess)
+        return fail(set_error(PAVA_HIP_EDEVICE, "work pending on the device failed: %s",
+                              hipGetErrorString(hipGetLastError())));
+    {
+        int *d_bad = nullptr, bad = 0;
+        hipError_t e = dmalloc(&d_bad, 4);
+        if (e == hipSuccess) e = hipMemsetAsync(d_bad, 0, 4, h->stream);
+        if (e == hipSuccess) e = cvr::launch_validate_csr(d_rowOffsets, d_cols, numRows, numCols, nnz, d_bad, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        dfree(d_bad);
+        if (e != hipSuccess) return fail(hip_fail(e, "device CSR check"));
+        if (bad & 1) return fail(set_error(PAVA_HIP_EINVAL, "device CSR: row offsets not 0..nnz and non-decreasing"));
+        if (bad & 2) return fail(set_error(PAVA_HIP_ERANGE, "device CSR: a column is outside [0, %d)", numCols));
+    }
     if (o.recompute || o.keep_fp64 || (o.precision == CVR_HIP_PREC_F64 && o.parity_tol > 0.0)) {
         // keep a private copy of the FP64 CSR for the recompute pass
         if ((rc = dev_alloc(h, (void **)&h->d_rp, (size_t)(numRows + 1) * 4)) ||
-            (rc = dev_alloc(h, (void **)&h->d_ci, (size_t)nnz * 4)) || (rc = dev_alloc(h, (void **)&h->d_va, (size_t)nnz * 8))) {
-            cvr_hip_destroy(h);
-            return rc;
-        }
-        hipMemcpyAsync(h->d_rp, d_rowOffsets, (size_t)(numRows + 1) * 4, hipMemcpyDeviceToDevice, h->stream);
-        if (nnz) {
-            hipMemcpyAsync(h->d_ci, d_cols, (size_t)nnz * 4, hipMemcpyDeviceToDevice, h->stream);
-            hipMemcpyAsync(h->d_va, d_vals, (size_t)nnz * 8, hipMemcpyDeviceToDevice, h->stream);
-        }
+            (rc = dev_alloc(h, (void **)&h->d_ci, (size_t)nnz * 4)) || (rc = dev_alloc(h, (void **)&h->d_va, (size_t)nnz * 8)))
+            return fail(rc);
+        hipError_t e = hipMemcpyAsync(h->d_rp, d_rowOffsets, (size_t)(numRows + 1) * 4, hipMemcpyDeviceToDevice, h->stream);
+        if (e == hipSuccess && nnz) e = hipMemcpyAsync(h->d_ci, d_cols, (size_t)nnz * 4, hipMemcpyDeviceToDevice, h->stream);
+        if (e == hipSuccess && nnz) e = hipMemcpyAsync(h->d_va, d_vals, (size_t)nnz * 8, hipMemcpyDeviceToDevice, h->stream);
+        if (e != hipSuccess) return fail(set_error(PAVA_HIP_EDEVICE, "copy of the device CSR failed: %s", hipGetErrorString(e)));
     }
     rc = create_core(numRows, numCols, nnz, d_rowOffsets, d_cols, d_vals, &o, h);
     if (rc) {
@@ -779,11 +859,32 @@ struct cvr_hip_batch {
     long long nnz = 0;
 };
 
+static int batch_create_body(int nmat, cvr_hip_handle *const *handles, int k, cvr_hip_batch **out);
 extern "C" int cvr_hip_batch_create(int nmat, cvr_hip_handle *const *handles, int k, cvr_hip_batch **out) {
+    try {
+        return batch_create_body(nmat, handles, k, out);
+    } catch (const std::bad_alloc &) {
+        return set_error(PAVA_HIP_ENOMEM, "host allocation failed while building the batch");
+    }
+}
+
+static int batch_create_body(int nmat, cvr_hip_handle *const *handles, int k, cvr_hip_batch **out) {
     if (!out || nmat <= 0 || !handles) return set_error(PAVA_HIP_EINVAL, "bad batch arguments");
     *out = nullptr;
     if (k < 1 || k > 32) return set_error(PAVA_HIP_EINVAL, "k must be in [1, 32]");
-    cvr_hip_batch *b = new cvr_hip_batch();
+    // every entry's cross-chunk carries live in its handle: a handle may appear
+    // once per batch
+    {
+        std::vector<const cvr_hip_handle *> seen(handles, handles + nmat);
+        for (const cvr_hip_handle *x : seen)
+            if (!x) return set_error(PAVA_HIP_EINVAL, "null handle in a batch");
+        std::sort(seen.begin(), seen.end());
+        if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+            return set_error(PAVA_HIP_EINVAL, "a handle appears more than once in a batch");
+    }
+    cvr_hip_batch *b = new (std::nothrow) cvr_hip_batch();
+    if (!b) return set_error(PAVA_HIP_ENOMEM, "host allocation of the batch failed");
+    std::unique_ptr<cvr_hip_batch> guard(b);  // released once the batch is complete
     b->nmat = nmat;
     b->k = k;
     b->mats.assign(handles, handles + nmat);
@@ -792,26 +893,24 @@ extern "C" int cvr_hip_batch_create(int nmat, cvr_hip_handle *const *handles, in
     std::vector<int2> queue;
     for (int i = 0; i < nmat; ++i) {
         cvr_hip_handle *h = handles[i];
-        if (!h || h->device != b->device || h->prec != b->prec || h->nchunks == 0) {
-            delete b;
+        if (h->device != b->device || h->prec != b->prec || h->nchunks == 0)
             return set_error(PAVA_HIP_EINVAL, "batch matrices must be non-empty and share device and precision");
-        }
         b->max_slots = std::max(b->max_slots, h->max_slots);
         b->nnz += h->nnz;
         for (int c = 0; c < h->nchunks; ++c) queue.push_back(make_int2(i, c));
     }
     b->total = (int)queue.size();
-    auto fail = [&](int code) {
-        if (b->d_descs) hipFree(b->d_descs);
-        if (b->d_queue) hipFree(b->d_queue);
-        if (b->d_counter) hipFree(b->d_counter);
-        delete b;
+    b->descs.resize(nmat);
+    auto fail = [&](int code) {  // the guard deletes b
+        dfree(b->d_descs);
+        dfree(b->d_queue);
+        dfree(b->d_counter);
         return code;
     };
     if (hipSetDevice(b->device) != hipSuccess) return fail(set_error(PAVA_HIP_EDEVICE, "hipSetDevice failed"));
-    if (hipMalloc(&b->d_descs, sizeof(cvr::SpmmDesc) * nmat) != hipSuccess ||
-        hipMalloc(&b->d_queue, sizeof(int2) * std::max(b->total, 1)) != hipSuccess ||
-        hipMalloc(&b->d_counter, 256) != hipSuccess)
+    if (dmalloc(&b->d_descs, sizeof(cvr::SpmmDesc) * nmat) != hipSuccess ||
+        dmalloc(&b->d_queue, sizeof(int2) * std::max(b->total, 1)) != hipSuccess ||
+        dmalloc(&b->d_counter, 256) != hipSuccess)
         return fail(set_error(PAVA_HIP_ENOMEM, "batch allocation failed"));
     if (hipMemcpy(b->d_queue, queue.data(), sizeof(int2) * queue.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(b->d_counter, 0, 256) != hipSuccess)
@@ -819,8 +918,7 @@ extern "C" int cvr_hip_batch_create(int nmat, cvr_hip_handle *const *handles, in
     b->lds = cvr::spmm_lds_bytes(b->max_slots, k);
     if (cvr::spmm_batch_grid(b->prec, k, b->lds, &b->grid) != hipSuccess)
         return fail(set_error(PAVA_HIP_EDEVICE, "occupancy query failed"));
-    b->descs.resize(nmat);
-    *out = b;
+    *out = guard.release();
     return 0;
 }
 
@@ -850,11 +948,11 @@ extern "C" int cvr_hip_batch_graph(cvr_hip_batch *b, void *stream) {
     if (!b || !stream) return set_error(PAVA_HIP_EINVAL, "graph capture needs a non-null stream");
     HIP_TRY(hipSetDevice(b->device));
     if (b->exec) {
-        hipGraphExecDestroy(b->exec);
+        (void)hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
     }
     if (b->graph) {
-        hipGraphDestroy(b->graph);
+        (void)hipGraphDestroy(b->graph);
         b->graph = nullptr;
     }
     hipStream_t st = (hipStream_t)stream;
@@ -895,13 +993,13 @@ extern "C" int cvr_hip_batch_queue_counts(const cvr_hip_batch *b, int *counts, i
 
 extern "C" int cvr_hip_batch_destroy(cvr_hip_batch *b) {
     if (!b) return 0;
-    hipSetDevice(b->device);
-    hipDeviceSynchronize();
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    hipFree(b->d_descs);
-    hipFree(b->d_queue);
-    hipFree(b->d_counter);
+    (void)hipSetDevice(b->device);
+    (void)hipDeviceSynchronize();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    dfree(b->d_descs);
+    dfree(b->d_queue);
+    dfree(b->d_counter);
     delete b;
     return 0;
 }

@@ -232,6 +232,8 @@ hipError_t launch_spmm_batch(const SpmmDesc *d_descs, const int2 *d_queue, int t
                              size_t lds, int grid, hipStream_t st);
 
 hipError_t launch_empty_rows(const int *rp, int M, unsigned *bits, int *status, hipStream_t st);
+// *bad |= 1 if the device CSR's offsets are not 0 .. nnz and monotone, |= 2 if a column is outside [0, N)
+hipError_t launch_validate_csr(const int *rp, const int *ci, int M, int N, long long nnz, int *bad, hipStream_t st);
 // *flag |= 1 if a chunk of C elements of ci spans 65,536 columns or more (flag cleared by the caller)
 hipError_t launch_chunk_span(const int *ci, long long nnz, long long C, int nchunks, int *flag, hipStream_t st);
 hipError_t launch_convert(const ConvertArgs &a, hipStream_t st);

@@ -123,6 +123,7 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
         for (int k = 0; k < K; ++k) xo[k] = x[(unsigned)c[k]];
     };
     // prologue: rounds 0 .. D gathered, round D + 1 columns loading
+    static_assert(D >= 1 && D <= 3, "pipeline depth");
 #pragma unroll
     for (int d = 0; d <= D; ++d) {
         load_cv(b + d * R, cn, vq[d]);
@@ -139,7 +140,6 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
             vv[k] = in ? vq[0][k] : 0.0;
             xx[k] = in ? xq[0][k] : 0.0;
         }
-#pragma unroll 4
         for (int l = 0; l < nl64; ++l) {
             double t = __builtin_fma(vv[0], xx[0], s);
 #pragma unroll

@@ -49,7 +49,20 @@ int set_error(int code, const char *fmt, ...) {
     return code;
 }
 
+// Allocation-failure injection (pava_hip_test_fail_alloc): the n-th host
+// allocation through aligned_alloc64, or the n-th device allocation of the
+// CVR handle code, counted from the call, fails as if memory had run out.
+static std::atomic<int> g_fail_host{0}, g_fail_device{0};
+static bool take_failure(std::atomic<int> &ctr) {
+    int v = ctr.load();
+    while (v > 0 && !ctr.compare_exchange_weak(v, v - 1)) {
+    }
+    return v == 1;
+}
+bool inject_device_failure() { return take_failure(g_fail_device); }
+
 void *aligned_alloc64(size_t bytes) {
+    if (take_failure(g_fail_host)) return nullptr;
     if (bytes == 0) bytes = 64;
     void *p = nullptr;
     if (posix_memalign(&p, 64, (bytes + 63) / 64 * 64) != 0) return nullptr;
@@ -61,6 +74,11 @@ void *aligned_alloc64(size_t bytes) {
 using namespace pava;
 
 extern "C" const char *pava_hip_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" void pava_hip_test_fail_alloc(int host_nth, int device_nth) {
+    g_fail_host.store(host_nth > 0 ? host_nth : 0);
+    g_fail_device.store(device_nth > 0 ? device_nth : 0);
+}
 extern "C" const char *pava_hip_version(void) { return "pava_hip 0.1 (gfx950, CVR-64)"; }
 
 // ---------------------------------------------------------------- MM reader

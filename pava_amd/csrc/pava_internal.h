@@ -14,6 +14,8 @@ namespace pava {
 extern thread_local std::string g_last_error;
 int set_error(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void *aligned_alloc64(size_t bytes);
+// true if this device allocation is to fail (pava_hip_test_fail_alloc)
+bool inject_device_failure();
 // PAVA_HIP_ERANGE if a dimension or nnz exceeds PAVA_HIP_MAX_SIZE
 int check_gpu_sizes(long long M, long long N, long long nnz);
 

@@ -129,6 +129,46 @@ int main(int argc, char **argv) {
             pava_delete_csr_matrix(&G);
         }
     }
+    // allocation failures (readCOOMatrix's -5, utils/basicMatrix.cpp:84-132):
+    // the n-th host allocation fails; every call reports it and, under the
+    // leak checker, frees whatever it had taken
+    {
+        const std::string okp = write(dir, "ok2.mtx", ok);
+        int reader_fails = 0;
+        for (int n = 1; n <= 8; ++n) {
+            pava_coo_matrix Co{};
+            pava_hip_test_fail_alloc(n, 0);
+            const int rc = pava_read_coo_matrix(okp.c_str(), &Co);
+            pava_hip_test_fail_alloc(0, 0);
+            if (rc == 0) {
+                pava_delete_coo_matrix(&Co);
+                break;
+            }
+            CHECK(rc == PAVA_MM_ENOMEM);
+            CHECK(Co.vals == nullptr && Co.cols == nullptr && Co.rows == nullptr);
+            ++reader_fails;
+        }
+        CHECK(reader_fails >= 1);
+        pava_coo_matrix Co{};
+        CHECK(pava_read_coo_matrix(okp.c_str(), &Co) == 0);
+        int convert_fails = 0;
+        for (int n = 1; n <= 8; ++n) {
+            pava_csr_matrix Cs{};
+            pava_hip_test_fail_alloc(n, 0);
+            const int rc = pava_convert_coo_to_csr(&Co, &Cs);
+            pava_hip_test_fail_alloc(0, 0);
+            if (rc == 0) {
+                pava_delete_csr_matrix(&Cs);
+                break;
+            }
+            CHECK(rc == PAVA_HIP_ENOMEM);
+            CHECK(Cs.vals == nullptr && Cs.cols == nullptr && Cs.rowOffsets == nullptr);
+            ++convert_fails;
+        }
+        CHECK(convert_fails >= 1);
+        pava_delete_coo_matrix(&Co);
+        std::printf("allocation failures: reader %d, convert %d\n", reader_fails, convert_fails);
+    }
     std::printf("host harness: %d failures\n", fails);
     return fails != 0;
 }

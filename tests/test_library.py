@@ -14,7 +14,7 @@ ROOT = Path(__file__).resolve().parent.parent
 def declared_functions():
     text = (ROOT / "include" / "pava_hip.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*([a-z_0-9]+)\s*\(", text, flags=re.M)
+    names = re.findall(r"^\s*(?:const\s+)?(?:long\s+)?[a-z_0-9]+\s*\*?\s*([a-z_0-9]+)\s*\(", text, flags=re.M)
     return sorted(set(names))
 
 
@@ -106,3 +106,28 @@ def test_sizes_beyond_gpu_limit_rejected_before_device(P):
             assert call() == -6, (name, M, N, nnz)
             assert "limit" in L.pava_hip_last_error().decode(), name
             assert not h.value, name
+
+
+def test_columns_beyond_2_30_rejected_for_32bit_layouts(P):
+    """32-bit CVR layouts keep flags in bits 31 and 30 of a column: numCols
+    above 2^30 is PAVA_HIP_ERANGE before anything is read or a device is used."""
+    L = P.lib()
+    V = C.c_void_p
+    rp = np.zeros(2, np.int32)
+    one = np.zeros(1, np.int32)
+    val = np.zeros(1)
+    N = 2 ** 30 + 1
+    s = P._CSR()
+    s.numRows, s.numCols, s.nnz = 1, N, 0
+    s.rowOffsets = rp.ctypes.data_as(C.POINTER(C.c_int))
+    s.cols = one.ctypes.data_as(C.POINTER(C.c_int))
+    s.vals = val.ctypes.data_as(C.POINTER(C.c_double))
+    o = P._Opts()
+    L.cvr_hip_default_opts(C.byref(o))
+    o.col_index_bits = 32
+    h = V()
+    assert L.cvr_hip_create(C.byref(s), C.byref(o), C.byref(h)) == -6
+    assert "2^30" in L.pava_hip_last_error().decode()
+    assert L.cvr_hip_create_device(1, N, 0, rp.ctypes.data, one.ctypes.data, val.ctypes.data, C.byref(o),
+                                   C.byref(h)) == -6
+    assert not h.value

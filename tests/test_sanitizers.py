@@ -10,7 +10,10 @@ Two harnesses under tests/sanitize/ are compiled with gcc/g++
   Market reader, COO->CSR, binary cache, writer, generators, features) on
   valid files (duplicates, symmetric/pattern, CRLF, no final newline, empty
   matrix, a file above the parallel-parse threshold) and on malformed or
-  truncated ones, each of which must be rejected with an error.
+  truncated ones, each of which must be rejected with an error; and with the
+  n-th host allocation failing (pava_hip_test_fail_alloc), where the reader
+  returns PAVA_MM_ENOMEM (-5, as readCOOMatrix, utils/basicMatrix.cpp:84-132)
+  and the converter PAVA_HIP_ENOMEM, with nothing leaked (LeakSanitizer).
 
 A planted out-of-bounds read is compiled the same way first, to show that
 the sanitizer is active in this toolchain (a harness that "passes" with the
