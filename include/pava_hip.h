@@ -246,6 +246,17 @@ int cvr_hip_spmv_mixed(cvr_hip_handle *h, const float *d_x32, const double *d_x6
  * make one call before capturing the call in a graph. cvr_hip_spmv and
  * cvr_hip_spmv_mixed enqueue kernels only and can be captured at any time. */
 int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *stream);
+/* cvr_hip_spmm without any allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). Same kernels,
+ * arguments and results (bit for bit) as cvr_hip_spmm, but the cross-chunk
+ * partial sums go to a 16 MiB scratch array that the library holds on every
+ * GPU it runs on until the process ends, instead of a buffer the handle
+ * allocates on its first SpMM; so it may be captured in a graph at any time.
+ * Needs 2 * nchunks * k * 8 bytes <= 16 MiB (PAVA_HIP_ERANGE otherwise, before
+ * any device work). All calls on one GPU share the array: calls that overlap
+ * on one GPU (several streams, host threads or graphs in flight) race. */
+int cvr_hip_spmm_noalloc(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *stream);
 /* Batched multi-RHS SpMM: the chunks of many matrices (same device and
  * precision, 16- or 32-bit columns, nnz > 0) form one work queue consumed by a
  * persistent grid sized to the GPU's resident capacity; a fixup kernel closes

@@ -86,7 +86,7 @@ EXPORTED = [
     "pava_read_csr_binary", "conduct_csr_hip", "csr5_hip_create", "csr5_hip_spmv", "csr5_hip_spmv_host",
     "csr5_hip_get_info", "csr5_hip_export_layout", "csr5_hip_destroy", "conduct_csr5_hip", "coo_hip_create",
     "coo_hip_spmv", "coo_hip_get_info", "coo_hip_destroy", "conduct_coo_hip", "cvr_hip_batch_queue_counts",
-    "pava_hip_test_fail_alloc", "pava_hip_test_live_device_allocs",
+    "pava_hip_test_fail_alloc", "pava_hip_test_live_device_allocs", "cvr_hip_spmm_noalloc",
 ]
 
 
@@ -137,6 +137,7 @@ def lib() -> C.CDLL:
     L.cvr_hip_spmv_host.argtypes = [P, P, P]
     L.cvr_hip_spmv_mixed.argtypes = [P, P, P, P, P]
     L.cvr_hip_spmm.argtypes = [P, i, P, P, P]
+    L.cvr_hip_spmm_noalloc.argtypes = [P, i, P, P, P]
     L.cvr_hip_last_recompute_count.argtypes = [P, C.POINTER(i)]
     L.cvr_hip_time_spmv.argtypes = [P, P, P, i, i, C.POINTER(d), C.POINTER(d)]
     L.cvr_hip_time_spmv_x64.argtypes = [P, P, P, P, i, i, C.POINTER(d), C.POINTER(d)]

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <atomic>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <chrono>
 #include <cstdio>
@@ -795,16 +796,13 @@ extern "C" int cvr_hip_spmv_host(cvr_hip_handle *h, const double *x, double *y) 
     return 0;
 }
 
-static int spmm_desc(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, cvr::SpmmDesc *a) {
+// The SpMM descriptor of a handle, its cross-chunk buffer (2 * nchunks * k
+// doubles) taken from `carry`; allocates nothing.
+static int spmm_desc_fill(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, double *carry, cvr::SpmmDesc *a) {
     if (k < 1 || k > 32) return set_error(PAVA_HIP_EINVAL, "k must be in [1, 32], got %d", k);
     if ((!d_X && h->N > 0) || (!d_Y && h->M > 0)) return set_error(PAVA_HIP_EINVAL, "null X or Y");
     if (k == cvr::spmm_kp(k) && (((uintptr_t)d_X | (uintptr_t)d_Y) & 15))
         return set_error(PAVA_HIP_EINVAL, "X and Y must be 16-byte aligned");
-    const size_t need = (size_t)h->nchunks * 2 * 32 * 8 + 64;
-    if (!h->d_carry_mm && h->nchunks > 0) {  // allocated by the first call (none for an empty matrix)
-        int rc = dev_alloc(h, (void **)&h->d_carry_mm, need);
-        if (rc) return rc;
-    }
     *a = cvr::SpmmDesc{};
     a->vals = h->d_vals;
     a->cols = (const int *)h->d_cols;
@@ -820,7 +818,7 @@ static int spmm_desc(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, cvr
     a->col16 = h->col16;
     a->X = d_X;
     a->Y = d_Y;
-    a->carry = h->d_carry_mm;
+    a->carry = carry;
     a->layout_elems = h->layout_elems;
     a->nchunks = h->nchunks;
     a->S = h->S;
@@ -830,6 +828,18 @@ static int spmm_desc(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, cvr
     return 0;
 }
 
+// As spmm_desc_fill, with the handle's own cross-chunk buffer (allocated by
+// the first call; none for an empty matrix).
+static int spmm_desc(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, cvr::SpmmDesc *a) {
+    if (k < 1 || k > 32) return set_error(PAVA_HIP_EINVAL, "k must be in [1, 32], got %d", k);
+    const size_t need = (size_t)h->nchunks * 2 * 32 * 8 + 64;
+    if (!h->d_carry_mm && h->nchunks > 0) {
+        int rc = dev_alloc(h, (void **)&h->d_carry_mm, need);
+        if (rc) return rc;
+    }
+    return spmm_desc_fill(h, k, d_X, d_Y, h->d_carry_mm, a);
+}
+
 
 extern "C" int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *stream) {
     if (!h) return set_error(PAVA_HIP_EINVAL, "null handle");
@@ -837,6 +847,61 @@ extern "C" int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d
     cvr::SpmmDesc a;
     int rc = spmm_desc(h, k, d_X, d_Y, &a);
     if (rc) return rc;
+    if (h->nchunks == 0) {
+        HIP_TRY(cvr::launch_zero_f32(d_Y, (long long)h->M * k, (hipStream_t)stream));
+        return 0;
+    }
+    HIP_TRY(cvr::launch_spmm(a, h->prec, h->max_slots, (hipStream_t)stream));
+    return 0;
+}
+
+// ------------------------------------------------ SpMM without allocation
+// cvr_hip_spmm_noalloc: the same kernels as cvr_hip_spmm, with the cross-chunk
+// buffer in `scratch`, a file-scope device array each GPU holds from module
+// load until the process ends. Its device address is looked up once per
+// device (hipGetSymbolAddress, with that device current) and kept in
+// g_scratch_ptr; host code passes that address, never the array's name. Every
+// call on a GPU shares the array: calls that overlap on one GPU race. Each
+// entry read by a call is written earlier in the same call (the head and tail
+// partials of every chunk of a row spanning chunks), so nothing needs
+// clearing.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;  // 16 MiB: 2 * nchunks * k doubles
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices];
+
+// the scratch address on device `dev` (current), looked up on first use
+static int scratch_for(int dev, double **out) {
+    if (dev < 0 || dev >= kMaxDevices) return set_error(PAVA_HIP_EINVAL, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(PAVA_HIP_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                             hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+extern "C" int cvr_hip_spmm_noalloc(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *stream) {
+    if (!h) return set_error(PAVA_HIP_EINVAL, "null handle");
+    if (k < 1 || k > 32) return set_error(PAVA_HIP_EINVAL, "k must be in [1, 32], got %d", k);
+    const size_t need = (size_t)h->nchunks * 2 * (size_t)k * sizeof(double);
+    if (need > kNoAllocScratchBytes)
+        return set_error(PAVA_HIP_ERANGE,
+                         "cvr_hip_spmm_noalloc: %d chunks x %d right-hand sides need %zu bytes of scratch, more than "
+                         "its %zu (use cvr_hip_spmm, or more steps per chunk)",
+                         h->nchunks, k, need, kNoAllocScratchBytes);
+    HIP_TRY(hipSetDevice(h->device));
+    double *scr = nullptr;
+    int rc = scratch_for(h->device, &scr);
+    if (rc) return rc;
+    cvr::SpmmDesc a;
+    if ((rc = spmm_desc_fill(h, k, d_X, d_Y, scr, &a))) return rc;
     if (h->nchunks == 0) {
         HIP_TRY(cvr::launch_zero_f32(d_Y, (long long)h->M * k, (hipStream_t)stream));
         return 0;
