@@ -106,6 +106,20 @@ def _cpu_model() -> str:
     return "unknown CPU"
 
 
+def _physical_cores():
+    """Distinct (package, core) pairs in /proc/cpuinfo; None where it lists none."""
+    seen, pkg = set(), None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("physical id"):
+                pkg = line.split(":", 1)[1].strip()
+            elif line.startswith("core id"):
+                seen.add((pkg, line.split(":", 1)[1].strip()))
+    except OSError:
+        return None
+    return len(seen) or None
+
+
 def cpu_baseline(O, A, x, budget_s: float = 12.0):
     """Oracle CSR SpMV (FP64, OpenMP over rows) on the host: a short sweep of
     thread counts (P/4, P/2, P, 2P, P = this process's CPU share), then a
@@ -140,10 +154,14 @@ def cpu_baseline(O, A, x, budget_s: float = 12.0):
         "value": algorithmic_bytes(A.numRows, A.numCols, A.nnz) / per / 1e9,
         "unit": "GB/s",
         "cores": threads,
-        "kind": "port",
+        "kind": "stand-in: project OpenMP CSR (pava CVR/CSR5 not built)",
+        "threads": threads,
+        "physical_cores": _physical_cores(),
+        "nproc": len(os.sched_getaffinity(0)),
+        "cpu_model": _cpu_model(),
         "sample": f"{n} full FP64 CSR SpMVs of the same matrix ({el:.1f} s) at the fastest of threads {{{sw}}} "
                   f"ms/SpMV, oracle orc_csr_spmv_omp, {per * 1e3:.2f} ms each, {A.nnz / per:.3e} nnz/s; "
-                  f"host {_cpu_model()}, {os.cpu_count()} logical CPUs visible",
+                  f"{os.cpu_count()} logical CPUs on the host",
     }
 
 
@@ -196,79 +214,211 @@ def run_precision(P, A, x, steps: int, S: int = 0) -> dict:
     return out
 
 
-def run_batch(P, k: int, nmat: int, steps: int, S: int = 64, check: int = 4) -> dict:
-    """nmat random matrices (rows log-uniform in [1e3, 2e5], Poisson row
-    lengths with mean U[5, 60], 70% of columns within +-1000 of the diagonal,
-    FP32 N(0,1) values) times FP32 RHS blocks of k columns, in one persistent
-    launch replayed from a hipGraph; compared with plain FP64 CVR-HIP SpMVs of
-    the same matrices (one launch each)."""
+def _time_calls(fn, iters: int, stream) -> float:
+    """ms per call of fn() (launches on `stream`), hipEvent-timed after 3 warm calls."""
     import torch
-    rng = np.random.default_rng(4)
-    dev = torch.device("cuda:0")
-    mats, hs, h64 = [], [], []
-    t0 = time.perf_counter()
-    for i in range(nmat):
-        n = int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
-        A = P.synthetic(n, P.RL_POISSON, float(rng.uniform(5, 60)), col_struct=P.COL_MIXED, half_band=1000,
-                        band_frac=0.7, value_dist=0, seed=10_000 + i)
-        A.vals = A.vals.astype(np.float32).astype(np.float64)
-        mats.append(A)
-        hs.append(P.CvrHip(A, steps_per_chunk=S, precision=P.PREC_F32))
-        h64.append(P.CvrHip(A))
-        if i % 64 == 63:
-            print(f"[bench] batch: {i + 1}/{nmat} matrices built ({time.perf_counter() - t0:.0f}s)", file=sys.stderr,
-                  flush=True)
-    nnz = sum(A.nnz for A in mats)
-    Xs = [torch.rand((A.numCols, k), dtype=torch.float32, device=dev) * 2 - 1 for A in mats]
-    Ys = [torch.empty((A.numRows, k), dtype=torch.float32, device=dev) for A in mats]
-    b = P.Batch(hs, k)
-    b.set_operands([t.data_ptr() for t in Xs], [t.data_ptr() for t in Ys])
-    st = torch.cuda.Stream(device=dev)
-    b.capture(st.cuda_stream)
     for _ in range(3):
-        b.replay(st.cuda_stream)
+        fn()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(st)
-    for _ in range(steps):
-        b.replay(st.cuda_stream)
-    e1.record(st)
+    e0.record(stream)
+    for _ in range(iters):
+        fn()
+    e1.record(stream)
     e1.synchronize()
-    ms_batch = e0.elapsed_time(e1) / steps
-    # plain FP64 CVR-HIP, one SpMV per matrix (x = first column of the block)
-    x64 = [X[:, 0].double().contiguous() for X in Xs]
-    y64 = [torch.empty(A.numRows, dtype=torch.float64, device=dev) for A in mats]
-    for _ in range(2):
-        for h, xx, yy in zip(h64, x64, y64):
-            h.spmv(xx.data_ptr(), yy.data_ptr(), st.cuda_stream)
-    e0.record(st)
-    reps = max(1, steps // 4)
-    for _ in range(reps):
-        for h, xx, yy in zip(h64, x64, y64):
-            h.spmv(xx.data_ptr(), yy.data_ptr(), st.cuda_stream)
-    e1.record(st)
-    e1.synchronize()
-    ms_fp64 = e0.elapsed_time(e1) / reps
-    # accuracy on a sample of matrices: FP64 reference SpMV on the same FP32 inputs
-    errs = []
-    for i in range(min(check, nmat)):
-        X = Xs[i].cpu().numpy().astype(np.float64)
-        Y = Ys[i].cpu().numpy().astype(np.float64)
-        for j in range(k):
-            errs.append(error_stats(Y[:, j], P.reference_spmv(mats[i], X[:, j])))
+    return e0.elapsed_time(e1) / iters
+
+
+def run_batch(P, k: int, nmat: int, steps: int, check: int = 8, mats=None, tile=None) -> dict:
+    """nmat matrices of the batching family times FP32 RHS blocks of k columns
+    in one persistent launch replayed from a hipGraph, with the tile (steps
+    per chunk, column width) chosen by the batched autotuner on a seeded
+    sample of 64 of the matrices; beside it the same batch at the fixed tile
+    it used before (S = 64, automatic width), and the plain FP64 CVR-HIP SpMV
+    (library defaults) of every matrix, one launch each. With `tile` given
+    (S, bits) the autotuner and the fixed-tile comparison are skipped (a
+    profiler then sees only the full batch's launches)."""
+    import torch
+    from pava_amd import selector as SEL
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    if mats is None:
+        mats = SEL.batch_family(nmat, threads=min(8, os.cpu_count() or 1))
+    t_gen = time.perf_counter() - t0
+    print(f"[bench] batch: {nmat} matrices generated ({t_gen:.0f}s)", file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    given = tile is not None
+    tune = {}
+    if not given:
+        tile, tune = SEL.autotune_batch(mats, k)
+    t_tune = time.perf_counter() - t1
+    print(f"[bench] batch k={k}: autotuned tile S={tile[0]} bits={tile[1]} ({t_tune:.0f}s)", file=sys.stderr,
+          flush=True)
+    nnz = sum(A.nnz for A in mats)
     rows = sum(A.numRows for A in mats)
     cols = sum(A.numCols for A in mats)
+    res = {"matrices": nmat, "k": k, "nnz": nnz, "rows": rows,
+           "tile": {"steps_per_chunk": tile[0], "col_index_bits": "auto" if tile[1] == 0 else tile[1]},
+           "autotune_ms_on_sample": {f"S{s}-{'auto' if b == 0 else b}": round(v, 4) for (s, b), v in tune.items()},
+           "autotune_s": round(t_tune, 1)}
+
+    def batch_ms(S, bits):
+        hs = SEL.build_batch_handles(mats, S, bits)
+        try:
+            b = P.Batch(hs, k)
+            inf = b.info()
+            b.close()
+            return SEL.time_batch(hs, mats, k, iters=max(steps, 5)), inf
+        finally:
+            for h in hs:
+                h.close()
+
+    ms_batch, binf = batch_ms(*tile)
+    ms_fixed = None if given else batch_ms(*SEL.BATCH_FIXED)[0]
+    # accuracy of the tuned batch on `check` matrices: FP64 reference SpMV on the same FP32 inputs
+    hs = SEL.build_batch_handles(mats[:check], *tile)
+    errs = []
+    try:
+        b = P.Batch(hs, k)
+        g = torch.Generator(device="cuda")
+        g.manual_seed(1)
+        Xs = [torch.rand((A.numCols, k), dtype=torch.float32, device=dev, generator=g) * 2 - 1 for A in mats[:check]]
+        Ys = [torch.empty((A.numRows, k), dtype=torch.float32, device=dev) for A in mats[:check]]
+        b.set_operands([t.data_ptr() for t in Xs], [t.data_ptr() for t in Ys])
+        b.spmm(0)
+        torch.cuda.synchronize()
+        b.close()
+        for A, Xd, Yd in zip(mats[:check], Xs, Ys):
+            X = Xd.cpu().numpy().astype(np.float64)
+            Y = Yd.cpu().numpy().astype(np.float64)
+            for j in range(k):
+                errs.append(error_stats(Y[:, j], P.reference_spmv(A, np.ascontiguousarray(X[:, j]))))
+    finally:
+        for h in hs:
+            h.close()
+    # plain FP64 CVR-HIP, library defaults, one SpMV per matrix
+    h64 = [P.CvrHip(A) for A in mats]
+    try:
+        st = torch.cuda.Stream(device=dev)
+        x64 = [torch.rand(A.numCols, dtype=torch.float64, device=dev) * 2 - 1 for A in mats]
+        y64 = [torch.empty(A.numRows, dtype=torch.float64, device=dev) for A in mats]
+
+        def all64():
+            for h, xx, yy in zip(h64, x64, y64):
+                h.spmv(xx.data_ptr(), yy.data_ptr(), st.cuda_stream)
+
+        ms_fp64 = _time_calls(all64, max(1, steps // 4), st)
+    finally:
+        for h in h64:
+            h.close()
     Bb = nnz * 8 + (rows + 1) * 4 + cols * k * 4 + rows * k * 4
-    res = {"matrices": nmat, "k": k, "nnz": nnz, "rows": rows, "chunks": b.info()["chunks"], "grid": b.info()["grid"],
-           "ms_batch": round(ms_batch, 4), "GB_s": round(Bb / ms_batch / 1e6, 1),
-           "nnz_rhs_per_s": nnz * k / (ms_batch * 1e-3),
-           "ms_fp64_spmv_all": round(ms_fp64, 4), "fp64_nnz_per_s": nnz / (ms_fp64 * 1e-3),
-           "speedup_nnz_rhs_vs_fp64_spmv": round((nnz * k / ms_batch) / (nnz / ms_fp64), 2),
-           "max_rel": max(e["max_rel"] for e in errs), "median_rel": float(np.median([e["median_rel"] for e in errs])),
-           "build_s": round(time.perf_counter() - t0, 1)}
-    b.close()
-    for h in hs + h64:
-        h.close()
+    res.update({
+        "chunks": binf["chunks"], "grid": binf["grid"],
+        "ms_batch": round(ms_batch, 4),
+        "ms_batch_fixed_tile_S64": None if ms_fixed is None else round(ms_fixed, 4),
+        "tuned_vs_fixed_tile": None if ms_fixed is None else round(ms_fixed / ms_batch, 3),
+        "GB_s": round(Bb / ms_batch / 1e6, 1), "nnz_rhs_per_s": nnz * k / (ms_batch * 1e-3),
+        "ms_fp64_spmv_all": round(ms_fp64, 4), "fp64_nnz_per_s": nnz / (ms_fp64 * 1e-3),
+        "speedup_nnz_rhs_vs_fp64_spmv": round((nnz * k / ms_batch) / (nnz / ms_fp64), 2),
+        "max_rel": max(e["max_rel"] for e in errs), "median_rel": float(np.median([e["median_rel"] for e in errs])),
+        "error_checked": f"{check} matrices x {k} columns vs the FP64 reference SpMV on the same FP32 inputs",
+        "build_s": round(time.perf_counter() - t0, 1)})
     return res
+
+
+def trained_selector():
+    """The MLP selector trained on the GPU (library kernels) on the committed
+    autotune dataset's training matrices."""
+    from pava_amd import selector as SEL
+    from pava_amd import selector_data as D
+    d = np.load(Path(SEL.__file__).parent / "data" / "selector_dataset.npz", allow_pickle=False)
+    assert [str(n) for n in d["configs"]] == [c.name for c in SEL.CONFIGS], "dataset built for another config space"
+    tr = np.asarray([D.is_train(int(i)) for i in d["index"]])
+    sel, _ = SEL.Selector.fit(d["features"][tr], d["ms"][tr], epochs=400, seed=0)
+    return sel, int(tr.sum())
+
+
+def run_mixed_spmv(P, A, x, sel, iters: int = 200) -> dict:
+    """The selector's config for matrix A (MLP over its 16 CSR features) against
+    the plain FP64 CVR-HIP default on the same matrix and x: nnz/s of each
+    (hipEvents over `iters` launches), and the chosen config's per-row error
+    against the FP64 reference SpMV (pava's serial FMA sum)."""
+    import torch
+    from pava_amd import selector as SEL
+    dev = torch.device("cuda:0")
+    F = SEL.features(A)
+    pred = sel.mlp.predict(((F - sel.mu) / sel.sd)[None, :])[0]
+    order = [int(i) for i in np.argsort(pred)]
+    yref = P.reference_spmv(A, x)
+    x64 = torch.from_numpy(x).to(dev)
+    x32 = x64.float()
+    st = torch.cuda.Stream(device=dev)
+    for ci in order:  # the best predicted config that applies to this matrix
+        cfg = SEL.CONFIGS[ci]
+        try:
+            h = SEL.build(A, cfg)
+        except P.PavaError:
+            continue
+        break
+    try:
+        if cfg.fmt in ("csr", "csr5") or cfg.prec == "f64":
+            y = torch.empty(A.numRows, dtype=torch.float64, device=dev)
+            fn = lambda: h.spmv(x64.data_ptr(), y.data_ptr(), st.cuda_stream)
+        else:
+            y = torch.empty(A.numRows, dtype=torch.float32, device=dev)
+            fn = lambda: h.spmv_mixed(x32.data_ptr(), y.data_ptr(), x64.data_ptr(), st.cuda_stream)
+        ms_sel = _time_calls(fn, iters, st)
+        fn()
+        st.synchronize()
+        err = error_stats(y.double().cpu().numpy(), yref)
+    finally:
+        h.close()
+    with P.CvrHip(A) as h0:
+        y0 = torch.empty(A.numRows, dtype=torch.float64, device=dev)
+        ms_64 = _time_calls(lambda: h0.spmv(x64.data_ptr(), y0.data_ptr(), st.cuda_stream), iters, st)
+        h0.spmv(x64.data_ptr(), y0.data_ptr(), st.cuda_stream)
+        st.synchronize()
+        err0 = error_stats(y0.cpu().numpy(), yref)
+    return {"rows": A.numRows, "nnz": A.nnz, "selected": cfg.name,
+            "selected_nnz_per_s": A.nnz / (ms_sel * 1e-3), "selected_ms": round(ms_sel, 5),
+            "selected_max_rel": err["max_rel"], "selected_median_rel": err["median_rel"],
+            "fp64_default_nnz_per_s": A.nnz / (ms_64 * 1e-3), "fp64_default_ms": round(ms_64, 5),
+            "fp64_default_max_rel": err0["max_rel"],
+            "ratio": round(ms_64 / ms_sel, 3)}
+
+
+def run_mixed_legs(P, A, x, args) -> dict:
+    """The metric's second half: the learned, mixed-precision and batched path
+    against plain FP64 CVR-HIP (the library default, FP64 serial parity) on
+    the same matrices, at the error bound 1e-5 of the FP64 reference."""
+    t0 = time.perf_counter()
+    sel, n_train = trained_selector()
+    out = {"error_bound": 1e-5,
+           "baseline": "plain FP64 CVR-HIP: library defaults (automatic chunk size, column width and hot-x; "
+                       "FP64 serial parity 1e-12), hipEvent-timed on the same matrix and x",
+           "selector": f"MLP trained on the GPU at bench start on {n_train} matrices of the committed autotune "
+                       "dataset; the best predicted config that applies"}
+    legs = {}
+    if args.workload != "stencil27_128_fp64":
+        St = P.stencil27(128, 0.01, 0)
+        legs["spmv_stencil27_128"] = run_mixed_spmv(P, St, P.fill_uniform(St.numCols, 0), sel)
+        del St
+    legs[f"spmv_{args.workload}"] = run_mixed_spmv(P, A, x, sel)
+    sel.mlp.close()
+    print(f"[bench] mixed legs (SpMV) {time.perf_counter() - t0:.0f}s", file=sys.stderr, flush=True)
+    b = run_batch(P, 8, args.nmat, 20)
+    legs["batch512_k8"] = {
+        "matrices": b["matrices"], "k": 8, "nnz": b["nnz"], "tile": b["tile"],
+        "selected_nnz_rhs_per_s": b["nnz_rhs_per_s"], "selected_max_rel": b["max_rel"],
+        "selected_median_rel": b["median_rel"], "error_checked": b["error_checked"],
+        "fp64_default_nnz_per_s": b["fp64_nnz_per_s"], "ratio": b["speedup_nnz_rhs_vs_fp64_spmv"],
+        "ms_batch": b["ms_batch"], "ms_batch_fixed_tile_S64": b["ms_batch_fixed_tile_S64"],
+        "ms_fp64_spmv_all": b["ms_fp64_spmv_all"]}
+    out["legs"] = legs
+    out["note"] = ("ratio = nonzeros (x right-hand sides) per second of the selected path over the plain FP64 "
+                   "default on the same matrices; a batch leg counts nnz x k products against one FP64 SpMV "
+                   "per matrix and right-hand side")
+    out["seconds"] = round(time.perf_counter() - t0, 1)
+    return out
 
 
 def main():
@@ -280,11 +430,16 @@ def main():
     ap.add_argument("--steps-per-chunk", type=int, default=0, help="CVR chunk steps (0 = automatic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed_vs_fp64 legs (selector-chosen SpMV on the stencil and R-MAT matrices, "
+                         "autotuned batched SpMM)")
     ap.add_argument("--variant", choices=["none", "precision", "batch", "create"], default="none",
                     help="precision: FP64/FP32/BF16 storage on the workload's matrix; batch: 512-matrix SpMM; "
                          "create: --steps handle creations from the host CSR")
     ap.add_argument("--k", type=int, default=8, help="batch variant: right-hand sides")
     ap.add_argument("--nmat", type=int, default=512, help="batch variant: matrices")
+    ap.add_argument("--batch-tile", default="",
+                    help="batch variant: fixed tile S:bits (bits 0 = automatic width) instead of the autotuner")
     ap.add_argument("--x-hot-kb", type=int, default=-1,
                     help="hot-x staging buffer (KiB; 0 = off; -1 = automatic: 2048 KiB when column use is skewed): "
                          "the most used columns read from a dense copy")
@@ -314,7 +469,8 @@ def main():
         del src, dst
 
     if args.variant == "batch":
-        res = run_batch(P, args.k, args.nmat, args.steps)
+        tile = tuple(int(v) for v in args.batch_tile.split(":")) if args.batch_tile else None
+        res = run_batch(P, args.k, args.nmat, args.steps, tile=tile)
         print(json.dumps({"variant": "batch", **res}), flush=True)
         return
 
@@ -413,7 +569,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
                      "kernel_ms": round(ms_main, 5), "kernel_ms_median": round(tst["main_median"], 5),
-                     "kernel_ms_min_max": [round(tst["main_min"], 5), round(tst["main_max"], 5)]},
+                     "kernel_ms_min_max": [round(tst["main_min"], 5), round(tst["main_max"], 5)],
+                     # a step is the hot-x gather (when x is staged) and the SpMV
+                     # kernel; events around back-to-back steps time both
+                     "step_kernels": 2 if info["n_hot_cols"] else 1,
+                     "step_kernels_ms": round(tst["ms_total"], 5)},
     }
     # the staging actually in effect (the automatic mode stages 2048 KiB or nothing)
     hot_kb = 0 if info["n_hot_cols"] == 0 else (2048 if args.x_hot_kb < 0 else args.x_hot_kb)
@@ -454,12 +614,14 @@ def main():
                 "cvr_hip": float(np.max(np.abs(y[out_rows] - ydd) / den)),
                 "serial_reference": float(np.max(np.abs(yref[out_rows] - ydd) / den))}
         result["error_vs_fp64_reference"] = st
+    h.close()
+    if not args.no_mixed:
+        result["mixed_vs_fp64"] = run_mixed_legs(P, A, x, args)
     if not args.no_cpu_baseline:
         import oracle as O
         result["cpu_baseline"] = cpu_baseline(O, A, x)
     else:
         result["cpu_baseline"] = None
-    h.close()
     print(json.dumps(result), flush=True)
 
 

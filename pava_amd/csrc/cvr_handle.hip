@@ -26,6 +26,7 @@ struct cvr_hip_handle {
     hipStream_t stream = nullptr;
     cvr_hip_opts opts{};
     int M = 0, N = 0, nnz = 0, nchunks = 0, S = 256, prec = 0, col16 = 0;
+    int auto_s = 0;  // steps per chunk chosen by the library (auto_steps)
     long long layout_elems = 0, seg_cap = 0;
     int has_empty = 0, n_fix = 0, max_slots = 0;
     void *d_vals = nullptr, *d_cols = nullptr;
@@ -39,8 +40,8 @@ struct cvr_hip_handle {
     // FP64 serial parity (opts.parity_tol > 0): bound terms of the carries
     int par = 0;
     double2 *d_carry_bnd = nullptr;
-    int *d_close_of = nullptr, *d_close_cnt = nullptr;  // rows spanning chunks closed inside the SpMV
-    int *d_row_intra = nullptr;    // per closing chunk: the row's chunks all run in one workgroup
+    int *d_close_cnt = nullptr;    // rows spanning chunks closed inside the SpMV: arrival counters
+    int4 *d_pinfo = nullptr;       // per chunk: closing and carry-scope information (SpmvArgs::pinfo)
     int *d_wg_list = nullptr, *d_wg_start = nullptr;  // LDS variant: chunks of each workgroup
     unsigned char *d_rflag = nullptr;
     int *d_rc_count = nullptr;
@@ -64,6 +65,11 @@ struct cvr_hip_handle {
     int n_hot = 0;
     int n_lds = 0, lds_blocks = 0;  // hot columns held in LDS by the SpMV, and its grid
     std::vector<void *> allocs;
+    // one device allocation holding the arrays whose sizes follow from the
+    // layout's shape, carved in order by dev_alloc (fewer hipMalloc/hipFree
+    // calls per handle); anything beyond it is allocated on its own
+    char *arena = nullptr;
+    size_t arena_size = 0, arena_used = 0;
 };
 
 #define HIP_TRY(expr)                                                                                   \
@@ -105,8 +111,17 @@ static int hip_fail(hipError_t e, const char *what) {
                      hipGetErrorString(e));
 }
 
+constexpr size_t kArenaAlign = 256;
+static size_t arena_round(size_t b) { return (std::max<size_t>(b, 1) + kArenaAlign - 1) / kArenaAlign * kArenaAlign; }
+
 static int dev_alloc(cvr_hip_handle *h, void **p, size_t bytes) {
     if (bytes == 0) bytes = 256;
+    if (h->arena && h->arena_used + arena_round(bytes) <= h->arena_size) {
+        *p = h->arena + h->arena_used;
+        h->arena_used += arena_round(bytes);
+        h->device_bytes += (long long)bytes;
+        return 0;
+    }
     hipError_t e = dmalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
@@ -147,6 +162,12 @@ static int value_size(int prec) { return prec == CVR_HIP_PREC_F64 ? 8 : prec == 
 // at least twice as often as an average used column (power-law column use;
 // not stencils or FEM meshes).
 constexpr int kAutoRounds = 4;  // automatic chunk size: chunks per resident wave slot
+// ... and with FP64 parity on a 16-bit layout: each chunk end then also counts
+// in and may close the rows spanning chunks, and fewer, longer chunks pay
+// that less often (27-point stencil 128^3: 0.140 ms at four rounds, S = 56;
+// 0.132 ms at two, S = 112)
+constexpr int kAutoRoundsParity16 = 2;
+static int auto_steps(long long nnz, int device, int rounds = kAutoRounds);
 constexpr int kAutoHotKb = 2048;
 constexpr double kAutoHotSkew = 2.0;
 
@@ -190,6 +211,38 @@ static int create_core(int M, int N, int nnz, const int *d_rp, const int *d_ci, 
     }
 }
 
+// The arena: the sizes of the layout arrays, the per-chunk tables, the
+// parity tables and the largest hot-x staging the options allow, each
+// rounded up as dev_alloc carves them.
+static int reserve_arena(cvr_hip_handle *h, const cvr_hip_opts *o) {
+    const size_t L = (size_t)h->layout_elems, nc = (size_t)h->nchunks, M = (size_t)h->M, N = (size_t)h->N;
+    auto r = arena_round;
+    size_t t = r(L * value_size(h->prec)) + r(L * (h->col16 ? 2 : 4));
+    if (h->col16) t += r(L / 64 * 8) + r(nc * 4);
+    t += r((nc + 1) * 4) + r((size_t)h->seg_cap * 4) + 2 * r(nc * 4) + r(nc * cvr::kSplitCap * 8) + r(nc * 8) +
+         r(nc * 16 + 16);
+    if (o->recompute && h->prec != CVR_HIP_PREC_F64)
+        t += r(nc * 16 + 16) + r(cvr::rflag_bytes((int)M)) + r(cvr::rc_count_len((int)M, (int)nc) * 4 + 4);
+    if (h->par)
+        t += r(nc * 2 * sizeof(double2) + 32) + r(nc * 8 + 4) + r(nc * 4) + r(nc * 2 * sizeof(int4)) + r(nc * 4) +
+             r((size_t)4096 * 4);
+    t += r((M + 31) / 32 * 4) + r(64);
+    const long long max_hot = (long long)(o->x_hot_kb < 0 ? kAutoHotKb : o->x_hot_kb) * 1024 /
+                              (h->prec == CVR_HIP_PREC_F64 ? 8 : 4);
+    if (max_hot > 0 && !h->col16) {
+        const size_t nh = (size_t)std::min<long long>(max_hot, (long long)N);
+        t += r(nh * 4 + 4) + r(nh * (h->prec == CVR_HIP_PREC_F64 ? 8 : 4));
+    }
+    void *p = nullptr;
+    if (dmalloc(&p, t) != hipSuccess)
+        return set_error(PAVA_HIP_ENOMEM, "hipMalloc(%zu) of the handle's arrays failed", t);
+    h->allocs.push_back(p);
+    h->arena = static_cast<char *>(p);
+    h->arena_size = t;
+    h->arena_used = 0;
+    return 0;
+}
+
 static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d_ci, const double *d_va,
                             const cvr_hip_opts *o, cvr_hip_handle *h) {
     h->opts = *o;
@@ -198,26 +251,45 @@ static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d
     h->nnz = nnz;
     h->S = o->steps_per_chunk;
     h->prec = o->precision;
-    const long long C = 64LL * h->S;
+    long long C = 64LL * h->S;
     h->nchunks = nnz > 0 ? (int)((nnz + C - 1) / C) : 0;
     h->col16 = (o->col_index_bits == 16);
     h->par = (h->prec == CVR_HIP_PREC_F64 && o->parity_tol > 0.0 && h->d_rp && nnz > 0) ? 1 : 0;
     if (o->col_index_bits == 0 && h->nchunks > 0 && o->x_hot_kb <= 0) {
         // automatic: 16-bit chunk-relative columns when every chunk's nonzeros
         // span fewer than 65,536 columns (an explicit hot-x request keeps 32)
-        if (N <= 65536) {
-            h->col16 = 1;
-        } else {
+        auto spans_fit = [&](long long Cs, int ncs, int *fit) -> int {
+            if (N <= 65536) {
+                *fit = 1;
+                return 0;
+            }
             int *d_flag = nullptr, flag = 1;
             hipError_t e = dmalloc(&d_flag, 4);
             if (e == hipSuccess) e = hipMemsetAsync(d_flag, 0, 4, h->stream);
-            if (e == hipSuccess) e = cvr::launch_chunk_span(d_ci, nnz, C, h->nchunks, d_flag, h->stream);
+            if (e == hipSuccess) e = cvr::launch_chunk_span(d_ci, nnz, Cs, ncs, d_flag, h->stream);
             if (e == hipSuccess) e = hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, h->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
             dfree(d_flag);
             if (e != hipSuccess) return hip_fail(e, "column span check");
-            h->col16 = flag == 0;
+            *fit = flag == 0;
+            return 0;
+        };
+        int fit = 0, rc0;
+        if (h->par && h->auto_s) {
+            // FP64 parity: the longer automatic chunks, if 16-bit columns still fit
+            const int S2 = auto_steps(nnz, h->device, kAutoRoundsParity16);
+            const long long C2 = 64LL * S2;
+            const int nc2 = (int)((nnz + C2 - 1) / C2);
+            if (S2 != h->S && (rc0 = spans_fit(C2, nc2, &fit))) return rc0;
+            if (S2 != h->S && fit) {
+                h->S = S2;
+                h->opts.steps_per_chunk = S2;
+                h->nchunks = nc2;
+            }
         }
+        if (!fit && (rc0 = spans_fit(64LL * h->S, h->nchunks, &fit))) return rc0;
+        h->col16 = fit;
+        C = 64LL * h->S;
     }
     if (!h->col16 && h->nchunks > 0 && (long long)N > kCol32Limit)
         return set_error(PAVA_HIP_ERANGE, "a chunk spans 65,536 columns or more and there are more than 2^30 columns: "
@@ -237,6 +309,7 @@ static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d
                          h->seg_cap);
     int rc;
     const int vs = value_size(h->prec);
+    if ((rc = reserve_arena(h, o))) return rc;
     if ((rc = dev_alloc(h, &h->d_vals, (size_t)h->layout_elems * vs))) return rc;
     if ((rc = dev_alloc(h, &h->d_cols, (size_t)h->layout_elems * (h->col16 ? 2 : 4)))) return rc;
     if (h->col16) {
@@ -262,9 +335,7 @@ static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d
         HIP_TRY(hipMemsetAsync(h->d_carry_bnd, 0, (size_t)h->nchunks * 2 * sizeof(double2) + 32, h->stream));
         HIP_TRY(hipMemsetAsync(h->d_rc_count, 0, (size_t)h->nchunks * 8 + 4,
                                h->stream));
-        if ((rc = dev_alloc(h, (void **)&h->d_close_of, (size_t)h->nchunks * 4))) return rc;
         if ((rc = dev_alloc(h, (void **)&h->d_close_cnt, (size_t)h->nchunks * 4))) return rc;
-        HIP_TRY(hipMemsetAsync(h->d_close_of, 0xff, (size_t)h->nchunks * 4, h->stream));
         HIP_TRY(hipMemsetAsync(h->d_close_cnt, 0, (size_t)h->nchunks * 4, h->stream));
     }
     if ((rc = dev_alloc(h, (void **)&h->d_empty, (size_t)((M + 31) / 32) * 4))) return rc;
@@ -354,10 +425,6 @@ static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d
         return set_error(PAVA_HIP_ERANGE, "a chunk spans 65536 columns or more: 16-bit column layout not possible");
     h->n_fix = status[3];
     h->has_empty = status[4] ? 1 : 0;
-    if (h->par) {
-        HIP_TRY(cvr::launch_close_of(h->d_fix, h->d_close_of, h->nchunks, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-    }
 
     // hot-x staging: renumber the most used columns into a dense copy of x
     const int xsize = h->prec == CVR_HIP_PREC_F64 ? 8 : 4;
@@ -369,8 +436,10 @@ static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d
         int *cnt = nullptr, *colmap = nullptr;
         hipError_t e = hipSuccess;
         int nh = 0;
-        if (dmalloc(&cnt, (size_t)N * 4 + 4) != hipSuccess || dmalloc(&colmap, (size_t)N * 4 + 4) != hipSuccess ||
-            dev_alloc(h, (void **)&h->d_hot_cols, (size_t)std::min<long long>(max_hot, N) * 4 + 4) != 0) {
+        // the column counts and the column map: one temporary allocation
+        const size_t half = arena_round((size_t)N * 4 + 4);
+        if (dmalloc(&cnt, 2 * half) == hipSuccess) colmap = reinterpret_cast<int *>(reinterpret_cast<char *>(cnt) + half);
+        if (!cnt || dev_alloc(h, (void **)&h->d_hot_cols, (size_t)std::min<long long>(max_hot, N) * 4 + 4) != 0) {
             e = hipErrorOutOfMemory;
         } else {
             e = cvr::launch_hot_columns(d_ci, nnz, N, (int)std::min<long long>(max_hot, N), auto_hot ? kAutoHotSkew : 0.0,
@@ -378,7 +447,6 @@ static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d
         }
         (void)hipStreamSynchronize(h->stream);
         dfree(cnt);
-        dfree(colmap);
         if (e != hipSuccess) return hip_fail(e, "hot-column staging");
         h->n_hot = nh;
         if (nh > 0) {
@@ -476,15 +544,28 @@ static int plan_parity(cvr_hip_handle *h) {
     } else {
         for (int c = 0; c < nc; ++c) wg_of[(size_t)c] = c / cvr::kWavesPerBlock;
     }
-    std::vector<int> intra((size_t)nc, 0);
-    for (int k = 0; k < nc; ++k) {
-        if (fix[2 * (size_t)k] < 0) continue;
-        int same = 1;
-        for (int c = fix[2 * (size_t)k + 1]; c < k && same; ++c) same = wg_of[(size_t)c] == wg_of[(size_t)k];
-        intra[(size_t)k] = same;
+    // per chunk: the row it closes, the chunk closing the row its tail carry
+    // belongs to, and whether each of those rows runs in one workgroup
+    std::vector<int4> pinfo((size_t)nc * 2);
+    for (int c = 0; c < nc; ++c) {
+        pinfo[2 * (size_t)c] = make_int4(fix[2 * (size_t)c], fix[2 * (size_t)c + 1], -1, 0);
+        pinfo[2 * (size_t)c + 1] = make_int4(-1, 0, 0, 0);
     }
-    if ((rc = dev_alloc(h, (void **)&h->d_row_intra, (size_t)nc * 4))) return rc;
-    HIP_TRY(hipMemcpy(h->d_row_intra, intra.data(), (size_t)nc * 4, hipMemcpyHostToDevice));
+    for (int k = 0; k < nc; ++k) {
+        const int row = fix[2 * (size_t)k], c0 = fix[2 * (size_t)k + 1];
+        if (row < 0) continue;
+        bool same = true;
+        for (int c = c0; c < k && same; ++c) same = wg_of[(size_t)c] == wg_of[(size_t)k];
+        if (same) pinfo[2 * (size_t)k].w |= cvr::kPiHeadIntra;
+        for (int c = c0; c < k; ++c) {
+            int4 &p = pinfo[2 * (size_t)c];
+            p.z = k;
+            if (same) p.w |= cvr::kPiTailIntra;
+            pinfo[2 * (size_t)c + 1] = make_int4(row, c0, 0, 0);
+        }
+    }
+    if ((rc = dev_alloc(h, (void **)&h->d_pinfo, (size_t)nc * 2 * sizeof(int4)))) return rc;
+    HIP_TRY(hipMemcpy(h->d_pinfo, pinfo.data(), (size_t)nc * 2 * sizeof(int4), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -494,11 +575,11 @@ static int plan_parity(cvr_hip_handle *h) {
 // keep it short while the per-chunk work (split rows, the row spanning chunks)
 // stays small. Measured on the 27-point stencil 128^3: 0.149 ms at two rounds
 // (112 steps), 0.140 ms at four (56); R-MAT scale 22: 0.395 -> 0.389 ms.
-static int auto_steps(long long nnz, int device) {
+static int auto_steps(long long nnz, int device, int rounds) {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
         ncu = 256;
-    const long long per = 64LL * kAutoRounds * 16 * ncu;  // elements per chunk-step over the rounds
+    const long long per = 64LL * rounds * 16 * ncu;  // elements per chunk-step over the rounds
     long long s = (nnz + per - 1) / per;
     s = (s + cvr::kStepGroup - 1) / cvr::kStepGroup * cvr::kStepGroup;
     return (int)std::min<long long>(std::max<long long>(s, cvr::kStepGroup), 256);
@@ -519,10 +600,12 @@ static int begin_handle(const cvr_hip_opts *opts, long long nnz, cvr_hip_opts *o
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(PAVA_HIP_EDEVICE, "no HIP device");
     if (o->device < 0 || o->device >= ndev) return set_error(PAVA_HIP_EINVAL, "device %d out of range", o->device);
     HIP_TRY(hipSetDevice(o->device));
-    if (o->steps_per_chunk == 0) o->steps_per_chunk = auto_steps(nnz, o->device);
+    const int auto_s = o->steps_per_chunk == 0;
+    if (auto_s) o->steps_per_chunk = auto_steps(nnz, o->device);
     cvr_hip_handle *h = new (std::nothrow) cvr_hip_handle();
     if (!h) return set_error(PAVA_HIP_ENOMEM, "host allocation of the handle failed");
     h->device = o->device;
+    h->auto_s = auto_s;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return set_error(PAVA_HIP_EDEVICE, "hipStreamCreate failed");
@@ -687,9 +770,8 @@ static cvr::SpmvArgs spmv_args(const cvr_hip_handle *h, const void *d_x, void *d
         a.carry_bnd = h->d_carry_bnd;
         a.rc_count = h->d_rc_count;
         a.fix = h->d_fix;
-        a.close_of = h->d_close_of;
+        a.pinfo = h->d_pinfo;
         a.close_cnt = h->d_close_cnt;
-        a.row_intra = h->d_row_intra;
         a.wg_list = h->d_wg_list;
         a.wg_start = h->d_wg_start;
         a.parity_tol = h->opts.parity_tol;

@@ -105,9 +105,14 @@ struct SpmvArgs {
     int *rc_count;       // [c]: rows of chunk c recomputed in serial order by the last SpMV;
                          // [nchunks + c]: 1 if the row closed by chunk c was
     const int *fix;      // per chunk (row closed by the chunk or -1, its first chunk)
-    const int *close_of; // per chunk: the chunk closing the row its tail carry belongs to, or -1
+    // per chunk c, what its end needs, read when the chunk starts:
+    //   pinfo[2c]     = (row closed by c or -1, that row's first chunk,
+    //                    chunk k closing the row c's tail carry belongs to or -1, flags)
+    //   pinfo[2c + 1] = (row closed by k, that row's first chunk, 0, 0)
+    // flags: kPiHeadIntra / kPiTailIntra -- every chunk of the row closed by c /
+    // by k runs in one workgroup
+    const int4 *pinfo;
     int *close_cnt;      // per chunk: contributing chunks arrived (0 between SpMVs)
-    const int *row_intra;    // per closing chunk: 1 if every chunk of its row runs in one workgroup
     const int *wg_list;      // LDS variant: the chunks of each workgroup (or null: contiguous ranges)
     const int *wg_start;     // LDS variant: workgroup w runs wg_list[wg_start[w] .. wg_start[w + 1])
     double parity_tol;
@@ -134,6 +139,7 @@ struct SpmvArgs {
 // exceed parity_tol * |y| is recomputed as one serial FMA chain, which is
 // bit-identical to the reference. Underflow is covered by an absolute term of
 // 2^-1074 per element.
+constexpr int kPiHeadIntra = 1, kPiTailIntra = 2;  // pinfo flags
 constexpr int kParRigorousLen = 4096;
 constexpr double kParLambda = 10.0;
 constexpr long long kParNMask = 0xFFFFF;  // low 20 bits of a packed Q hold N (N <= 64 * 4096)
@@ -244,8 +250,6 @@ hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st);
 hipError_t plan_spmv_lds(int prec, int bound, int n_hot, long long max_bytes, int nchunks, int *n_lds,
                          int *lds_blocks);
 hipError_t launch_fixup(const int *fix, const double *carry, void *y, int nchunks, int prec, hipStream_t st);
-// close_of[k] = the chunk closing the row that continues past chunk k (-1 if none)
-hipError_t launch_close_of(const int *fix, int *close_of, int nchunks, hipStream_t st);
 // len[c] = length of the row chunk c closes, or -1
 hipError_t launch_closer_len(const int *fix, const int *rp, int *len, int nchunks, hipStream_t st);
 hipError_t launch_finish(const FinishArgs &a, hipStream_t st);

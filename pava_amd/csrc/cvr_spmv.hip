@@ -247,6 +247,9 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
     // recompute (ints [0, 128))
     __shared__ __align__(16) int segbuf[WPB][64 * kStepGroup];
     __shared__ int fcnt[PAR ? WPB : 1];
+    // PAR, plain variant: arrival counters of the rows whose chunks all run in
+    // this workgroup (one per chunk of the workgroup, zero at launch)
+    __shared__ int lcnt[PAR && !LDSX ? WPB : 1];
     extern __shared__ __align__(16) unsigned char lds_tab[];
     const XT *__restrict__ tab = reinterpret_cast<const XT *>(lds_tab);
 
@@ -262,6 +265,12 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
         const unsigned short *__restrict__ cp16 = reinterpret_cast<const unsigned short *>(a.cols) + base + lane;
         const uint64_t *__restrict__ emp = a.end_mask + (base >> 6);
         const int cbase = COL16 ? a.col_base[c] : 0;
+        // PAR: what the chunk's end needs (row closing, carry scope), read now
+        int4 pi0 = make_int4(-1, 0, -1, 0), pi1 = make_int4(-1, 0, 0, 0);
+        if constexpr (PAR) {
+            pi0 = a.pinfo[2 * (size_t)c];
+            pi1 = a.pinfo[2 * (size_t)c + 1];
+        }
 
         // Pipeline, one 8-step group deep: while group g is processed, the x
         // gathers and the segment-list entries of group g+1 (exactly as many as
@@ -449,10 +458,9 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                         flist[atomicAdd(&fcnt[wv], 1)] = t.x;
                 } else if (t.x == kCodeHead) {
                     // the row closed by this chunk
-                    put_carry(a, 2 * c, s, A, par_pack(Q, N), !a.row_intra[c]);
+                    put_carry(a, 2 * c, s, A, par_pack(Q, N), !(pi0.w & kPiHeadIntra));
                 } else if (t.x == kCodeTail) {
-                    const int k = a.close_of[c];
-                    put_carry(a, 2 * c + 1, s, A, par_pack(Q, N), !a.row_intra[k]);
+                    put_carry(a, 2 * c + 1, s, A, par_pack(Q, N), !(pi0.w & kPiTailIntra));
                 }
                 continue;
             }
@@ -486,37 +494,40 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
             if (lane == 0) a.rc_count[c] = nf;
 
             // rows spanning chunks: this chunk's head carry belongs to the row
-            // the chunk closes, its tail carry to the row closed by
-            // close_of[c]. Each contributing chunk counts itself in; the last
-            // one to arrive sums the carries in chunk order with their bound
-            // terms (carry: sum |S + w| <= N |S| + A, sum (S + w)^2 <=
-            // 2 N S^2 + 2 Q) and writes the row, recomputing it serially when
-            // par_ok fails. No wave waits for another; the counter is reset
-            // by its closer for the next SpMV.
-            const int targets[2] = {a.fix[2 * c] >= 0 ? c : -1, a.close_of[c]};
+            // the chunk closes, its tail carry to the row closed by chunk
+            // pi0.z. Each contributing chunk counts itself in; the last one to
+            // arrive sums the carries in chunk order with their bound terms
+            // (carry: sum |S + w| <= N |S| + A, sum (S + w)^2 <= 2 N S^2 +
+            // 2 Q) and writes the row, recomputing it serially when par_ok
+            // fails. No wave waits for another; a global counter is reset by
+            // its closer for the next SpMV.
             // the carries are in place before this chunk counts itself in:
-            // the workgroup-scope release for rows inside the range, and the
-            // completion of the agent-scope carry stores for rows spanning it
+            // the workgroup-scope release for rows inside the workgroup, and
+            // the completion of the agent-scope carry stores for rows spanning
+            // workgroups
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             {
-                const int kt = targets[1];
-                const bool span_head = targets[0] >= 0 && !a.row_intra[c];
-                const bool span_tail = kt >= 0 && !a.row_intra[kt];
+                const bool span_head = pi0.x >= 0 && !(pi0.w & kPiHeadIntra);
+                const bool span_tail = pi0.z >= 0 && !(pi0.w & kPiTailIntra);
                 if (span_head || span_tail) __builtin_amdgcn_s_waitcnt(kWaitVm0);
             }
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
-                const int k = targets[t];
+                const int k = t == 0 ? (pi0.x >= 0 ? c : -1) : pi0.z;
                 if (k < 0) continue;
-                const int c0 = a.fix[2 * k + 1];
-                const bool intra = a.row_intra[k] != 0;
+                const int row = t == 0 ? pi0.x : pi1.x;
+                const int c0 = t == 0 ? pi0.y : pi1.y;
+                const bool intra = (pi0.w & (t == 0 ? kPiHeadIntra : kPiTailIntra)) != 0;
+                // rows inside one workgroup of the plain variant count in LDS
+                // (chunk k is then the workgroup's chunk k % WPB)
+                const bool local = !LDSX && intra;
                 int old = 0;
                 if (lane == 0)
-                    old = __hip_atomic_fetch_add(&a.close_cnt[k], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    old = local ? atomicAdd(&lcnt[k % WPB], 1)
+                                : __hip_atomic_fetch_add(&a.close_cnt[k], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 old = __builtin_amdgcn_readfirstlane(old);
                 if (old != k - c0) continue;
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                const int row = a.fix[2 * k];
                 double s, A, Q, QN;
                 int N;
                 get_carry(a, 2 * c0 + 1, &s, &A, &QN, !intra);
@@ -542,7 +553,7 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                 if (lane == 0) {
                     store_y<YT>(a.y, row, s);
                     a.rc_count[a.nchunks + k] = ok ? 0 : 1;
-                    __hip_atomic_store(&a.close_cnt[k], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (!local) __hip_atomic_store(&a.close_cnt[k], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
         }
@@ -595,6 +606,10 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
     } else {
         const int g = xcd_remap(blockIdx.x, gridDim.x) * WPB;
         const int c = g + wv;
+        if constexpr (PAR) {
+            if (threadIdx.x < WPB) lcnt[threadIdx.x] = 0;
+            __syncthreads();
+        }
         if (c < a.nchunks) run_chunk(c);
     }
 }
@@ -622,14 +637,6 @@ __global__ void closer_len_kernel(const int *__restrict__ fix, const int *__rest
     if (c >= nchunks) return;
     const int r = fix[2 * c];
     len[c] = r >= 0 ? rp[r + 1] - rp[r] : -1;
-}
-
-// close_of[k] = c for every chunk k in [c0, c) of a row that chunk c closes
-// (close_of is -1 beforehand)
-__global__ void close_of_kernel(const int *__restrict__ fix, int *__restrict__ close_of, int nchunks) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nchunks || fix[2 * c] < 0) return;
-    for (int k = fix[2 * c + 1]; k < c; ++k) close_of[k] = c;
 }
 
 template <typename YT>
@@ -1012,12 +1019,6 @@ hipError_t launch_long_rows(const LongArgs &a, hipStream_t st) {
 hipError_t launch_closer_len(const int *fix, const int *rp, int *len, int nchunks, hipStream_t st) {
     if (nchunks == 0) return hipSuccess;
     hipLaunchKernelGGL(closer_len_kernel, dim3((nchunks + 255) / 256), dim3(256), 0, st, fix, rp, len, nchunks);
-    return hipGetLastError();
-}
-
-hipError_t launch_close_of(const int *fix, int *close_of, int nchunks, hipStream_t st) {
-    if (nchunks == 0) return hipSuccess;
-    hipLaunchKernelGGL(close_of_kernel, dim3((nchunks + 255) / 256), dim3(256), 0, st, fix, close_of, nchunks);
     return hipGetLastError();
 }
 

@@ -3,9 +3,9 @@
 * ``CONFIGS`` -- the candidate space: format (cvr_hip / csr5_hip / csr_hip), value
   precision (FP64; FP32 with FP64 recompute of rows whose cancellation ratio
   exceeds 64, which keeps every row within ~1e-5 of FP64; the same with 16-bit
-  chunk-relative columns), tile (CVR steps per chunk S, CSR5 elements per lane
-  sigma, or CSR lanes per row L) and x staging (hot-x copy of the most used
-  columns, its head held in LDS).
+  chunk-relative columns), tile (CVR steps per chunk S or the library's
+  automatic size, CSR5 elements per lane sigma, or CSR lanes per row L) and x
+  staging (hot-x copy of the most used columns, its head held in LDS).
 * ``autotune`` -- exhaustive search: builds each candidate on the GPU, times
   it with hipEvents, returns per-config times (inf where a candidate does not
   apply, e.g. a chunk spanning more than 65536 columns for 16-bit columns).
@@ -48,7 +48,8 @@ CONFIGS = ([Config("cvr", "f64", s) for s in (32, 128, 512)]
            + [Config("cvr", "f64", s, 2048) for s in (128, 512)]
            + [Config("cvr", "f32rc", 128, 2048)]
            + [Config("csr5", "f64", s) for s in (8, 32)]
-           + [Config("cvr", "f64", 0, -1)])
+           + [Config("cvr", "f64", 0, -1)]
+           + [Config("cvr", "f32rc", 0, -1), Config("cvr", "f32rc16", 0)])
 DEFAULT = Config("cvr", "f64", 0, -1)  # the fallback without a model: the library's default
 
 
@@ -237,3 +238,99 @@ def gen_params(rng: np.random.Generator, max_rows: int = 1_000_000, min_rows: in
     return dict(rows=rows, row_dist=dist, len_param=mean, shape=shape, col_struct=col,
                 half_band=int(rng.integers(8, 4096)), block=int(2 ** rng.integers(5, 10)), band_frac=0.7,
                 value_dist=0, max_len=min(rows, 20000), seed=int(rng.integers(0, 2 ** 31)))
+
+
+# ------------------------------------------------------- batched SpMM tiles
+# Candidate tiles of a batched multi-RHS SpMM, one choice per batch: CVR steps
+# per chunk S, and the column width (0: the library's automatic width, 16-bit
+# chunk-relative columns wherever a chunk spans fewer than 65,536 columns; 32:
+# 32-bit columns). X is read in place (no hot-x copy: a staged copy of X rows
+# would add an indirection to every nonzero without saving a line).
+BATCH_TILES = [(s, bits) for s in (16, 32, 64, 128, 256) for bits in (0, 32)]
+BATCH_FIXED = (64, 0)  # the tile the batched path used before it was autotuned
+
+
+def batch_family(nmat: int = 512, seed: int = 4, threads: int = 8):
+    """The batching workload: nmat square matrices, rows log-uniform in
+    [1e3, 2e5], Poisson row lengths with mean U[5, 60], 70% of columns within
+    +-1000 of the diagonal and 30% uniform, FP32 N(0,1) values (held in the
+    FP64 CSR as their FP32 roundings). Generated on `threads` host threads
+    (the generator releases the GIL); matrix i is seeded 10,000 + i."""
+    from concurrent.futures import ThreadPoolExecutor
+    from . import COL_MIXED, RL_POISSON, synthetic
+    rng = np.random.default_rng(seed)
+    prm = [(int(np.exp(rng.uniform(np.log(1e3), np.log(2e5)))), float(rng.uniform(5, 60))) for _ in range(nmat)]
+
+    def one(i):
+        n, mean = prm[i]
+        A = synthetic(n, RL_POISSON, mean, col_struct=COL_MIXED, half_band=1000, band_frac=0.7, value_dist=0,
+                      seed=10_000 + i)
+        A.vals = A.vals.astype(np.float32).astype(np.float64)
+        return A
+
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        return list(ex.map(one, range(nmat)))
+
+
+def build_batch_handles(mats, S: int, bits: int):
+    """FP32 CvrHip handles of every matrix at one tile (the batched SpMM's input)."""
+    hs = []
+    try:
+        for A in mats:
+            hs.append(CvrHip(A, steps_per_chunk=S, precision=PREC_F32, col_index_bits=bits, x_hot_kb=0))
+    except BaseException:
+        for h in hs:
+            h.close()
+        raise
+    return hs
+
+
+def time_batch(hs, mats, k: int, iters: int = 10, seed: int = 0) -> float:
+    """Milliseconds per replay of the captured persistent batched SpMM over
+    handles hs (FP32 X blocks of k columns, uniform [-1, 1])."""
+    import torch
+    from . import Batch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    Xs = [torch.rand((A.numCols, k), dtype=torch.float32, device="cuda", generator=g) * 2 - 1 for A in mats]
+    Ys = [torch.empty((A.numRows, k), dtype=torch.float32, device="cuda") for A in mats]
+    b = Batch(hs, k)
+    try:
+        b.set_operands([t.data_ptr() for t in Xs], [t.data_ptr() for t in Ys])
+        st = torch.cuda.Stream()
+        b.capture(st.cuda_stream)
+        for _ in range(3):
+            b.replay(st.cuda_stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(iters):
+            b.replay(st.cuda_stream)
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / iters
+    finally:
+        b.close()
+
+
+def autotune_batch(mats, k: int, tiles=BATCH_TILES, sample: int = 64, iters: int = 10, seed: int = 0):
+    """Exhaustive tile search for a batched SpMM: every candidate (S, column
+    width) is built and timed as one persistent launch over a seeded sample of
+    the batch's matrices. Returns (best tile, {tile: ms per launch on the
+    sample}); inf where a candidate cannot be built."""
+    rng = np.random.default_rng(seed)
+    pick = np.sort(rng.choice(len(mats), size=min(sample, len(mats)), replace=False))
+    sub = [mats[int(i)] for i in pick]
+    times = {}
+    for S, bits in tiles:
+        try:
+            hs = build_batch_handles(sub, S, bits)
+        except PavaError:
+            times[(S, bits)] = float("inf")
+            continue
+        try:
+            times[(S, bits)] = time_batch(hs, sub, k, iters=iters, seed=seed)
+        finally:
+            for h in hs:
+                h.close()
+    best = min(times, key=times.get)
+    return best, times

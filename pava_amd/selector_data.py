@@ -4,8 +4,9 @@
 Matrix i of the family is ``synthetic(**gen_params(default_rng([SEED, i])))``:
 indices [0, N_TRAIN) train, [N_TRAIN, N_TRAIN + N_HELD) are held out, and
 [N_TRAIN + N_HELD, N_TRAIN + N_HELD + N_LARGE) is a further held-out set of
-large matrices (1e6 to 5e6 rows). The training and first held-out sets use rows
-1e4 to 1e6.
+large matrices (1e6 to 5e6 rows); the N_LARGE_TRAIN matrices after those are
+large training matrices. The other sets use rows 1e4 to 1e6, so training
+covers rows 1e4 to 5e6.
 
   python -m pava_amd.selector_data gen --first 0 --count 500 --out DIR
       autotunes matrices [first, first+count) over every config and writes
@@ -36,14 +37,24 @@ import numpy as np
 
 SEED = 20260
 N_TRAIN, N_HELD, N_LARGE = 2000, 200, 20
+N_LARGE_TRAIN = 300
+LARGE0 = N_TRAIN + N_HELD  # first index of the large (1e6-5e6 rows) matrices
 
 
 def params_for(i: int) -> dict:
     from . import selector as S
     rng = np.random.default_rng([SEED, i])
-    if i >= N_TRAIN + N_HELD:
+    if i >= LARGE0:
         return S.gen_params(rng, max_rows=5_000_000, min_rows=1_000_000)
     return S.gen_params(rng, max_rows=1_000_000)
+
+
+def is_train(i: int) -> bool:
+    return i < N_TRAIN or i >= LARGE0 + N_LARGE
+
+
+def is_held_out_large(i: int) -> bool:
+    return LARGE0 <= i < LARGE0 + N_LARGE
 
 
 def matrix(i: int):
@@ -64,7 +75,7 @@ def gen(first: int, count: int, out: Path, iters: int = 5) -> Path:
         rows.append(A.numRows)
         nnz.append(A.nnz)
         idx.append(i)
-        if (i - first) % 10 == 9 or i == first + count - 1:
+        if (i - first) % 10 == 9 or i == first + count - 1 or A.nnz > 20_000_000:
             print(f"[selector] {i - first + 1}/{count} matrices ({time.perf_counter() - t0:.0f}s, last: "
                   f"{A.numRows} rows {A.nnz} nnz)", flush=True)
     out.mkdir(parents=True, exist_ok=True)
@@ -172,11 +183,13 @@ def evaluate(data: Path, epochs: int = 400, seed: int = 0, check_errors: int = 2
     names = [str(n) for n in d["configs"]]
     assert names == [c.name for c in S.CONFIGS], "dataset built for another config space"
     idx, F, T = d["index"], d["features"], d["ms"]
-    tr = idx < N_TRAIN
-    sets = {"held_out": (idx >= N_TRAIN) & (idx < N_TRAIN + N_HELD), "held_out_large": idx >= N_TRAIN + N_HELD}
+    tr = np.asarray([is_train(int(i)) for i in idx])
+    sets = {"held_out": (idx >= N_TRAIN) & (idx < LARGE0),
+            "held_out_large": np.asarray([is_held_out_large(int(i)) for i in idx])}
     t0 = time.perf_counter()
     sel, hist = S.Selector.fit(F[tr], T[tr], epochs=epochs, seed=seed)
-    report = {"configs": names, "n_train": int(tr.sum()), "epochs": epochs,
+    report = {"configs": names, "n_train": int(tr.sum()), "n_train_large": int(np.sum(idx >= LARGE0 + N_LARGE)),
+              "epochs": epochs,
               "train_loss_first_last": [float(hist[0]), float(hist[-1])],
               "train_s": round(time.perf_counter() - t0, 2)}
     dflt = S.CONFIGS.index(S.DEFAULT)

@@ -279,18 +279,23 @@ def test_low_precision_bit_exact_vs_emulation(P, O, mats, prec):
 
 def test_automatic_chunk_size(P, O):
     """steps_per_chunk = 0: about four chunks per resident wave slot (16 per CU),
-    a multiple of 8 in [8, 256]; the layout is the emulation's at that size."""
+    a multiple of 8 in [8, 256]; two per slot with FP64 parity on a 16-bit
+    layout. The layout is the emulation's at that size."""
     import torch
     ncu = torch.cuda.get_device_properties(0).multi_processor_count
     for A in (P.stencil27(64, 0.01, 0), P.laplacian2d(40, 40)):
-        with P.CvrHip(A) as h:
-            S = h.info()["steps_per_chunk"]
-            x = P.fill_uniform(A.numCols, 1)
-            y = h.spmv_host(x)
-        steps = -(-A.nnz // (64 * 4 * 16 * ncu))  # ceil: four chunks per wave slot
-        want = min(max(-(-steps // 8) * 8, 8), 256)
-        assert S == want, (S, want)
-        assert np.array_equal(y, O.cvr64_spmv(A, O.cvr64_build(A, S, 0), x))
+        for tol in (0.0, None):
+            with P.CvrHip(A, parity_tol=tol) as h:
+                inf = h.info()
+                S = inf["steps_per_chunk"]
+                x = P.fill_uniform(A.numCols, 1)
+                y = h.spmv_host(x)
+            rounds = 2 if inf["parity_tol"] > 0 and inf["col_index_bits"] == 16 else 4
+            steps = -(-A.nnz // (64 * rounds * 16 * ncu))  # ceil: chunks per wave slot
+            want = min(max(-(-steps // 8) * 8, 8), 256)
+            assert S == want, (S, want, tol)
+            lay = O.cvr64_build(A, S, 0)
+            assert np.array_equal(y, O.cvr64_spmv(A, lay, x, parity_tol=inf["parity_tol"]))
 
 
 def test_invalid_options_rejected(P):

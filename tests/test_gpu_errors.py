@@ -57,7 +57,9 @@ def test_device_allocation_failure_releases_everything(P, case):
         make = lambda: P.CvrHip(A, steps_per_chunk=16)
     n = _sweep(P, make)
     _say(f"{case}: {n} allocation sites failed cleanly")
-    assert n >= 10
+    # the handle's arrays come from one arena allocation; the CSR copy, the
+    # temporaries of hot-x staging and the parity plan are allocated apart
+    assert n >= 3
     # the library still works afterwards
     x = P.fill_uniform(A.numCols, 1)
     with P.CvrHip(A, steps_per_chunk=16) as h:

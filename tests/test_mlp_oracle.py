@@ -58,7 +58,7 @@ def test_features_closed_forms(P):
 def test_config_space():
     from pava_amd import selector as S
     names = [c.name for c in S.CONFIGS]
-    assert len(names) == len(set(names)) == 19
+    assert len(names) == len(set(names)) == 21
     assert S.DEFAULT in S.CONFIGS
     # every axis the autotuner searches is present: format, precision, tile, x staging
     assert {c.fmt for c in S.CONFIGS} == {"cvr", "csr5", "csr"}
@@ -75,8 +75,10 @@ def test_committed_selector_dataset_is_consistent():
     from pava_amd import selector_data as D
     d = np.load(Path(S.__file__).parent / "data" / "selector_dataset.npz", allow_pickle=False)
     assert [str(n) for n in d["configs"]] == [c.name for c in S.CONFIGS]
-    n = D.N_TRAIN + D.N_HELD + D.N_LARGE
+    n = D.N_TRAIN + D.N_HELD + D.N_LARGE + D.N_LARGE_TRAIN
     assert np.array_equal(d["index"], np.arange(n))
+    # training covers rows 1e4 to 5e6: the large training matrices are in
+    assert d["rows"][D.LARGE0 + D.N_LARGE:].min() >= 1_000_000 and d["rows"].max() <= 5_000_000
     assert d["features"].shape == (n, S.N_FEATURES) and d["ms"].shape == (n, len(S.CONFIGS))
     assert np.all(np.isfinite(d["ms"]).sum(1) >= 12) and np.all(d["ms"][np.isfinite(d["ms"])] > 0)
     for i in (0, 7, 2001):
