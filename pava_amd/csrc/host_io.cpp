@@ -75,6 +75,17 @@ using namespace pava;
 
 extern "C" const char *pava_hip_last_error(void) { return g_last_error.c_str(); }
 
+void pava::parallel_memcpy(void *dst, const void *src, size_t bytes) {
+    constexpr size_t kBlock = size_t(1) << 20;
+    const long long nb = (long long)((bytes + kBlock - 1) / kBlock);
+#pragma omp parallel for schedule(static) num_threads(8) if (nb > 4)
+    for (long long b = 0; b < nb; ++b) {
+        const size_t off = (size_t)b * kBlock;
+        std::memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off,
+                    std::min(kBlock, bytes - off));
+    }
+}
+
 extern "C" void pava_hip_test_fail_alloc(int host_nth, int device_nth) {
     g_fail_host.store(host_nth > 0 ? host_nth : 0);
     g_fail_device.store(device_nth > 0 ? device_nth : 0);

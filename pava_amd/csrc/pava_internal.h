@@ -18,6 +18,8 @@ void *aligned_alloc64(size_t bytes);
 bool inject_device_failure();
 // PAVA_HIP_ERANGE if a dimension or nnz exceeds PAVA_HIP_MAX_SIZE
 int check_gpu_sizes(long long M, long long N, long long nnz);
+// memcpy split over the host's OpenMP threads (large copies into pinned staging memory)
+void parallel_memcpy(void *dst, const void *src, size_t bytes);
 
 // Counter-based generator: every draw is a pure function of (key, index), so
 // generation is deterministic whatever the thread count. The mixer is the
