@@ -90,10 +90,11 @@ def measured_traffic(workload: str, kernel: str, S: int, x_hot_kb: int):
     f = ROOT / "profiles" / "traffic_per_launch.json"
     if not f.exists():
         return None
-    m = json.loads(f.read_text()).get(workload)
-    if not m or m["kernel"] != kernel or m["steps_per_chunk"] != S or m["x_hot_kb"] != x_hot_kb:
-        return None
-    return m["traffic_bytes"]
+    ms = json.loads(f.read_text()).get(workload) or []
+    for m in ms if isinstance(ms, list) else [ms]:
+        if m["kernel"] == kernel and m["steps_per_chunk"] == S and m["x_hot_kb"] == x_hot_kb:
+            return m["traffic_bytes"]
+    return None
 
 
 def _cpu_model() -> str:
@@ -231,11 +232,11 @@ def _time_calls(fn, iters: int, stream) -> float:
 def run_batch(P, k: int, nmat: int, steps: int, check: int = 8, mats=None, tile=None) -> dict:
     """nmat matrices of the batching family times FP32 RHS blocks of k columns
     in one persistent launch replayed from a hipGraph, with the tile (steps
-    per chunk, column width) chosen by the batched autotuner on a seeded
+    per chunk, column width, X staging) chosen by the batched autotuner on a seeded
     sample of 64 of the matrices; beside it the same batch at the fixed tile
     it used before (S = 64, automatic width), and the plain FP64 CVR-HIP SpMV
     (library defaults) of every matrix, one launch each. With `tile` given
-    (S, bits) the autotuner and the fixed-tile comparison are skipped (a
+    (S, bits, win) the autotuner and the fixed-tile comparison are skipped (a
     profiler then sees only the full batch's launches)."""
     import torch
     from pava_amd import selector as SEL
@@ -251,23 +252,28 @@ def run_batch(P, k: int, nmat: int, steps: int, check: int = 8, mats=None, tile=
     if not given:
         tile, tune = SEL.autotune_batch(mats, k)
     t_tune = time.perf_counter() - t1
-    print(f"[bench] batch k={k}: autotuned tile S={tile[0]} bits={tile[1]} ({t_tune:.0f}s)", file=sys.stderr,
+    print(f"[bench] batch k={k}: tile S={tile[0]} bits={tile[1]} x-window={tile[2]} ({t_tune:.0f}s)", file=sys.stderr,
           flush=True)
     nnz = sum(A.nnz for A in mats)
     rows = sum(A.numRows for A in mats)
     cols = sum(A.numCols for A in mats)
     res = {"matrices": nmat, "k": k, "nnz": nnz, "rows": rows,
-           "tile": {"steps_per_chunk": tile[0], "col_index_bits": "auto" if tile[1] == 0 else tile[1]},
-           "autotune_ms_on_sample": {f"S{s}-{'auto' if b == 0 else b}": round(v, 4) for (s, b), v in tune.items()},
+           "tile": {"steps_per_chunk": tile[0], "col_index_bits": "auto" if tile[1] == 0 else tile[1],
+                    "x_window_wgs_per_cu": tile[2]},
+           "autotune_ms_on_sample": {f"S{s}-{'auto' if b == 0 else b}-w{w}": round(v, 4)
+                                     for (s, b, w), v in tune.items()},
            "autotune_s": round(t_tune, 1)}
 
-    def batch_ms(S, bits):
+    def batch_ms(S, bits, win):
         hs = SEL.build_batch_handles(mats, S, bits)
         try:
             b = P.Batch(hs, k)
+            if win:
+                b.set_x_window(win)
             inf = b.info()
+            inf["col16_nnz"] = sum(A.nnz for A, h in zip(mats, hs) if h.info()["col_index_bits"] == 16)
             b.close()
-            return SEL.time_batch(hs, mats, k, iters=max(steps, 5)), inf
+            return SEL.time_batch(hs, mats, k, iters=max(steps, 5), win=win), inf
         finally:
             for h in hs:
                 h.close()
@@ -275,10 +281,12 @@ def run_batch(P, k: int, nmat: int, steps: int, check: int = 8, mats=None, tile=
     ms_batch, binf = batch_ms(*tile)
     ms_fixed = None if given else batch_ms(*SEL.BATCH_FIXED)[0]
     # accuracy of the tuned batch on `check` matrices: FP64 reference SpMV on the same FP32 inputs
-    hs = SEL.build_batch_handles(mats[:check], *tile)
+    hs = SEL.build_batch_handles(mats[:check], tile[0], tile[1])
     errs = []
     try:
         b = P.Batch(hs, k)
+        if tile[2]:
+            b.set_x_window(tile[2])
         g = torch.Generator(device="cuda")
         g.manual_seed(1)
         Xs = [torch.rand((A.numCols, k), dtype=torch.float32, device=dev, generator=g) * 2 - 1 for A in mats[:check]]
@@ -310,9 +318,13 @@ def run_batch(P, k: int, nmat: int, steps: int, check: int = 8, mats=None, tile=
     finally:
         for h in h64:
             h.close()
-    Bb = nnz * 8 + (rows + 1) * 4 + cols * k * 4 + rows * k * 4
+    # algorithmic bytes with each layout's index width: FP32 values, 4-byte
+    # columns or 2.125 B (16-bit offsets + end masks), row pointers, X and Y
+    c16 = binf["col16_nnz"]
+    Bb = int(round((nnz - c16) * 8 + c16 * 6.125 + (rows + len(mats)) * 4 + cols * k * 4 + rows * k * 4))
     res.update({
-        "chunks": binf["chunks"], "grid": binf["grid"],
+        "chunks": binf["chunks"], "items": binf["items"], "grid": binf["grid"], "algorithmic_bytes": Bb,
+        "nnz_16bit_columns": c16, "roofline_frac": round(Bb / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "ms_batch": round(ms_batch, 4),
         "ms_batch_fixed_tile_S64": None if ms_fixed is None else round(ms_fixed, 4),
         "tuned_vs_fixed_tile": None if ms_fixed is None else round(ms_fixed / ms_batch, 3),
@@ -378,7 +390,12 @@ def run_mixed_spmv(P, A, x, sel, iters: int = 200) -> dict:
         h0.spmv(x64.data_ptr(), y0.data_ptr(), st.cuda_stream)
         st.synchronize()
         err0 = error_stats(y0.cpu().numpy(), yref)
-    return {"rows": A.numRows, "nnz": A.nnz, "selected": cfg.name,
+    # exhaustive autotuning over the same configs: the judge of the prediction
+    tt = SEL.autotune(A, x, iters=20)
+    ib = int(np.argmin(tt))
+    auto = {"autotuned_best": SEL.CONFIGS[ib].name, "autotuned_best_ms": round(float(tt[ib]), 5),
+            "selected_slowdown_vs_autotuned_best": round(float(tt[SEL.CONFIGS.index(cfg)] / tt[ib]), 3)}
+    return {"rows": A.numRows, "nnz": A.nnz, "selected": cfg.name, **auto,
             "selected_nnz_per_s": A.nnz / (ms_sel * 1e-3), "selected_ms": round(ms_sel, 5),
             "selected_max_rel": err["max_rel"], "selected_median_rel": err["median_rel"],
             "fp64_default_nnz_per_s": A.nnz / (ms_64 * 1e-3), "fp64_default_ms": round(ms_64, 5),
@@ -433,13 +450,14 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed_vs_fp64 legs (selector-chosen SpMV on the stencil and R-MAT matrices, "
                          "autotuned batched SpMM)")
-    ap.add_argument("--variant", choices=["none", "precision", "batch", "create"], default="none",
+    ap.add_argument("--variant", choices=["none", "precision", "batch", "create", "gathers"], default="none",
                     help="precision: FP64/FP32/BF16 storage on the workload's matrix; batch: 512-matrix SpMM; "
                          "create: --steps handle creations from the host CSR")
     ap.add_argument("--k", type=int, default=8, help="batch variant: right-hand sides")
     ap.add_argument("--nmat", type=int, default=512, help="batch variant: matrices")
     ap.add_argument("--batch-tile", default="",
-                    help="batch variant: fixed tile S:bits (bits 0 = automatic width) instead of the autotuner")
+                    help="batch variant: fixed tile S:bits:win (bits 0 = automatic width; win = X window "
+                         "workgroups per CU, 0 = off) instead of the autotuner")
     ap.add_argument("--x-hot-kb", type=int, default=-1,
                     help="hot-x staging buffer (KiB; 0 = off; -1 = automatic: 2048 KiB when column use is skewed): "
                          "the most used columns read from a dense copy")
@@ -492,6 +510,27 @@ def main():
                          "hot_setup_ms": round(inf["hot_setup_ms"], 3)})
             h.close()
         print(json.dumps({"variant": "create", "workload": args.workload, "nnz": A.nnz, "reps": reps}), flush=True)
+        return
+    if args.variant == "gathers":
+        # where each x gather of the SpMV is served from: the LDS table, the
+        # hot-x copy or x itself (cold), from the handle's own staging
+        A = make_matrix(P, w)
+        with P.CvrHip(A, steps_per_chunk=args.steps_per_chunk, col_index_bits=args.col_index_bits,
+                      x_hot_kb=args.x_hot_kb, x_lds_kb=args.x_lds_kb) as h:
+            inf = h.info()
+            hot = h.hot_cols()
+        rank = np.full(A.numCols, np.iinfo(np.int64).max, np.int64)
+        rank[hot] = np.arange(hot.size)
+        r = rank[A.cols]
+        n_lds, n_hot = inf["n_lds_cols"], inf["n_hot_cols"]
+        lds = int(np.sum(r < n_lds))
+        hotn = int(np.sum((r >= n_lds) & (r < n_hot)))
+        cold_cols = np.unique(A.cols[r >= n_hot])
+        print(json.dumps({"variant": "gathers", "workload": args.workload, "nnz": A.nnz, "lds_columns": n_lds,
+                          "hot_columns": n_hot, "frac_lds": lds / A.nnz, "frac_hot_copy": hotn / A.nnz,
+                          "frac_cold": (A.nnz - lds - hotn) / A.nnz,
+                          "cold_distinct_columns": int(cold_cols.size),
+                          "cold_distinct_x_lines_128B": int(np.unique(cold_cols // 16).size)}), flush=True)
         return
     if args.variant == "precision":
         A = make_matrix(P, w)

@@ -272,10 +272,21 @@ int cvr_hip_batch_create(int nmat, cvr_hip_handle *const *handles, int k, cvr_hi
  * row-major, 16-byte aligned when k is 8, 16 or 32. Synchronous upload. */
 int cvr_hip_batch_set_operands(cvr_hip_batch *b, const float *const *d_X, float *const *d_Y);
 int cvr_hip_batch_spmm(cvr_hip_batch *b, void *stream);
+/* X staging for k <= 8 (no counterpart in pava: its batching is this
+ * library's own): with wgs_per_cu = 1 or 2, workgroups of 8 waves take groups
+ * of 4 consecutive chunks of one matrix and first stage in LDS the rows of
+ * that matrix's X around the group's rows (as many as fit with wgs_per_cu
+ * workgroups per CU); nonzeros whose column falls there read X from LDS. Y is
+ * unchanged bit for bit. 0 turns it off (the default). EINVAL for k > 8,
+ * ERANGE if no window of 256 rows fits. Drops a captured graph: call
+ * cvr_hip_batch_graph again after it. */
+int cvr_hip_batch_set_x_window(cvr_hip_batch *b, int wgs_per_cu);
 /* Captures one batched SpMM on `stream` (non-null) into a hipGraph. */
 int cvr_hip_batch_graph(cvr_hip_batch *b, void *stream);
 int cvr_hip_batch_replay(cvr_hip_batch *b, void *stream);
-int cvr_hip_batch_info(const cvr_hip_batch *b, int *nchunks_total, int *grid, long long *nnz);
+/* *items = work items of the queue (chunks; groups of 4 chunks with the X
+ * window), *grid = workgroups of the persistent launch. */
+int cvr_hip_batch_info(const cvr_hip_batch *b, int *items, int *grid, long long *nnz);
 /* Queue check of the last completed batched SpMM (synchronises the device).
  * The queue is cut into *nranges = min(grid, 8) ranges; range q holds work
  * items [total*q/n, total*(q+1)/n). counts[q] (8 entries) = pops from range q:

@@ -80,6 +80,7 @@ EXPORTED = [
     "cvr_hip_time_spmv", "cvr_hip_time_spmv_x64", "cvr_hip_time_spmv_stats", "cvr_hip_export_layout", "conduct_cvr_hip", "pava_bench_file",
     "pava_hip_last_error", "pava_hip_version", "cvr_hip_batch_create", "cvr_hip_batch_set_operands",
     "cvr_hip_batch_spmm", "cvr_hip_batch_graph", "cvr_hip_batch_replay", "cvr_hip_batch_info",
+    "cvr_hip_batch_set_x_window",
     "cvr_hip_batch_destroy", "csr_hip_create", "csr_hip_spmv", "csr_hip_destroy", "pava_csr_features",
     "pava_mlp_create", "pava_mlp_num_params", "pava_mlp_get_params", "pava_mlp_set_params", "pava_mlp_predict",
     "pava_mlp_loss_grad", "pava_mlp_train", "pava_mlp_destroy", "cvr_hip_export_hot", "pava_write_csr_binary",
@@ -460,10 +461,18 @@ class Batch:
     def replay(self, stream: int) -> None:
         _check(lib().cvr_hip_batch_replay(self._b, stream))
 
+    def set_x_window(self, wgs_per_cu: int) -> None:
+        """LDS window of X (k <= 8) sized for wgs_per_cu workgroups per CU; 0 = off."""
+        lib().cvr_hip_batch_set_x_window.argtypes = [C.c_void_p, C.c_int]
+        _check(lib().cvr_hip_batch_set_x_window(self._b, wgs_per_cu))
+
     def info(self) -> dict:
+        """items: queue work items (chunks, or groups of 4 with the X window);
+        chunks: all chunks of the batch; grid: persistent workgroups."""
         t, g, n = C.c_int(), C.c_int(), C.c_longlong()
         _check(lib().cvr_hip_batch_info(self._b, C.byref(t), C.byref(g), C.byref(n)))
-        return {"chunks": t.value, "grid": g.value, "nnz": n.value}
+        return {"items": t.value, "chunks": sum(m.info()["nchunks"] for m in self.mats), "grid": g.value,
+                "nnz": n.value}
 
     def queue_counts(self) -> tuple:
         """(pops per queue range in the last completed launch, number of ranges)."""

@@ -943,6 +943,7 @@ static int spmm_desc_fill(cvr_hip_handle *h, int k, const float *d_X, float *d_Y
     a->nchunks = h->nchunks;
     a->S = h->S;
     a->M = h->M;
+    a->N = h->N;
     a->k = k;
     a->has_empty = h->has_empty;
     return 0;
@@ -1037,6 +1038,10 @@ struct cvr_hip_batch {
     std::vector<cvr_hip_handle *> mats;
     cvr::SpmmDesc *d_descs = nullptr;
     int2 *d_queue = nullptr;
+    // X window (k <= 8): groups of kWinChunks chunks, win workgroups per CU (0: off)
+    int2 *d_gqueue = nullptr;
+    int total_g = 0, win = 0, win_rows = 0, win_grid = 0;
+    size_t win_lds = 0;
     int *d_counter = nullptr;
     std::vector<cvr::SpmmDesc> descs;
     hipGraph_t graph = nullptr;
@@ -1085,19 +1090,26 @@ static int batch_create_body(int nmat, cvr_hip_handle *const *handles, int k, cv
         for (int c = 0; c < h->nchunks; ++c) queue.push_back(make_int2(i, c));
     }
     b->total = (int)queue.size();
+    std::vector<int2> gqueue;
+    for (int i = 0; i < nmat; ++i)
+        for (int c = 0; c < handles[i]->nchunks; c += cvr::kWinChunks) gqueue.push_back(make_int2(i, c));
+    b->total_g = (int)gqueue.size();
     b->descs.resize(nmat);
     auto fail = [&](int code) {  // the guard deletes b
         dfree(b->d_descs);
         dfree(b->d_queue);
+        dfree(b->d_gqueue);
         dfree(b->d_counter);
         return code;
     };
     if (hipSetDevice(b->device) != hipSuccess) return fail(set_error(PAVA_HIP_EDEVICE, "hipSetDevice failed"));
     if (dmalloc(&b->d_descs, sizeof(cvr::SpmmDesc) * nmat) != hipSuccess ||
         dmalloc(&b->d_queue, sizeof(int2) * std::max(b->total, 1)) != hipSuccess ||
+        dmalloc(&b->d_gqueue, sizeof(int2) * std::max(b->total_g, 1)) != hipSuccess ||
         dmalloc(&b->d_counter, 256) != hipSuccess)
         return fail(set_error(PAVA_HIP_ENOMEM, "batch allocation failed"));
     if (hipMemcpy(b->d_queue, queue.data(), sizeof(int2) * queue.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_gqueue, gqueue.data(), sizeof(int2) * gqueue.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(b->d_counter, 0, 256) != hipSuccess)
         return fail(set_error(PAVA_HIP_EDEVICE, "batch upload failed"));
     b->lds = cvr::spmm_lds_bytes(b->max_slots, k);
@@ -1119,7 +1131,39 @@ extern "C" int cvr_hip_batch_set_operands(cvr_hip_batch *b, const float *const *
 }
 
 static int batch_enqueue(cvr_hip_batch *b, hipStream_t st) {
-    HIP_TRY(cvr::launch_spmm_batch(b->d_descs, b->d_queue, b->total, b->d_counter, b->prec, b->k, b->lds, b->grid, st));
+    if (b->win > 0)
+        HIP_TRY(cvr::launch_spmm_batch_win(b->d_descs, b->d_gqueue, b->total_g, b->d_queue, b->total, b->d_counter,
+                                           b->prec, b->max_slots, b->win_rows, b->win_lds, b->win_grid, st));
+    else
+        HIP_TRY(cvr::launch_spmm_batch(b->d_descs, b->d_queue, b->total, b->d_counter, b->prec, b->k, b->lds, b->grid,
+                                       st));
+    return 0;
+}
+
+extern "C" int cvr_hip_batch_set_x_window(cvr_hip_batch *b, int wgs_per_cu) {
+    if (!b) return set_error(PAVA_HIP_EINVAL, "null batch");
+    if (wgs_per_cu < 0 || wgs_per_cu > 4) return set_error(PAVA_HIP_EINVAL, "wgs_per_cu must be in [0, 4]");
+    if (wgs_per_cu > 0 && b->k > 8) return set_error(PAVA_HIP_EINVAL, "the X window needs k <= 8, got %d", b->k);
+    HIP_TRY(hipSetDevice(b->device));
+    // a captured launch no longer matches: it is dropped
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    if (wgs_per_cu == 0) {
+        b->win = 0;
+        return 0;
+    }
+    int rows = 0, grid = 0;
+    size_t lds = 0;
+    HIP_TRY(cvr::spmm_batch_win_plan(b->prec, b->max_slots, wgs_per_cu, &rows, &lds, &grid));
+    if (rows == 0)
+        return set_error(PAVA_HIP_ERANGE, "no X window of %d rows or more fits %d workgroups per CU",
+                         cvr::kWinMinRows, wgs_per_cu);
+    b->win = wgs_per_cu;
+    b->win_rows = rows;
+    b->win_lds = lds;
+    b->win_grid = grid;
     return 0;
 }
 
@@ -1159,10 +1203,10 @@ extern "C" int cvr_hip_batch_replay(cvr_hip_batch *b, void *stream) {
     return 0;
 }
 
-extern "C" int cvr_hip_batch_info(const cvr_hip_batch *b, int *nchunks_total, int *grid, long long *nnz) {
+extern "C" int cvr_hip_batch_info(const cvr_hip_batch *b, int *items, int *grid, long long *nnz) {
     if (!b) return set_error(PAVA_HIP_EINVAL, "null batch");
-    if (nchunks_total) *nchunks_total = b->total;
-    if (grid) *grid = b->grid;
+    if (items) *items = b->win > 0 ? b->total_g : b->total;
+    if (grid) *grid = b->win > 0 ? b->win_grid : b->grid;
     if (nnz) *nnz = b->nnz;
     return 0;
 }
@@ -1172,7 +1216,7 @@ extern "C" int cvr_hip_batch_queue_counts(const cvr_hip_batch *b, int *counts, i
     HIP_TRY(hipSetDevice(b->device));
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(counts, b->d_counter + cvr::kBatchQueues, cvr::kBatchQueues * sizeof(int), hipMemcpyDeviceToHost));
-    if (nranges) *nranges = std::min(b->grid, cvr::kBatchQueues);
+    if (nranges) *nranges = std::min(b->win > 0 ? b->win_grid : b->grid, cvr::kBatchQueues);
     return 0;
 }
 
@@ -1184,6 +1228,7 @@ extern "C" int cvr_hip_batch_destroy(cvr_hip_batch *b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     dfree(b->d_descs);
     dfree(b->d_queue);
+    dfree(b->d_gqueue);
     dfree(b->d_counter);
     delete b;
     return 0;

@@ -226,7 +226,7 @@ struct SpmmDesc {
     float *Y;         // numRows x k, row-major
     double *carry;    // 2 * nchunks * k
     long long layout_elems;
-    int nchunks, S, M, k, has_empty, col16;
+    int nchunks, S, M, N, k, has_empty, col16;
 };
 
 int spmm_kp(int k);
@@ -236,6 +236,18 @@ hipError_t launch_spmm(const SpmmDesc &a, int prec, int max_slots, hipStream_t s
 hipError_t spmm_batch_grid(int prec, int k, size_t lds, int *grid);
 hipError_t launch_spmm_batch(const SpmmDesc *d_descs, const int2 *d_queue, int total, int *d_counter, int prec, int k,
                              size_t lds, int grid, hipStream_t st);
+// The batched SpMM with an LDS window of X (k <= 8): workgroups of 8 waves
+// take groups of kWinChunks consecutive chunks of one matrix.
+constexpr int kWinChunks = 4;
+// For wgs_per_cu workgroups per CU: the window rows that fit beside the
+// slot arrays (0 if fewer than kWinMinRows), the dynamic LDS and the grid.
+constexpr int kWinMinRows = 256, kWinMaxRows = 8192;
+hipError_t spmm_batch_win_plan(int prec, int max_slots, int wgs_per_cu, int *win_rows, size_t *lds, int *grid);
+// d_gqueue: (matrix, first chunk) of each group; d_queue: every (matrix,
+// chunk), for the fixup that closes rows spanning chunks
+hipError_t launch_spmm_batch_win(const SpmmDesc *d_descs, const int2 *d_gqueue, int total_g, const int2 *d_queue,
+                                 int total, int *d_counter, int prec, int max_slots, int win_rows, size_t lds, int grid,
+                                 hipStream_t st);
 
 hipError_t launch_empty_rows(const int *rp, int M, unsigned *bits, int *status, hipStream_t st);
 // *bad |= 1 if the device CSR's offsets are not 0 .. nnz and monotone, |= 2 if a column is outside [0, N)

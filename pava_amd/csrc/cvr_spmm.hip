@@ -40,6 +40,9 @@ __device__ __forceinline__ void wave_sync_lds_m() {
 // The layout, X and Y live in global memory; pointers read from a descriptor
 // in memory (the batched path) go through to_global.
 typedef float vfloat4 __attribute__((ext_vector_type(4)));  // 16-byte loads and stores of X and Y rows
+// LDS pointers (the X window): address space 3, so reads are ds_read, not flat
+typedef __attribute__((address_space(3))) const float lds_cf32;
+typedef __attribute__((address_space(3))) const vfloat4 lds_cv4;
 
 template <typename VT>
 __device__ __forceinline__ double val_f64(VT v);
@@ -82,9 +85,14 @@ __device__ __forceinline__ unsigned long long group_step_mask(unsigned long long
 
 // C16: 16-bit layouts (chunk-relative column offsets, end flags in a mask per
 // step) instead of 32-bit columns whose sign bit flags a segment end.
-template <typename VT, int KP, bool C16>
+// WIN: rows [w0, w0 + W) of X are staged in LDS (`win`, KP floats per row):
+// a nonzero whose column falls there reads its X row from LDS. ltid / lnt:
+// this chunk's thread index and thread count for the work spread over the
+// chunk's threads (split rows, empty rows).
+template <typename VT, int KP, bool C16, bool WIN = false>
 __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const int lane, double *slots,
-                           int *segbuf) {
+                           int *segbuf, lds_cf32 *win = nullptr, const int w0 = 0, const int W = 0,
+                           const int ltid = -1, const int lnt = 0) {
     constexpr int TPL = KP / 4;   // threads per CVR lane
     constexpr int VL = 64 / TPL;  // CVR lanes per wave
     // steps per pipeline group: 8 for k <= 8; 4 for wider blocks, whose X-row
@@ -148,6 +156,14 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
         }
 #pragma unroll
         for (int i = 0; i < G; ++i) {
+            if constexpr (WIN) {
+                const unsigned rel = (unsigned)(col[i] - w0);
+                if (rel < (unsigned)W) {
+                    const vfloat4 q = *reinterpret_cast<lds_cv4 *>(win + rel * KP + j0);
+                    xo[i] = make_float4(q.x, q.y, q.z, q.w);
+                    continue;
+                }
+            }
             const gptr<const float> p = X + (size_t)col[i] * k + j0;
             if (full) {
                 const vfloat4 q = *reinterpret_cast<gptr<const vfloat4>>(p);
@@ -257,7 +273,7 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
 
     // split rows: (group, column) pairs over the workgroup, slots summed in element order
     __syncthreads();
-    const int tid = threadIdx.x, nthreads = blockDim.x;
+    const int tid = ltid >= 0 ? ltid : (int)threadIdx.x, nthreads = ltid >= 0 ? lnt : (int)blockDim.x;
     const int ns = a.split_n[c];
     const int2 *tab = a.split_tab + (size_t)c * kSplitCap;
     for (int t = tid; t < ns * KP; t += nthreads) {
@@ -357,6 +373,81 @@ __global__ __launch_bounds__(spmm_threads<KP>()) void spmm_batch_kernel(const Sp
 }
 
 
+
+// Persistent batched SpMM with an LDS window of X (k <= 8): a workgroup of
+// 8 waves pops groups of kWinChunks consecutive chunks of one matrix, stages
+// the rows [w0, w0 + W) of that matrix's X in LDS -- a window centred on the
+// group's rows, scaled to the column range, which holds the X rows of a
+// banded matrix's nonzeros -- and runs the group's chunks on its wave pairs
+// side by side. Nonzeros whose column falls in the window read their X row
+// from LDS instead of fetching a line through the L1; the rest read X as in
+// spmm_batch_kernel. The dynamic LDS holds kWinChunks slot arrays
+// (slot_doubles each) and then the window (W rows x 8 floats, zero beyond k).
+template <typename VT>
+__global__ __launch_bounds__(64 * 2 * kWinChunks) void spmm_batch_win_kernel(const SpmmDesc *__restrict__ descs,
+                                                                           const int2 *__restrict__ queue,
+                                                                           int total, int *__restrict__ counter,
+                                                                           int slot_doubles, int win_rows) {
+    extern __shared__ __align__(16) double smem[];
+    __shared__ int segbuf[2 * kWinChunks][64 * kStepGroup];
+    __shared__ int item;
+    constexpr int KP = 8;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int slot = wv >> 1;
+    float *win = reinterpret_cast<float *>(smem + (size_t)kWinChunks * slot_doubles);
+    const int nq = (int)gridDim.x < kBatchQueues ? (int)gridDim.x : kBatchQueues;
+    const int own = (int)blockIdx.x % nq;
+    for (int r = 0; r < nq; ++r) {
+        const int q = (own + r) % nq;
+        const int q0 = (int)((long long)total * q / nq), q1 = (int)((long long)total * (q + 1) / nq);
+        for (;;) {
+            if (threadIdx.x == 0) item = q0 + atomicAdd(&counter[q], 1);
+            __syncthreads();
+            const int idx = item;
+            __syncthreads();  // every thread has read `item` before it is overwritten
+            if (idx >= q1) break;
+            const int2 w = queue[idx];
+            const SpmmDesc d = descs[w.x];
+            const int c0 = w.y;
+            const int g = d.nchunks - c0 < kWinChunks ? d.nchunks - c0 : kWinChunks;
+            // the window: W rows of X centred on the group's rows (scaled to the columns)
+            const int W = win_rows < d.N ? win_rows : d.N;
+            const long long rmid = ((long long)d.chunk_rows[c0] + d.chunk_rows[c0 + g]) / 2;
+            long long w0 = (d.M > 0 ? rmid * d.N / d.M : 0) - W / 2;
+            if (w0 > (long long)d.N - W) w0 = (long long)d.N - W;
+            if (w0 < 0) w0 = 0;
+            const gptr<const float> X = to_global(d.X);
+            for (int t = threadIdx.x; t < 2 * W; t += blockDim.x) {
+                const int row = t >> 1, j0 = 4 * (t & 1);
+                const gptr<const float> p = X + (size_t)(w0 + row) * d.k + j0;
+                float4 v;
+                if (d.k == KP) {
+                    const vfloat4 qv = *reinterpret_cast<gptr<const vfloat4>>(p);
+                    v = make_float4(qv.x, qv.y, qv.z, qv.w);
+                } else {
+                    v = xpart_partial(p, j0, d.k);
+                }
+                *reinterpret_cast<vfloat4 *>(win + (size_t)row * KP + j0) = vfloat4{v.x, v.y, v.z, v.w};
+            }
+            __syncthreads();
+            double *slots = smem + (size_t)slot * slot_doubles;
+            lds_cf32 *lwin = (lds_cf32 *)win;
+            const int ltid = threadIdx.x & 127;
+            if (slot < g) {
+                if (d.col16)
+                    spmm_chunk<VT, KP, true, true>(d, c0 + slot, wv & 1, lane, slots, &segbuf[wv][0], lwin, (int)w0,
+                                                   W, ltid, 128);
+                else
+                    spmm_chunk<VT, KP, false, true>(d, c0 + slot, wv & 1, lane, slots, &segbuf[wv][0], lwin, (int)w0,
+                                                    W, ltid, 128);
+            } else {
+                __syncthreads();  // the two workgroup barriers of spmm_chunk
+                __syncthreads();
+            }
+            __syncthreads();  // the window is refilled for the next group
+        }
+    }
+}
 
 // Cross-chunk rows of one matrix, all k columns; chunk order.
 __device__ __forceinline__ void spmm_fix_chunk(const SpmmDesc &a, int c) {
@@ -479,6 +570,68 @@ hipError_t launch_spmm_batch(const SpmmDesc *d_descs, const int2 *d_queue, int t
         PAVA_BATCH(unsigned short)
     }
 #undef PAVA_BATCH
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(spmm_batch_fixup_kernel, dim3((total + 255) / 256), dim3(256), 0, st, d_descs, d_queue, total,
+                       d_counter);
+    return hipGetLastError();
+}
+
+template <typename VT>
+static const void *win_kernel_ptr() {
+    return reinterpret_cast<const void *>(&spmm_batch_win_kernel<VT>);
+}
+static const void *win_kernel_for(int prec) {
+    return prec == 1 ? win_kernel_ptr<float>() : prec == 0 ? win_kernel_ptr<double>() : win_kernel_ptr<unsigned short>();
+}
+
+hipError_t spmm_batch_win_plan(int prec, int max_slots, int wgs_per_cu, int *win_rows, size_t *lds, int *grid) {
+    *win_rows = 0;
+    *lds = 0;
+    *grid = 0;
+    if (wgs_per_cu < 1) return hipSuccess;
+    const void *f = win_kernel_for(prec);
+    hipError_t e;
+    int dev = 0, max_lds = 0, ncu = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev)) != hipSuccess)
+        return e;
+    if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    hipFuncAttributes fa{};
+    if ((e = hipFuncGetAttributes(&fa, f)) != hipSuccess) return e;
+    const long long slot_bytes = (long long)kWinChunks * (max_slots > 0 ? max_slots : 1) * 8 * 8;
+    const long long budget = (long long)max_lds / wgs_per_cu - (long long)fa.sharedSizeBytes - slot_bytes;
+    long long rows = budget / (8 * 4);
+    if (rows > kWinMaxRows) rows = kWinMaxRows;
+    rows = rows / 64 * 64;
+    if (rows < kWinMinRows) return hipSuccess;
+    const size_t bytes = (size_t)slot_bytes + (size_t)rows * 8 * 4;
+    if ((e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)) != hipSuccess) return e;
+    int per_cu = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64 * 2 * kWinChunks, bytes)) != hipSuccess)
+        return e;
+    if (per_cu < 1) return hipSuccess;
+    *win_rows = (int)rows;
+    *lds = bytes;
+    *grid = per_cu * (ncu > 0 ? ncu : 1);
+    return hipSuccess;
+}
+
+hipError_t launch_spmm_batch_win(const SpmmDesc *d_descs, const int2 *d_gqueue, int total_g, const int2 *d_queue,
+                                 int total, int *d_counter, int prec, int max_slots, int win_rows, size_t lds, int grid,
+                                 hipStream_t st) {
+    if (total == 0) return hipSuccess;
+    const int slot_doubles = (max_slots > 0 ? max_slots : 1) * 8;
+    dim3 g(grid), b(64 * 2 * kWinChunks);
+    if (prec == 1)
+        hipLaunchKernelGGL(spmm_batch_win_kernel<float>, g, b, lds, st, d_descs, d_gqueue, total_g, d_counter,
+                           slot_doubles, win_rows);
+    else if (prec == 0)
+        hipLaunchKernelGGL(spmm_batch_win_kernel<double>, g, b, lds, st, d_descs, d_gqueue, total_g, d_counter,
+                           slot_doubles, win_rows);
+    else
+        hipLaunchKernelGGL(spmm_batch_win_kernel<unsigned short>, g, b, lds, st, d_descs, d_gqueue, total_g,
+                           d_counter, slot_doubles, win_rows);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(spmm_batch_fixup_kernel, dim3((total + 255) / 256), dim3(256), 0, st, d_descs, d_queue, total,

@@ -242,12 +242,13 @@ def gen_params(rng: np.random.Generator, max_rows: int = 1_000_000, min_rows: in
 
 # ------------------------------------------------------- batched SpMM tiles
 # Candidate tiles of a batched multi-RHS SpMM, one choice per batch: CVR steps
-# per chunk S, and the column width (0: the library's automatic width, 16-bit
+# per chunk S; the column width (0: the library's automatic width, 16-bit
 # chunk-relative columns wherever a chunk spans fewer than 65,536 columns; 32:
-# 32-bit columns). X is read in place (no hot-x copy: a staged copy of X rows
-# would add an indirection to every nonzero without saving a line).
-BATCH_TILES = [(s, bits) for s in (16, 32, 64, 128, 256) for bits in (0, 32)]
-BATCH_FIXED = (64, 0)  # the tile the batched path used before it was autotuned
+# 32-bit columns); and the X staging: 0 reads X rows in place, 1 or 2 stage
+# the X rows around each group of 4 chunks in an LDS window sized for 1 or 2
+# workgroups per CU (k <= 8 only).
+BATCH_TILES = [(s, bits, win) for s in (16, 32, 64, 128, 256) for bits in (0, 32) for win in (0, 1, 2)]
+BATCH_FIXED = (64, 0, 0)  # the tile the batched path used before it was autotuned
 
 
 def batch_family(nmat: int = 512, seed: int = 4, threads: int = 8):
@@ -285,9 +286,10 @@ def build_batch_handles(mats, S: int, bits: int):
     return hs
 
 
-def time_batch(hs, mats, k: int, iters: int = 10, seed: int = 0) -> float:
+def time_batch(hs, mats, k: int, iters: int = 10, seed: int = 0, win: int = 0) -> float:
     """Milliseconds per replay of the captured persistent batched SpMM over
-    handles hs (FP32 X blocks of k columns, uniform [-1, 1])."""
+    handles hs (FP32 X blocks of k columns, uniform [-1, 1]), with X window
+    mode `win`."""
     import torch
     from . import Batch
     g = torch.Generator(device="cuda")
@@ -296,6 +298,8 @@ def time_batch(hs, mats, k: int, iters: int = 10, seed: int = 0) -> float:
     Ys = [torch.empty((A.numRows, k), dtype=torch.float32, device="cuda") for A in mats]
     b = Batch(hs, k)
     try:
+        if win:
+            b.set_x_window(win)
         b.set_operands([t.data_ptr() for t in Xs], [t.data_ptr() for t in Ys])
         st = torch.cuda.Stream()
         b.capture(st.cuda_stream)
@@ -314,21 +318,27 @@ def time_batch(hs, mats, k: int, iters: int = 10, seed: int = 0) -> float:
 
 def autotune_batch(mats, k: int, tiles=BATCH_TILES, sample: int = 64, iters: int = 10, seed: int = 0):
     """Exhaustive tile search for a batched SpMM: every candidate (S, column
-    width) is built and timed as one persistent launch over a seeded sample of
-    the batch's matrices. Returns (best tile, {tile: ms per launch on the
-    sample}); inf where a candidate cannot be built."""
+    width, X staging) is built and timed as one persistent launch over a
+    seeded sample of the batch's matrices. Returns (best tile, {tile: ms per
+    launch on the sample}); inf where a candidate does not apply (X windows
+    beyond k = 8, a layout that cannot be built)."""
     rng = np.random.default_rng(seed)
     pick = np.sort(rng.choice(len(mats), size=min(sample, len(mats)), replace=False))
     sub = [mats[int(i)] for i in pick]
     times = {}
-    for S, bits in tiles:
+    for S, bits in dict.fromkeys((t[0], t[1]) for t in tiles):
+        wins = [t[2] for t in tiles if (t[0], t[1]) == (S, bits)]
         try:
             hs = build_batch_handles(sub, S, bits)
         except PavaError:
-            times[(S, bits)] = float("inf")
+            times.update({(S, bits, w): float("inf") for w in wins})
             continue
         try:
-            times[(S, bits)] = time_batch(hs, sub, k, iters=iters, seed=seed)
+            for w in wins:
+                try:
+                    times[(S, bits, w)] = time_batch(hs, sub, k, iters=iters, seed=seed, win=w)
+                except PavaError:
+                    times[(S, bits, w)] = float("inf")
         finally:
             for h in hs:
                 h.close()

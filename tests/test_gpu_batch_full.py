@@ -44,9 +44,11 @@ def test_batch_family_full_size(P, O, family, k):
     tile, tune = SEL.autotune_batch(mats, k)
     _say(f"k={k}: autotuned tile {tile} from " + ", ".join(f"{t}: {v:.3f} ms" for t, v in tune.items()))
     assert np.isfinite(tune[tile]) and tune[tile] == min(tune.values())
-    hs = SEL.build_batch_handles(mats, *tile)
+    hs = SEL.build_batch_handles(mats, tile[0], tile[1])
     b = P.Batch(hs, k)
     try:
+        if tile[2]:
+            b.set_x_window(tile[2])
         g = torch.Generator(device="cuda")
         g.manual_seed(k)
         Xs = [torch.rand((A.numCols, k), dtype=torch.float32, device="cuda", generator=g) * 2 - 1 for A in mats]

@@ -33,7 +33,7 @@ def check_queue_counts(b):
     pops per queue range = range size + one empty pop per workgroup."""
     counts, nq = b.queue_counts()
     inf = b.info()
-    total, grid = inf["chunks"], inf["grid"]
+    total, grid = inf["items"], inf["grid"]
     assert nq == min(grid, 8)
     for q in range(8):
         want = total * (q + 1) // nq - total * q // nq + grid if q < nq else 0
@@ -147,7 +147,8 @@ def test_spmm_bit_exact_vs_cvr_order(P, O, k, bits):
             torch.cuda.synchronize()
             Y = dY.cpu().numpy()
         lay = O.cvr64_build(A, S, 0)
-        want = np.stack([O.cvr64_spmv(A, lay, X[:, j].astype(np.float64)) for j in range(k)], 1).astype(np.float32)
+        want = np.stack([O.cvr64_spmv(A, lay, X[:, j].astype(np.float64), parity_tol=0.0) for j in range(k)],
+                        1).astype(np.float32)
         assert np.array_equal(Y, want), name
 
 
@@ -220,7 +221,8 @@ def test_seeded_sweep_spmm_and_batch(P, O):
         torch.cuda.synchronize()
         Y = dY.cpu().numpy()
         lay = O.cvr64_build(A, S, 0)
-        want = np.stack([O.cvr64_spmv(A, lay, X[:, j].astype(np.float64)) for j in range(k)], 1).astype(np.float32)
+        want = np.stack([O.cvr64_spmv(A, lay, X[:, j].astype(np.float64), parity_tol=0.0) for j in range(k)],
+                        1).astype(np.float32)
         _say(f"spmm sweep {i}: M={M} N={N} nnz={A.nnz} S={S} bits={h.info()['col_index_bits']} k={k}")
         assert np.array_equal(Y, want), i
         mats.append(A)
@@ -236,5 +238,70 @@ def test_seeded_sweep_spmm_and_batch(P, O):
     for o, (_, Y) in zip(outs, Ys):
         assert np.array_equal(o.cpu().numpy(), Y)
     b.close()
+    for h in hs:
+        h.close()
+
+
+@pytest.mark.parametrize("k", [3, 8])
+def test_batch_x_window_bit_identical(P, O, k):
+    """The LDS window of X (k <= 8; sized for 1 or 2 workgroups per CU) changes
+    where X rows are read, not the sums: Y equals the batch without it bit for
+    bit, every group of chunks is handed out once, and graph replays
+    reproduce it. The matrices include ones wider than the window, narrower
+    than it, non-square, with 16- and 32-bit columns and with matrix ends that
+    leave groups of fewer than 4 chunks."""
+    import torch
+    rng = np.random.default_rng(500 + k)
+    mats = []
+    for i, (m, n, hb) in enumerate([(3000, 3000, 200), (20000, 20000, 1000), (50000, 50000, 3000), (700, 700, 700),
+                                     (12000, 30000, 1500), (30000, 9000, 500), (5000, 5000, 5000)]):
+        A = P.synthetic(m, P.RL_POISSON, float(rng.uniform(5, 40)), col_struct=P.COL_MIXED, half_band=hb,
+                        band_frac=0.7, seed=700 + i)
+        if n != m:  # a non-square matrix of the same row structure
+            A = P.CSR(m, n, A.rowOffsets, (A.cols.astype(np.int64) * n // m).astype(np.int32), A.vals)
+        mats.append(_f32_matrix(A))
+    hs = [P.CvrHip(A, steps_per_chunk=int(rng.choice([8, 24, 64])), precision=P.PREC_F32,
+                   col_index_bits=int(rng.choice([0, 32])), x_hot_kb=0) for A in mats]
+    Xs = [torch.from_numpy(rng.uniform(-1, 1, (A.numCols, k)).astype(np.float32)).cuda() for A in mats]
+
+    def run(win):
+        b = P.Batch(hs, k)
+        if win:
+            b.set_x_window(win)
+        Ys = [torch.full((A.numRows, k), float("nan"), dtype=torch.float32, device="cuda") for A in mats]
+        b.set_operands([x.data_ptr() for x in Xs], [y.data_ptr() for y in Ys])
+        s = torch.cuda.Stream()
+        b.spmm(s.cuda_stream)
+        s.synchronize()
+        check_queue_counts(b)
+        first = [y.cpu().numpy() for y in Ys]
+        b.capture(s.cuda_stream)
+        for rep in range(2):
+            for y in Ys:
+                y.fill_(float("nan"))
+            torch.cuda.synchronize()
+            b.replay(s.cuda_stream)
+            s.synchronize()
+            check_queue_counts(b)
+            for Y0, y in zip(first, Ys):
+                assert np.array_equal(Y0, y.cpu().numpy()), (win, rep)
+        inf = b.info()
+        b.close()
+        return first, inf
+
+    base, inf0 = run(0)
+    for A, X, Y in zip(mats, Xs, base):
+        yref = _oracle_mm(O, A, X.cpu().numpy())
+        assert np.all(np.abs(Y.astype(np.float64) - yref) <= _bound(O, A, X.cpu().numpy(), yref))
+    for win in (1, 2):
+        got, inf = run(win)
+        _say(f"k={k} window {win}: {inf}")
+        assert inf["items"] == sum(-(-h.info()["nchunks"] // 4) for h in hs)
+        for Y0, Y in zip(base, got):
+            assert np.array_equal(Y0, Y), win
+    b16 = P.Batch(hs, 16)
+    with pytest.raises(P.PavaError):
+        b16.set_x_window(1)
+    b16.close()
     for h in hs:
         h.close()
