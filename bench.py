@@ -358,8 +358,19 @@ def run_mixed_spmv(P, A, x, sel, iters: int = 200) -> dict:
     from pava_amd import selector as SEL
     dev = torch.device("cuda:0")
     F = SEL.features(A)
-    pred = sel.mlp.predict(((F - sel.mu) / sel.sd)[None, :])[0]
-    order = [int(i) for i in np.argsort(pred)]
+    # exhaustive autotuning over the same configs: the judge of the
+    # prediction, and the fallback where the features leave the training range
+    tt = SEL.autotune(A, x, iters=20)
+    ib = int(np.argmin(tt))
+    covered = bool(sel.covers(F)[0])
+    if covered:
+        pred = sel.mlp.predict(((F - sel.mu) / sel.sd)[None, :])[0]
+        order = [int(i) for i in np.argsort(pred)]
+        how = "MLP"
+    else:
+        order = [ib]
+        out_of = [int(i) for i in np.nonzero((F < sel.lo) | (F > sel.hi))[0]]
+        how = f"exhaustive autotune (fallback: features {out_of} outside the training range)"
     yref = P.reference_spmv(A, x)
     x64 = torch.from_numpy(x).to(dev)
     x32 = x64.float()
@@ -390,10 +401,7 @@ def run_mixed_spmv(P, A, x, sel, iters: int = 200) -> dict:
         h0.spmv(x64.data_ptr(), y0.data_ptr(), st.cuda_stream)
         st.synchronize()
         err0 = error_stats(y0.cpu().numpy(), yref)
-    # exhaustive autotuning over the same configs: the judge of the prediction
-    tt = SEL.autotune(A, x, iters=20)
-    ib = int(np.argmin(tt))
-    auto = {"autotuned_best": SEL.CONFIGS[ib].name, "autotuned_best_ms": round(float(tt[ib]), 5),
+    auto = {"selected_by": how, "autotuned_best": SEL.CONFIGS[ib].name, "autotuned_best_ms": round(float(tt[ib]), 5),
             "selected_slowdown_vs_autotuned_best": round(float(tt[SEL.CONFIGS.index(cfg)] / tt[ib]), 3)}
     return {"rows": A.numRows, "nnz": A.nnz, "selected": cfg.name, **auto,
             "selected_nnz_per_s": A.nnz / (ms_sel * 1e-3), "selected_ms": round(ms_sel, 5),

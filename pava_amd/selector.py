@@ -192,10 +192,14 @@ def targets(times: np.ndarray):
 
 
 class Selector:
-    """Standardised features -> MLP -> argmin predicted slowdown (valid configs)."""
+    """Standardised features -> MLP -> argmin predicted slowdown (valid configs).
+    `lo`/`hi` bound each feature over the training matrices: outside them the
+    model extrapolates, and `covers` tells the caller to fall back to the
+    exhaustive autotuner."""
 
-    def __init__(self, mlp: Mlp, mu: np.ndarray, sd: np.ndarray):
+    def __init__(self, mlp: Mlp, mu: np.ndarray, sd: np.ndarray, lo=None, hi=None):
         self.mlp, self.mu, self.sd = mlp, mu, sd
+        self.lo, self.hi = lo, hi
 
     @staticmethod
     def fit(F: np.ndarray, times: np.ndarray, epochs: int = 300, seed: int = 0, lr: float = 2e-3) -> tuple:
@@ -204,7 +208,16 @@ class Selector:
         T, M = targets(times)
         mlp = Mlp(seed=seed)
         hist = mlp.train((F - mu) / sd, T, M, epochs=epochs, batch=64, lr=lr, seed=seed + 1)
-        return Selector(mlp, mu, sd), hist
+        return Selector(mlp, mu, sd, F.min(0), F.max(0)), hist
+
+    def covers(self, F: np.ndarray, margin: float = 0.05) -> np.ndarray:
+        """Per matrix: every feature within the training range widened by
+        `margin` of its width."""
+        F = np.atleast_2d(F)
+        if self.lo is None:
+            return np.ones(F.shape[0], bool)
+        w = (self.hi - self.lo) * margin
+        return np.all((F >= self.lo - w) & (F <= self.hi + w), axis=1)
 
     def choose(self, F: np.ndarray, valid: np.ndarray | None = None) -> np.ndarray:
         P = self.mlp.predict((np.atleast_2d(F) - self.mu) / self.sd)

@@ -85,3 +85,12 @@ def test_committed_selector_dataset_is_consistent():
         A = D.matrix(i)
         assert A.numRows == d["rows"][i] and A.nnz == d["nnz"][i]
         assert np.allclose(S.features(A), d["features"][i], rtol=1e-5, atol=1e-6)
+
+
+def test_selector_covers_only_the_training_range():
+    """Outside the per-feature training range (widened by 5% of its width) the
+    selector reports no cover, and its caller autotunes instead."""
+    from pava_amd import selector as S
+    sel = S.Selector(None, np.zeros(3), np.ones(3), lo=np.zeros(3), hi=np.array([1.0, 2.0, 4.0]))
+    F = np.array([[0.5, 1.0, 2.0], [1.06, 1.0, 2.0], [1.04, 2.09, -0.19], [0.5, 1.0, 4.21]])
+    assert sel.covers(F).tolist() == [True, False, True, False]
