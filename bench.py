@@ -167,17 +167,20 @@ def cpu_baseline(O, A, x, budget_s: float = 12.0):
 
 
 def run_precision(P, A, x, steps: int, S: int = 0) -> dict:
-    """FP64 vs FP32 / BF16 storage (FP32 compensated accumulation), with and
-    without FP64 recompute of rows whose cancellation ratio exceeds 64, and with
-    16-bit chunk-relative columns; per-row error vs the FP64 reference SpMV.
-    32-bit layouts use automatic hot-x staging, as the main run does."""
+    """FP64 (with the default serial parity, and without: parity_tol = 0) vs
+    FP32 / BF16 storage (FP32 compensated accumulation), with and without FP64
+    recompute of rows whose cancellation ratio exceeds 64, and with 16-bit
+    chunk-relative columns; per-row error vs the FP64 reference SpMV. 32-bit
+    layouts use automatic hot-x staging, as the main run does."""
     import torch
     dev = torch.device("cuda:0")
     yref = P.reference_spmv(A, x)
     x64 = torch.from_numpy(x).to(dev)
     x32 = x64.float()
     variants = [("f64", dict(precision=0, col_index_bits=32), 8, 4, 8),
+                ("f64_parity_off", dict(precision=0, col_index_bits=32, parity_tol=0.0), 8, 4, 8),
                 ("f64_col16", dict(precision=0, col_index_bits=16), 8, 2.125, 8),
+                ("f64_col16_parity_off", dict(precision=0, col_index_bits=16, parity_tol=0.0), 8, 2.125, 8),
                 ("f32", dict(precision=1, col_index_bits=32), 4, 4, 4),
                 ("f32_recompute", dict(precision=1, recompute=True, col_index_bits=32), 4, 4, 4),
                 ("f32_col16_recompute", dict(precision=1, recompute=True, col_index_bits=16), 4, 2.125, 4),
@@ -206,7 +209,7 @@ def run_precision(P, A, x, steps: int, S: int = 0) -> dict:
         out[name] = {"ms": round(tot, 5), "main_kernel_ms": round(main, 5), "GB_s": round(B / tot / 1e6, 1),
                      "nnz_per_s": A.nnz / (tot * 1e-3), "bytes": int(B), "max_rel": st["max_rel"],
                      "median_rel": st["median_rel"],
-                     "recomputed_rows": h.last_recompute_count() if rc else 0}
+                     "recomputed_rows": h.last_recompute_count() if rc or kw["precision"] == 0 else 0}
         h.close()
     base = out["f64"]["nnz_per_s"]
     for v in out.values():

@@ -255,42 +255,54 @@ static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d
     h->nchunks = nnz > 0 ? (int)((nnz + C - 1) / C) : 0;
     h->col16 = (o->col_index_bits == 16);
     h->par = (h->prec == CVR_HIP_PREC_F64 && o->parity_tol > 0.0 && h->d_rp && nnz > 0) ? 1 : 0;
+    // every chunk of Cs elements spans fewer than 65,536 columns?
+    auto spans_fit = [&](long long Cs, int ncs, int *fit) -> int {
+        if (N <= 65536) {
+            *fit = 1;
+            return 0;
+        }
+        int *d_flag = nullptr, flag = 1;
+        hipError_t e = dmalloc(&d_flag, 4);
+        if (e == hipSuccess) e = hipMemsetAsync(d_flag, 0, 4, h->stream);
+        if (e == hipSuccess) e = cvr::launch_chunk_span(d_ci, nnz, Cs, ncs, d_flag, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        dfree(d_flag);
+        if (e != hipSuccess) return hip_fail(e, "column span check");
+        *fit = flag == 0;
+        return 0;
+    };
+    // FP64 parity on a 16-bit layout: the longer automatic chunks, if 16-bit
+    // columns still fit them
+    auto try_parity_size = [&](int *fit) -> int {
+        *fit = 0;
+        if (!(h->par && h->auto_s && h->nchunks > 0)) return 0;
+        const int S2 = auto_steps(nnz, h->device, kAutoRoundsParity16);
+        if (S2 == h->S) return 0;
+        const long long C2 = 64LL * S2;
+        const int nc2 = (int)((nnz + C2 - 1) / C2);
+        int rc0 = spans_fit(C2, nc2, fit);
+        if (rc0) return rc0;
+        if (*fit) {
+            h->S = S2;
+            h->opts.steps_per_chunk = S2;
+            h->nchunks = nc2;
+        }
+        return 0;
+    };
+    int rc0;
     if (o->col_index_bits == 0 && h->nchunks > 0 && o->x_hot_kb <= 0) {
         // automatic: 16-bit chunk-relative columns when every chunk's nonzeros
         // span fewer than 65,536 columns (an explicit hot-x request keeps 32)
-        auto spans_fit = [&](long long Cs, int ncs, int *fit) -> int {
-            if (N <= 65536) {
-                *fit = 1;
-                return 0;
-            }
-            int *d_flag = nullptr, flag = 1;
-            hipError_t e = dmalloc(&d_flag, 4);
-            if (e == hipSuccess) e = hipMemsetAsync(d_flag, 0, 4, h->stream);
-            if (e == hipSuccess) e = cvr::launch_chunk_span(d_ci, nnz, Cs, ncs, d_flag, h->stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, h->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-            dfree(d_flag);
-            if (e != hipSuccess) return hip_fail(e, "column span check");
-            *fit = flag == 0;
-            return 0;
-        };
-        int fit = 0, rc0;
-        if (h->par && h->auto_s) {
-            // FP64 parity: the longer automatic chunks, if 16-bit columns still fit
-            const int S2 = auto_steps(nnz, h->device, kAutoRoundsParity16);
-            const long long C2 = 64LL * S2;
-            const int nc2 = (int)((nnz + C2 - 1) / C2);
-            if (S2 != h->S && (rc0 = spans_fit(C2, nc2, &fit))) return rc0;
-            if (S2 != h->S && fit) {
-                h->S = S2;
-                h->opts.steps_per_chunk = S2;
-                h->nchunks = nc2;
-            }
-        }
+        int fit = 0;
+        if ((rc0 = try_parity_size(&fit))) return rc0;
         if (!fit && (rc0 = spans_fit(64LL * h->S, h->nchunks, &fit))) return rc0;
         h->col16 = fit;
-        C = 64LL * h->S;
+    } else if (h->col16) {
+        int fit = 0;
+        if ((rc0 = try_parity_size(&fit))) return rc0;
     }
+    C = 64LL * h->S;
     if (!h->col16 && h->nchunks > 0 && (long long)N > kCol32Limit)
         return set_error(PAVA_HIP_ERANGE, "a chunk spans 65,536 columns or more and there are more than 2^30 columns: "
                                           "no CVR column layout fits");
