@@ -140,11 +140,19 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
             vv[k] = in ? vq[0][k] : 0.0;
             xx[k] = in ? xq[0][k] : 0.0;
         }
-        for (int l = 0; l < nl64; ++l) {
+        // a full round with the lane loop unrolled (no loop control between
+        // the hops of the chain), a partial last round rolled
+        auto hop = [&](int l) {
             double t = __builtin_fma(vv[0], xx[0], s);
 #pragma unroll
             for (int k = 1; k < K; ++k) t = __builtin_fma(vv[k], xx[k], t);
             s = readlane_f64(t, l);
+        };
+        if (nl64 == 64) {
+#pragma unroll
+            for (int l = 0; l < 64; ++l) hop(l);
+        } else {
+            for (int l = 0; l < nl64; ++l) hop(l);
         }
         // shift: gather round r + D + 1, load round r + D + 2
         if (j0 + R < e) {
