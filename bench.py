@@ -235,12 +235,13 @@ def _time_calls(fn, iters: int, stream) -> float:
 def run_batch(P, k: int, nmat: int, steps: int, check: int = 8, mats=None, tile=None) -> dict:
     """nmat matrices of the batching family times FP32 RHS blocks of k columns
     in one persistent launch replayed from a hipGraph, with the tile (steps
-    per chunk, column width, X staging) chosen by the batched autotuner on a seeded
-    sample of 64 of the matrices; beside it the same batch at the fixed tile
-    it used before (S = 64, automatic width), and the plain FP64 CVR-HIP SpMV
-    (library defaults) of every matrix, one launch each. With `tile` given
-    (S, bits, win) the autotuner and the fixed-tile comparison are skipped (a
-    profiler then sees only the full batch's launches)."""
+    per chunk, column width, X staging, SpMM engine) chosen by the batched
+    autotuner on a seeded sample of 64 of the matrices; beside it the same
+    batch at the fixed tile it used before (S = 64, automatic width, FMA
+    engine), the chosen tile on the other engine, and the plain FP64 CVR-HIP
+    SpMV (library defaults) of every matrix, one launch each. With `tile`
+    given (S, bits, win, engine) the autotuner and the comparisons are skipped
+    (a profiler then sees only the full batch's launches)."""
     import torch
     from pava_amd import selector as SEL
     dev = torch.device("cuda:0")
@@ -255,39 +256,42 @@ def run_batch(P, k: int, nmat: int, steps: int, check: int = 8, mats=None, tile=
     if not given:
         tile, tune = SEL.autotune_batch(mats, k)
     t_tune = time.perf_counter() - t1
-    print(f"[bench] batch k={k}: tile S={tile[0]} bits={tile[1]} x-window={tile[2]} ({t_tune:.0f}s)", file=sys.stderr,
-          flush=True)
+    print(f"[bench] batch k={k}: tile {SEL.tile_name(tile)} ({t_tune:.0f}s)", file=sys.stderr, flush=True)
     nnz = sum(A.nnz for A in mats)
     rows = sum(A.numRows for A in mats)
     cols = sum(A.numCols for A in mats)
     res = {"matrices": nmat, "k": k, "nnz": nnz, "rows": rows,
            "tile": {"steps_per_chunk": tile[0], "col_index_bits": "auto" if tile[1] == 0 else tile[1],
-                    "x_window_wgs_per_cu": tile[2]},
-           "autotune_ms_on_sample": {f"S{s}-{'auto' if b == 0 else b}-w{w}": round(v, 4)
-                                     for (s, b, w), v in tune.items()},
+                    "x_window_wgs_per_cu": tile[2], "engine": "mfma" if tile[3] == P.SPMM_MFMA else "fma"},
+           "autotune_ms_on_sample": {SEL.tile_name(t): round(v, 4) for t, v in tune.items()},
            "autotune_s": round(t_tune, 1)}
 
-    def batch_ms(S, bits, win):
+    def batch_ms(S, bits, win, eng):
         hs = SEL.build_batch_handles(mats, S, bits)
         try:
             b = P.Batch(hs, k)
+            b.set_engine(eng)
             if win:
                 b.set_x_window(win)
             inf = b.info()
             inf["col16_nnz"] = sum(A.nnz for A, h in zip(mats, hs) if h.info()["col_index_bits"] == 16)
             b.close()
-            return SEL.time_batch(hs, mats, k, iters=max(steps, 5), win=win), inf
+            return SEL.time_batch(hs, mats, k, iters=max(steps, 5), win=win, engine=eng), inf
         finally:
             for h in hs:
                 h.close()
 
     ms_batch, binf = batch_ms(*tile)
     ms_fixed = None if given else batch_ms(*SEL.BATCH_FIXED)[0]
+    # the chosen tile on the other engine (the X window runs on the FMA engine only)
+    other = (tile[0], tile[1], 0, P.SPMM_FMA if tile[3] == P.SPMM_MFMA else P.SPMM_MFMA)
+    ms_other = None if given else batch_ms(*other)[0]
     # accuracy of the tuned batch on `check` matrices: FP64 reference SpMV on the same FP32 inputs
     hs = SEL.build_batch_handles(mats[:check], tile[0], tile[1])
     errs = []
     try:
         b = P.Batch(hs, k)
+        b.set_engine(tile[3])
         if tile[2]:
             b.set_x_window(tile[2])
         g = torch.Generator(device="cuda")
@@ -331,6 +335,7 @@ def run_batch(P, k: int, nmat: int, steps: int, check: int = 8, mats=None, tile=
         "ms_batch": round(ms_batch, 4),
         "ms_batch_fixed_tile_S64": None if ms_fixed is None else round(ms_fixed, 4),
         "tuned_vs_fixed_tile": None if ms_fixed is None else round(ms_fixed / ms_batch, 3),
+        "ms_batch_other_engine": None if ms_other is None else {SEL.tile_name(other): round(ms_other, 4)},
         "GB_s": round(Bb / ms_batch / 1e6, 1), "nnz_rhs_per_s": nnz * k / (ms_batch * 1e-3),
         "ms_fp64_spmv_all": round(ms_fp64, 4), "fp64_nnz_per_s": nnz / (ms_fp64 * 1e-3),
         "speedup_nnz_rhs_vs_fp64_spmv": round((nnz * k / ms_batch) / (nnz / ms_fp64), 2),
@@ -440,7 +445,7 @@ def run_mixed_legs(P, A, x, args) -> dict:
         "selected_median_rel": b["median_rel"], "error_checked": b["error_checked"],
         "fp64_default_nnz_per_s": b["fp64_nnz_per_s"], "ratio": b["speedup_nnz_rhs_vs_fp64_spmv"],
         "ms_batch": b["ms_batch"], "ms_batch_fixed_tile_S64": b["ms_batch_fixed_tile_S64"],
-        "ms_fp64_spmv_all": b["ms_fp64_spmv_all"]}
+        "ms_batch_other_engine": b["ms_batch_other_engine"], "ms_fp64_spmv_all": b["ms_fp64_spmv_all"]}
     out["legs"] = legs
     out["note"] = ("ratio = nonzeros (x right-hand sides) per second of the selected path over the plain FP64 "
                    "default on the same matrices; a batch leg counts nnz x k products against one FP64 SpMV "
@@ -467,8 +472,8 @@ def main():
     ap.add_argument("--k", type=int, default=8, help="batch variant: right-hand sides")
     ap.add_argument("--nmat", type=int, default=512, help="batch variant: matrices")
     ap.add_argument("--batch-tile", default="",
-                    help="batch variant: fixed tile S:bits:win (bits 0 = automatic width; win = X window "
-                         "workgroups per CU, 0 = off) instead of the autotuner")
+                    help="batch variant: fixed tile S:bits:win[:engine] (bits 0 = automatic width; win = X "
+                         "window workgroups per CU, 0 = off; engine 0 = FMA, 1 = MFMA) instead of the autotuner")
     ap.add_argument("--x-hot-kb", type=int, default=-1,
                     help="hot-x staging buffer (KiB; 0 = off; -1 = automatic: 2048 KiB when column use is skewed): "
                          "the most used columns read from a dense copy")
@@ -499,6 +504,8 @@ def main():
 
     if args.variant == "batch":
         tile = tuple(int(v) for v in args.batch_tile.split(":")) if args.batch_tile else None
+        if tile is not None and len(tile) == 3:
+            tile = tile + (P.SPMM_FMA,)
         res = run_batch(P, args.k, args.nmat, args.steps, tile=tile)
         print(json.dumps({"variant": "batch", **res}), flush=True)
         return

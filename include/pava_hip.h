@@ -246,6 +246,17 @@ int cvr_hip_spmv_mixed(cvr_hip_handle *h, const float *d_x32, const double *d_x6
  * make one call before capturing the call in a graph. cvr_hip_spmv and
  * cvr_hip_spmv_mixed enqueue kernels only and can be captured at any time. */
 int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d_Y, void *stream);
+/* SpMM engines (no counterpart in pava, which is single-RHS): per-lane FP64
+ * FMA on the vector ALUs (PAVA_HIP_SPMM_FMA, the default), or the matrix cores
+ * (PAVA_HIP_SPMM_MFMA): v_mfma_f64_4x4x4f64 over windows of 4 steps of 4 CVR
+ * lanes, each nonzero scaling a tile of its X row into its row segment's
+ * accumulator. Both accumulate exact products in FP64 and round once to FP32;
+ * they may differ only in the order of the additions inside a window. The
+ * engine applies to cvr_hip_spmm and cvr_hip_spmm_noalloc on this handle;
+ * EINVAL for another value. */
+#define PAVA_HIP_SPMM_FMA 0
+#define PAVA_HIP_SPMM_MFMA 1
+int cvr_hip_set_spmm_engine(cvr_hip_handle *h, int engine);
 /* cvr_hip_spmm without any allocation. No counterpart in the reference: it is
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, serving loops, harnesses that forbid hipMalloc). Same kernels,
@@ -281,6 +292,12 @@ int cvr_hip_batch_spmm(cvr_hip_batch *b, void *stream);
  * ERANGE if no window of 256 rows fits. Drops a captured graph: call
  * cvr_hip_batch_graph again after it. */
 int cvr_hip_batch_set_x_window(cvr_hip_batch *b, int wgs_per_cu);
+/* The batch's SpMM engine (PAVA_HIP_SPMM_FMA, the default, or
+ * PAVA_HIP_SPMM_MFMA; see cvr_hip_set_spmm_engine). The MFMA engine does not
+ * take the X window (EINVAL with a window on, and the window EINVAL with the
+ * MFMA engine). Drops a captured graph: call cvr_hip_batch_graph again after
+ * it. */
+int cvr_hip_batch_set_engine(cvr_hip_batch *b, int engine);
 /* Captures one batched SpMM on `stream` (non-null) into a hipGraph. */
 int cvr_hip_batch_graph(cvr_hip_batch *b, void *stream);
 int cvr_hip_batch_replay(cvr_hip_batch *b, void *stream);

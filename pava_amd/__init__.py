@@ -27,6 +27,7 @@ _HERE = Path(__file__).resolve().parent
 LIB_PATH = _HERE / "libpava_hip.so"
 
 PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
+SPMM_FMA, SPMM_MFMA = 0, 1  # SpMM engines: per-lane FP64 FMA, or the matrix cores
 SPLIT_CAP = 72
 SEG_PER_CHUNK_EXTRA = 66
 
@@ -88,6 +89,7 @@ EXPORTED = [
     "csr5_hip_get_info", "csr5_hip_export_layout", "csr5_hip_destroy", "conduct_csr5_hip", "coo_hip_create",
     "coo_hip_spmv", "coo_hip_get_info", "coo_hip_destroy", "conduct_coo_hip", "cvr_hip_batch_queue_counts",
     "pava_hip_test_fail_alloc", "pava_hip_test_live_device_allocs", "cvr_hip_spmm_noalloc",
+    "cvr_hip_set_spmm_engine", "cvr_hip_batch_set_engine",
 ]
 
 
@@ -139,6 +141,8 @@ def lib() -> C.CDLL:
     L.cvr_hip_spmv_mixed.argtypes = [P, P, P, P, P]
     L.cvr_hip_spmm.argtypes = [P, i, P, P, P]
     L.cvr_hip_spmm_noalloc.argtypes = [P, i, P, P, P]
+    L.cvr_hip_set_spmm_engine.argtypes = [P, i]
+    L.cvr_hip_batch_set_engine.argtypes = [P, i]
     L.cvr_hip_last_recompute_count.argtypes = [P, C.POINTER(i)]
     L.cvr_hip_time_spmv.argtypes = [P, P, P, i, i, C.POINTER(d), C.POINTER(d)]
     L.cvr_hip_time_spmv_x64.argtypes = [P, P, P, P, i, i, C.POINTER(d), C.POINTER(d)]
@@ -376,6 +380,10 @@ class CvrHip:
     def spmm(self, k: int, X_ptr: int, Y_ptr: int, stream: int = 0) -> None:
         _check(lib().cvr_hip_spmm(self._h, k, X_ptr, Y_ptr, stream))
 
+    def set_spmm_engine(self, engine: int) -> None:
+        """SPMM_FMA (per-lane FP64 FMA, the default) or SPMM_MFMA (matrix cores)."""
+        _check(lib().cvr_hip_set_spmm_engine(self._h, engine))
+
     def time_spmv(self, x_ptr: int, y_ptr: int, warmup: int = 10, iters: int = 100, x64_ptr: int = 0):
         """(ms per SpMV, ms of the main kernel alone), hipEvents on the handle's stream."""
         tot, main = C.c_double(), C.c_double()
@@ -460,6 +468,10 @@ class Batch:
 
     def replay(self, stream: int) -> None:
         _check(lib().cvr_hip_batch_replay(self._b, stream))
+
+    def set_engine(self, engine: int) -> None:
+        """SPMM_FMA (the default) or SPMM_MFMA for the batch's launch."""
+        _check(lib().cvr_hip_batch_set_engine(self._b, engine))
 
     def set_x_window(self, wgs_per_cu: int) -> None:
         """LDS window of X (k <= 8) sized for wgs_per_cu workgroups per CU; 0 = off."""

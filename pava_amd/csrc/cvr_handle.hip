@@ -37,6 +37,7 @@ struct cvr_hip_handle {
     int *d_fix = nullptr;
     double *d_carry = nullptr, *d_carry_abs = nullptr;
     double *d_carry_mm = nullptr;  // SpMM carries: 2 * nchunks * 32
+    int spmm_engine = 0;           // cvr::kSpmmFma or cvr::kSpmmMfma (cvr_hip_set_spmm_engine)
     // FP64 serial parity (opts.parity_tol > 0): bound terms of the carries
     int par = 0;
     double2 *d_carry_bnd = nullptr;
@@ -984,7 +985,17 @@ extern "C" int cvr_hip_spmm(cvr_hip_handle *h, int k, const float *d_X, float *d
         HIP_TRY(cvr::launch_zero_f32(d_Y, (long long)h->M * k, (hipStream_t)stream));
         return 0;
     }
-    HIP_TRY(cvr::launch_spmm(a, h->prec, h->max_slots, (hipStream_t)stream));
+    HIP_TRY(cvr::launch_spmm(a, h->prec, h->max_slots, h->spmm_engine, (hipStream_t)stream));
+    return 0;
+}
+
+static_assert(PAVA_HIP_SPMM_FMA == cvr::kSpmmFma && PAVA_HIP_SPMM_MFMA == cvr::kSpmmMfma, "engine codes");
+
+extern "C" int cvr_hip_set_spmm_engine(cvr_hip_handle *h, int engine) {
+    if (!h) return set_error(PAVA_HIP_EINVAL, "null handle");
+    if (engine != PAVA_HIP_SPMM_FMA && engine != PAVA_HIP_SPMM_MFMA)
+        return set_error(PAVA_HIP_EINVAL, "unknown SpMM engine %d", engine);
+    h->spmm_engine = engine;
     return 0;
 }
 
@@ -1039,7 +1050,7 @@ extern "C" int cvr_hip_spmm_noalloc(cvr_hip_handle *h, int k, const float *d_X, 
         HIP_TRY(cvr::launch_zero_f32(d_Y, (long long)h->M * k, (hipStream_t)stream));
         return 0;
     }
-    HIP_TRY(cvr::launch_spmm(a, h->prec, h->max_slots, (hipStream_t)stream));
+    HIP_TRY(cvr::launch_spmm(a, h->prec, h->max_slots, h->spmm_engine, (hipStream_t)stream));
     return 0;
 }
 
@@ -1054,6 +1065,7 @@ struct cvr_hip_batch {
     int2 *d_gqueue = nullptr;
     int total_g = 0, win = 0, win_rows = 0, win_grid = 0;
     size_t win_lds = 0;
+    int engine = 0;  // cvr::kSpmmFma or cvr::kSpmmMfma
     int *d_counter = nullptr;
     std::vector<cvr::SpmmDesc> descs;
     hipGraph_t graph = nullptr;
@@ -1125,7 +1137,7 @@ static int batch_create_body(int nmat, cvr_hip_handle *const *handles, int k, cv
         hipMemset(b->d_counter, 0, 256) != hipSuccess)
         return fail(set_error(PAVA_HIP_EDEVICE, "batch upload failed"));
     b->lds = cvr::spmm_lds_bytes(b->max_slots, k);
-    if (cvr::spmm_batch_grid(b->prec, k, b->lds, &b->grid) != hipSuccess)
+    if (cvr::spmm_batch_grid(b->prec, k, b->lds, b->engine, &b->grid) != hipSuccess)
         return fail(set_error(PAVA_HIP_EDEVICE, "occupancy query failed"));
     *out = guard.release();
     return 0;
@@ -1148,7 +1160,7 @@ static int batch_enqueue(cvr_hip_batch *b, hipStream_t st) {
                                            b->prec, b->max_slots, b->win_rows, b->win_lds, b->win_grid, st));
     else
         HIP_TRY(cvr::launch_spmm_batch(b->d_descs, b->d_queue, b->total, b->d_counter, b->prec, b->k, b->lds, b->grid,
-                                       st));
+                                       b->engine, st));
     return 0;
 }
 
@@ -1156,6 +1168,8 @@ extern "C" int cvr_hip_batch_set_x_window(cvr_hip_batch *b, int wgs_per_cu) {
     if (!b) return set_error(PAVA_HIP_EINVAL, "null batch");
     if (wgs_per_cu < 0 || wgs_per_cu > 4) return set_error(PAVA_HIP_EINVAL, "wgs_per_cu must be in [0, 4]");
     if (wgs_per_cu > 0 && b->k > 8) return set_error(PAVA_HIP_EINVAL, "the X window needs k <= 8, got %d", b->k);
+    if (wgs_per_cu > 0 && b->engine == cvr::kSpmmMfma)
+        return set_error(PAVA_HIP_EINVAL, "the X window runs on the FMA engine only");
     HIP_TRY(hipSetDevice(b->device));
     // a captured launch no longer matches: it is dropped
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -1176,6 +1190,25 @@ extern "C" int cvr_hip_batch_set_x_window(cvr_hip_batch *b, int wgs_per_cu) {
     b->win_rows = rows;
     b->win_lds = lds;
     b->win_grid = grid;
+    return 0;
+}
+
+extern "C" int cvr_hip_batch_set_engine(cvr_hip_batch *b, int engine) {
+    if (!b) return set_error(PAVA_HIP_EINVAL, "null batch");
+    if (engine != PAVA_HIP_SPMM_FMA && engine != PAVA_HIP_SPMM_MFMA)
+        return set_error(PAVA_HIP_EINVAL, "unknown SpMM engine %d", engine);
+    if (engine == PAVA_HIP_SPMM_MFMA && b->win > 0)
+        return set_error(PAVA_HIP_EINVAL, "the X window runs on the FMA engine only");
+    HIP_TRY(hipSetDevice(b->device));
+    int grid = 0;
+    HIP_TRY(cvr::spmm_batch_grid(b->prec, b->k, b->lds, engine, &grid));
+    // a captured launch no longer matches: it is dropped
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->engine = engine;
+    b->grid = grid;
     return 0;
 }
 

@@ -232,10 +232,13 @@ struct SpmmDesc {
 int spmm_kp(int k);
 hipError_t launch_zero_f32(float *y, long long n, hipStream_t st);
 size_t spmm_lds_bytes(int max_slots, int k);
-hipError_t launch_spmm(const SpmmDesc &a, int prec, int max_slots, hipStream_t st);
-hipError_t spmm_batch_grid(int prec, int k, size_t lds, int *grid);
+// SpMM engines: per-lane FP64 FMA (the default) or the matrix cores
+// (v_mfma_f64_4x4x4f64); same results within FP64 rounding.
+constexpr int kSpmmFma = 0, kSpmmMfma = 1;
+hipError_t launch_spmm(const SpmmDesc &a, int prec, int max_slots, int engine, hipStream_t st);
+hipError_t spmm_batch_grid(int prec, int k, size_t lds, int engine, int *grid);
 hipError_t launch_spmm_batch(const SpmmDesc *d_descs, const int2 *d_queue, int total, int *d_counter, int prec, int k,
-                             size_t lds, int grid, hipStream_t st);
+                             size_t lds, int grid, int engine, hipStream_t st);
 // The batched SpMM with an LDS window of X (k <= 8): workgroups of 8 waves
 // take groups of kWinChunks consecutive chunks of one matrix.
 constexpr int kWinChunks = 4;

@@ -40,6 +40,7 @@ __device__ __forceinline__ void wave_sync_lds_m() {
 // The layout, X and Y live in global memory; pointers read from a descriptor
 // in memory (the batched path) go through to_global.
 typedef float vfloat4 __attribute__((ext_vector_type(4)));  // 16-byte loads and stores of X and Y rows
+typedef float vfloat2 __attribute__((ext_vector_type(2)));  // 8-byte pieces (MFMA engine, k <= 8)
 // LDS pointers (the X window): address space 3, so reads are ds_read, not flat
 typedef __attribute__((address_space(3))) const float lds_cf32;
 typedef __attribute__((address_space(3))) const vfloat4 lds_cv4;
@@ -81,6 +82,54 @@ __device__ __forceinline__ unsigned long long group_step_mask(unsigned long long
     const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)em, i);
     const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(em >> 32), i);
     return ((unsigned long long)hi << 32) | lo;
+}
+
+// The end of a chunk, after its segments are stored: split rows, as (group,
+// column) pairs over the chunk's threads with each row's slots summed in
+// element order, then the empty rows of the chunk's row range.
+template <int KP>
+__device__ void spmm_chunk_tail(const SpmmDesc &a, const int c, const int zr, double *slots, const int ltid,
+                                const int lnt) {
+    __syncthreads();
+    const int tid = ltid >= 0 ? ltid : (int)threadIdx.x, nthreads = ltid >= 0 ? lnt : (int)blockDim.x;
+    const int k = a.k;
+    const int ns = a.split_n[c];
+    const int2 *tab = a.split_tab + (size_t)c * kSplitCap;
+    for (int t = tid; t < ns * KP; t += nthreads) {
+        const int gi = t / KP, j = t % KP;
+        if (j >= k) continue;
+        const int2 e = tab[gi];
+        const int se = tab[gi + 1].y;
+        double s = slots[(size_t)e.y * KP + j];
+        for (int q = e.y + 1; q < se; ++q) s += slots[(size_t)q * KP + j];
+        if (e.x >= 0)
+            a.Y[(size_t)e.x * k + j] = (float)s;
+        else if (e.x == kCodeHead)
+            a.carry[(size_t)(2 * c) * k + j] = s;
+        else if (e.x == kCodeTail)
+            a.carry[(size_t)(2 * c + 1) * k + j] = s;
+    }
+    // empty rows of the chunk's row range
+    if (a.has_empty) {
+        const int r0 = zr, r1 = a.chunk_rows[c + 1];
+        if (r1 > r0) {
+            const int wlo = r0 >> 5, whi = (r1 - 1) >> 5;
+            for (int w = wlo + tid; w <= whi; w += nthreads) {
+                unsigned bits = a.empty_bits[w];
+                const int rb = w << 5;
+                if (rb < r0) bits &= ~0u << (r0 - rb);
+                if (rb + 32 > r1) bits &= (r1 - rb >= 32) ? ~0u : ((1u << (r1 - rb)) - 1u);
+                while (bits) {
+                    const int jj = __ffs(bits) - 1;
+                    float *yr = a.Y + (size_t)(rb + jj) * k;
+                    for (int j = 0; j < k; ++j) yr[j] = 0.f;
+                    bits &= bits - 1;
+                }
+            }
+        }
+    }
+    // the slots are reused by the workgroup's next chunk (batched path)
+    __syncthreads();
 }
 
 // C16: 16-bit layouts (chunk-relative column offsets, end flags in a mask per
@@ -271,46 +320,244 @@ __device__ void spmm_chunk(const SpmmDesc &a, const int c, const int wv, const i
         wave_sync_lds_m();  // the next group overwrites segbuf
     }
 
-    // split rows: (group, column) pairs over the workgroup, slots summed in element order
-    __syncthreads();
-    const int tid = ltid >= 0 ? ltid : (int)threadIdx.x, nthreads = ltid >= 0 ? lnt : (int)blockDim.x;
-    const int ns = a.split_n[c];
-    const int2 *tab = a.split_tab + (size_t)c * kSplitCap;
-    for (int t = tid; t < ns * KP; t += nthreads) {
-        const int gi = t / KP, j = t % KP;
-        if (j >= k) continue;
-        const int2 e = tab[gi];
-        const int se = tab[gi + 1].y;
-        double s = slots[(size_t)e.y * KP + j];
-        for (int q = e.y + 1; q < se; ++q) s += slots[(size_t)q * KP + j];
-        if (e.x >= 0)
-            a.Y[(size_t)e.x * k + j] = (float)s;
-        else if (e.x == kCodeHead)
-            a.carry[(size_t)(2 * c) * k + j] = s;
-        else if (e.x == kCodeTail)
-            a.carry[(size_t)(2 * c + 1) * k + j] = s;
-    }
-    // empty rows of the chunk's row range
-    if (a.has_empty) {
-        const int r0 = zr, r1 = a.chunk_rows[c + 1];
-        if (r1 > r0) {
-            const int wlo = r0 >> 5, whi = (r1 - 1) >> 5;
-            for (int w = wlo + tid; w <= whi; w += nthreads) {
-                unsigned bits = a.empty_bits[w];
-                const int rb = w << 5;
-                if (rb < r0) bits &= ~0u << (r0 - rb);
-                if (rb + 32 > r1) bits &= (r1 - rb >= 32) ? ~0u : ((1u << (r1 - rb)) - 1u);
-                while (bits) {
-                    const int jj = __ffs(bits) - 1;
-                    float *yr = a.Y + (size_t)(rb + jj) * k;
-                    for (int j = 0; j < k; ++j) yr[j] = 0.f;
-                    bits &= bits - 1;
-                }
+    // split rows and empty rows
+    spmm_chunk_tail<KP>(a, c, zr, slots, ltid, lnt);
+}
+
+// ------------------------------------------------------------ MFMA engine
+// The multi-RHS chunk on the matrix cores. v_mfma_f64_4x4x4f64 computes, in
+// each of 4 blocks, D(4x4) += A(4x4) B(4x4) in FP64. Its lane layout
+// (measured on gfx950): block b = (lane >> 2) & 3; A[i][t] in lane
+// 16 t + 4 b + i, B[t][j] in lane 16 t + 4 b + j, D[i][j] in lane
+// 16 i + 4 b + j.
+//
+// One instruction takes a window of 4 steps of 4 CVR lanes (a quad), block b
+// serving CVR lane 4 q + b: B[t][j] is one column of the X row of that lane's
+// element at step t, and A[i][t] is the element's value in row i if the
+// element belongs to the segment held in row i (0 in the other rows). So each
+// nonzero scales a tile of its X row into its segment's row of D, and the
+// rows of D accumulate up to 4 segments; a quarter of the products are
+// nonzero. Segments take rows in turn (segment n of a CVR lane in row
+// n mod 4): the segment still open at a window's end keeps its row into the
+// next window, and a row is cleared once its segment is stored, so no data
+// moves between lanes.
+//
+// Right-hand-side columns: R = 2 (KP = 8) or 4 per lane and piece, KP / (4 R)
+// pieces. Instruction n = piece * R + g gives block column j the column
+// piece * 4 R + R j + g: a B lane loads R consecutive X values once per
+// window and a D lane stores R consecutive Y values.
+//
+// Waves: KP / 4 per chunk, as with the FMA engine; wave wv serves quads
+// [wv QW, wv QW + QW), QW = 64 / KP. Per window every wave reads the 4 steps'
+// columns of all 64 CVR lanes (their end flags rank the ending segments in the
+// chunk's segment list) and stages the decoded columns in LDS (`colbuf`, 256
+// ints), from which each B lane reads the column of its (step, CVR lane); the
+// next window's columns are in flight while the current window runs. X rows
+// and values are not double-buffered: without the second set of registers the
+// kernels fit 4 waves per SIMD for k > 8 (3 for k <= 8), and occupancy hides
+// the latency better (measured on the batching workload, k = 32: 2.70 ->
+// 1.72 ms per launch). `segbuf` (256 ints) holds the window's destinations.
+// Accumulation is FP64 over exact products of FP32 (or FP64) inputs, as in
+// the FMA engine; only the order of the four additions inside an instruction
+// may differ.
+template <typename VT, int KP, bool C16>
+__device__ void spmm_chunk_mfma(const SpmmDesc &a, const int c, const int wv, const int lane, double *slots,
+                                int *segbuf, int *colbuf) {
+    constexpr int QW = 64 / KP;        // quads per wave
+    constexpr int R = KP == 8 ? 2 : 4; // consecutive right-hand sides per lane and piece
+    constexpr int NPC = KP / (4 * R);  // pieces
+    constexpr int NI = NPC * R;        // instructions per quad and window (KP / 4)
+    constexpr int WS = 4;              // steps per window (the instruction's K)
+    const int lo = lane & 3, hi = lane >> 4;  // hi: step (A, B lanes) or row (D lanes)
+    const long long C = 64LL * a.S;
+    const size_t base = (size_t)C * (size_t)c;
+    const int steps = (c == a.nchunks - 1) ? (int)((a.layout_elems - (long long)base) >> 6) : a.S;
+    const int zr = a.chunk_rows[c];
+    const gptr<const int> segs = to_global(a.seg + zr + kSegExtra * c);
+    const gptr<const VT> vp = to_global(reinterpret_cast<const VT *>(a.vals) + base);
+    const gptr<const int> cpa = to_global(a.cols + base + lane);
+    const gptr<const unsigned short> cp16 =
+        to_global(reinterpret_cast<const unsigned short *>(a.cols) + base + lane);
+    const gptr<const uint64_t> emp = to_global(a.end_mask + (base >> 6));
+    const int cbase = C16 ? a.col_base[c] : 0;
+    const gptr<const float> X = to_global(a.X);
+    const gptr<const int> hot_cols = to_global(a.hot_cols);
+    const bool has_hot = a.hot_cols != nullptr;
+    const gptr<float> Y = to_global(a.Y);
+    const int k = a.k;
+    const bool full = (k == KP);
+    // CVR lane of this lane's block in quad t: 4 (wv QW + t) + b
+    const int lq0 = 4 * wv * QW + ((lane >> 2) & 3);
+
+    auto load_cols = [&](int w0, int (&cc)[WS], unsigned long long &em) {
+        if constexpr (C16) {
+#pragma unroll
+            for (int s = 0; s < WS; ++s) cc[s] = (int)cp16[(size_t)(w0 + s) * 64];
+            em = emp[w0 + (lane & (WS - 1))];
+        } else {
+#pragma unroll
+            for (int s = 0; s < WS; ++s) cc[s] = __builtin_nontemporal_load(&cpa[(size_t)(w0 + s) * 64]);
+        }
+    };
+    auto step_ends = [&](int cv, unsigned long long em, int s) -> unsigned long long {
+        if constexpr (C16)
+            return group_step_mask(em, s);
+        else
+            return __ballot(cv < 0);
+    };
+    // A window's end masks, destinations and (in flight) X tiles and values.
+    // sb: index of the window's first segment end in the chunk's list.
+    auto stage = [&](int w0, int sb, const int (&cc)[WS], unsigned long long em, unsigned long long (&m)[WS],
+                     int &ne, float (&xs)[QW][NI], VT (&vs)[QW], int (&sg)[WS]) {
+        ne = 0;
+#pragma unroll
+        for (int s = 0; s < WS; ++s) {
+            m[s] = step_ends(cc[s], em, s);
+            ne += __popcll(m[s]);
+        }
+        int dc[WS];
+#pragma unroll
+        for (int s = 0; s < WS; ++s) {
+            if constexpr (C16)
+                dc[s] = cc[s] + cbase;
+            else
+                dc[s] = cc[s] & kColMask;
+        }
+        if constexpr (!C16) {
+            if (has_hot) {
+#pragma unroll
+                for (int s = 0; s < WS; ++s)
+                    if (cc[s] & kHotFlag) dc[s] = hot_cols[dc[s]];
             }
         }
+#pragma unroll
+        for (int s = 0; s < WS; ++s) colbuf[s * 64 + lane] = dc[s];
+        wave_sync_lds_m();
+#pragma unroll
+        for (int t = 0; t < QW; ++t) {
+            const int L = lq0 + 4 * t;
+            const int col = colbuf[hi * 64 + L];
+            const gptr<const float> p = X + (size_t)col * k + R * lo;
+#pragma unroll
+            for (int pc = 0; pc < NPC; ++pc) {
+                const int j0 = pc * 4 * R + R * lo;
+                if (full) {
+                    if constexpr (R == 4) {
+                        const vfloat4 q = *reinterpret_cast<gptr<const vfloat4>>(p + pc * 4 * R);
+                        xs[t][pc * R + 0] = q.x;
+                        xs[t][pc * R + 1] = q.y;
+                        xs[t][pc * R + 2] = q.z;
+                        xs[t][pc * R + 3] = q.w;
+                    } else {
+                        const vfloat2 q = *reinterpret_cast<gptr<const vfloat2>>(p + pc * 4 * R);
+                        xs[t][pc * R + 0] = q.x;
+                        xs[t][pc * R + 1] = q.y;
+                    }
+                } else {
+#pragma unroll
+                    for (int g = 0; g < R; ++g) xs[t][pc * R + g] = j0 + g < k ? p[pc * 4 * R + g] : 0.f;
+                }
+            }
+            vs[t] = __builtin_nontemporal_load(&vp[(size_t)(w0 + hi) * 64 + L]);
+        }
+#pragma unroll
+        for (int j = 0; j < WS; ++j)
+            if (j * 64 < ne) sg[j] = __builtin_nontemporal_load(&segs[sb + j * 64 + lane]);
+    };
+
+    double D[QW][NI];
+    int sig[QW];  // segments ended so far by the CVR lane of this lane's block, per quad
+#pragma unroll
+    for (int t = 0; t < QW; ++t) {
+        sig[t] = 0;
+#pragma unroll
+        for (int n = 0; n < NI; ++n) D[t][n] = 0.0;
     }
-    // the slots are reused by the workgroup's next chunk (batched path)
-    __syncthreads();
+    unsigned long long mc[WS];
+    int nec = 0;
+    float xc[QW][NI];
+    VT vc[QW];
+    int sgc[WS];
+    int cn[WS];
+    unsigned long long en = 0;
+    load_cols(0, cn, en);
+    int segp = 0;  // segment ends before the current window
+
+    for (int w = 0; w < steps; w += WS) {
+        const bool has1 = (w + WS) < steps;
+        stage(w, segp, cn, en, mc, nec, xc, vc, sgc);
+        if (has1) load_cols(w + WS, cn, en);
+        // this window's destinations -> LDS
+#pragma unroll
+        for (int j = 0; j < WS; ++j)
+            if (j * 64 < nec) segbuf[j * 64 + lane] = sgc[j];
+        wave_sync_lds_m();
+        int pre[WS];
+        pre[0] = 0;
+#pragma unroll
+        for (int s = 1; s < WS; ++s) pre[s] = pre[s - 1] + __popcll(mc[s - 1]);
+        // end flags of this lane's CVR lanes (lq0 + 4 t), bit 4 t of ms[s]
+        unsigned ms[WS];
+#pragma unroll
+        for (int s = 0; s < WS; ++s) ms[s] = (unsigned)(mc[s] >> lq0);
+#pragma unroll
+        for (int t = 0; t < QW; ++t) {
+            const int L = lq0 + 4 * t;
+            int e4 = 0;
+#pragma unroll
+            for (int s = 0; s < WS; ++s) e4 |= (int)((ms[s] >> (4 * t)) & 1u) << s;
+            // A: the value of the element at step hi, in the row of its segment
+            const int rowA = (sig[t] + __popc(e4 & ((1 << hi) - 1))) & 3;
+            const double av = rowA == lo ? val_f64<VT>(vc[t]) : 0.0;
+#pragma unroll
+            for (int n = 0; n < NI; ++n)
+                D[t][n] = __builtin_amdgcn_mfma_f64_4x4x4f64(av, (double)xc[t][n], D[t][n], 0, 0, 0);
+            // D row hi holds a segment that ended in this window: store it, clear the row
+            const int ne = __popc(e4);
+            const int slot = (hi - sig[t]) & 3;
+            if (slot < ne) {
+                int e = e4;
+                for (int q = 0; q < slot; ++q) e &= e - 1;
+                const int ks = __ffs(e) - 1;
+                const unsigned long long mk = ks == 0 ? mc[0] : ks == 1 ? mc[1] : ks == 2 ? mc[2] : mc[3];
+                const int pk = ks == 0 ? pre[0] : ks == 1 ? pre[1] : ks == 2 ? pre[2] : pre[3];
+                const int d = segbuf[pk + __popcll(mk & ((1ull << L) - 1ull))];
+                if (d >= 0) {
+                    const gptr<float> yr = Y + (size_t)d * k;
+#pragma unroll
+                    for (int pc = 0; pc < NPC; ++pc) {
+                        const int j0 = pc * 4 * R + R * lo;
+                        if (full) {
+                            if constexpr (R == 4) {
+                                const vfloat4 q = {(float)D[t][pc * R + 0], (float)D[t][pc * R + 1],
+                                                   (float)D[t][pc * R + 2], (float)D[t][pc * R + 3]};
+                                *reinterpret_cast<gptr<vfloat4>>(yr + j0) = q;
+                            } else {
+                                const vfloat2 q = {(float)D[t][pc * R + 0], (float)D[t][pc * R + 1]};
+                                *reinterpret_cast<gptr<vfloat2>>(yr + j0) = q;
+                            }
+                        } else {
+#pragma unroll
+                            for (int g = 0; g < R; ++g)
+                                if (j0 + g < k) yr[j0 + g] = (float)D[t][pc * R + g];
+                        }
+                    }
+                } else {
+                    double *sl = slots + (size_t)(~d) * KP;
+#pragma unroll
+                    for (int pc = 0; pc < NPC; ++pc)
+#pragma unroll
+                        for (int g = 0; g < R; ++g) sl[pc * 4 * R + R * lo + g] = D[t][pc * R + g];
+                }
+#pragma unroll
+                for (int n = 0; n < NI; ++n) D[t][n] = 0.0;
+            }
+            sig[t] += ne;
+        }
+        segp += nec;
+        wave_sync_lds_m();  // the next window overwrites segbuf and colbuf
+    }
+    spmm_chunk_tail<KP>(a, c, zr, slots, -1, 0);
 }
 
 // Threads per workgroup: KP/4 waves, one chunk.
@@ -318,17 +565,30 @@ template <int KP>
 constexpr int spmm_threads() { return 16 * KP; }
 static inline int spmm_block(int kp) { return 16 * kp; }
 
-template <typename VT, int KP>
+// MF: the MFMA engine (spmm_chunk_mfma), else per-lane FMA (spmm_chunk). Each
+// wave's 64 * kStepGroup ints of segbuf: the FMA engine's group destinations,
+// or the MFMA engine's window destinations and staged columns.
+template <typename VT, int KP, bool MF>
 __device__ __forceinline__ void spmm_any_chunk(const SpmmDesc &a, int c, double *smem, int *segbuf) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (a.col16)
-        spmm_chunk<VT, KP, true>(a, c, wv, lane, smem, segbuf + wv * 64 * kStepGroup);
-    else
-        spmm_chunk<VT, KP, false>(a, c, wv, lane, smem, segbuf + wv * 64 * kStepGroup);
+    int *sb = segbuf + wv * 64 * kStepGroup;
+    if constexpr (MF) {
+        static_assert(64 * kStepGroup >= 2 * 256, "segbuf holds the window's destinations and columns");
+        if (a.col16)
+            spmm_chunk_mfma<VT, KP, true>(a, c, wv, lane, smem, sb, sb + 256);
+        else
+            spmm_chunk_mfma<VT, KP, false>(a, c, wv, lane, smem, sb, sb + 256);
+    } else {
+        if (a.col16)
+            spmm_chunk<VT, KP, true>(a, c, wv, lane, smem, sb);
+        else
+            spmm_chunk<VT, KP, false>(a, c, wv, lane, smem, sb);
+    }
 }
 
-template <typename VT, int KP>
-__global__ __launch_bounds__(spmm_threads<KP>()) void spmm_kernel(SpmmDesc a) {
+template <typename VT, int KP, bool MF>
+__global__ __launch_bounds__(spmm_threads<KP>()) __attribute__((amdgpu_waves_per_eu(MF && KP > 8 ? 4 : 1)))
+void spmm_kernel(SpmmDesc a) {
     extern __shared__ double smem[];
     __shared__ int segbuf[KP / 4][64 * kStepGroup];
     const int nb = gridDim.x;
@@ -336,7 +596,7 @@ __global__ __launch_bounds__(spmm_threads<KP>()) void spmm_kernel(SpmmDesc a) {
     const int q = nb >> 3, r = nb & 7, x = blockIdx.x & 7, i = blockIdx.x >> 3;
     const int c = (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
     if (c >= a.nchunks) return;
-    spmm_any_chunk<VT, KP>(a, c, smem, &segbuf[0][0]);
+    spmm_any_chunk<VT, KP, MF>(a, c, smem, &segbuf[0][0]);
 }
 
 // Persistent batched SpMM: workgroups pop (matrix, chunk) work items until the
@@ -345,10 +605,12 @@ __global__ __launch_bounds__(spmm_threads<KP>()) void spmm_kernel(SpmmDesc a) {
 // round-robin): a workgroup drains its own XCD's range first, so the chunks of
 // one matrix, and the rows of X they share, stay in one L2; then it helps
 // with the other ranges.
-template <typename VT, int KP>
-__global__ __launch_bounds__(spmm_threads<KP>()) void spmm_batch_kernel(const SpmmDesc *__restrict__ descs,
-                                                                        const int2 *__restrict__ queue, int total,
-                                                                        int *__restrict__ counter) {
+// (MFMA engine, k > 8: at least 4 waves per SIMD, i.e. 2 workgroups of 8 waves
+// per CU at k = 32.)
+template <typename VT, int KP, bool MF>
+__global__ __launch_bounds__(spmm_threads<KP>()) __attribute__((amdgpu_waves_per_eu(MF && KP > 8 ? 4 : 1)))
+void spmm_batch_kernel(const SpmmDesc *__restrict__ descs, const int2 *__restrict__ queue, int total,
+                       int *__restrict__ counter) {
     extern __shared__ double smem[];
     __shared__ int segbuf[KP / 4][64 * kStepGroup];
     __shared__ int item;
@@ -367,7 +629,7 @@ __global__ __launch_bounds__(spmm_threads<KP>()) void spmm_batch_kernel(const Sp
             // a register copy: fields read through a pointer into global
             // memory would be reloaded after every store (and be flat loads)
             const SpmmDesc d = descs[w.x];
-            spmm_any_chunk<VT, KP>(d, w.y, smem, &segbuf[0][0]);
+            spmm_any_chunk<VT, KP, MF>(d, w.y, smem, &segbuf[0][0]);
         }
     }
 }
@@ -482,13 +744,13 @@ __global__ void spmm_batch_fixup_kernel(const SpmmDesc *__restrict__ descs, cons
 }
 
 // ---------------------------------------------------------------- launchers
-template <typename VT>
+template <typename VT, bool MF>
 static hipError_t launch_spmm_vt(const SpmmDesc &a, int kp, size_t lds, hipStream_t st) {
     dim3 grid(a.nchunks), block(spmm_block(kp));
     switch (kp) {
-    case 8: hipLaunchKernelGGL((spmm_kernel<VT, 8>), grid, block, lds, st, a); break;
-    case 16: hipLaunchKernelGGL((spmm_kernel<VT, 16>), grid, block, lds, st, a); break;
-    default: hipLaunchKernelGGL((spmm_kernel<VT, 32>), grid, block, lds, st, a); break;
+    case 8: hipLaunchKernelGGL((spmm_kernel<VT, 8, MF>), grid, block, lds, st, a); break;
+    case 16: hipLaunchKernelGGL((spmm_kernel<VT, 16, MF>), grid, block, lds, st, a); break;
+    default: hipLaunchKernelGGL((spmm_kernel<VT, 32, MF>), grid, block, lds, st, a); break;
     }
     return hipGetLastError();
 }
@@ -511,37 +773,39 @@ hipError_t launch_zero_f32(float *y, long long n, hipStream_t st) {
 
 size_t spmm_lds_bytes(int max_slots, int k) { return (size_t)(max_slots > 0 ? max_slots : 1) * spmm_kp(k) * 8; }
 
-hipError_t launch_spmm(const SpmmDesc &a, int prec, int max_slots, hipStream_t st) {
+hipError_t launch_spmm(const SpmmDesc &a, int prec, int max_slots, int engine, hipStream_t st) {
     if (a.nchunks == 0) return hipSuccess;
     const int kp = spmm_kp(a.k);
     const size_t lds = spmm_lds_bytes(max_slots, a.k);
+    const bool mf = engine == kSpmmMfma;
     hipError_t e;
     if (prec == 0)
-        e = launch_spmm_vt<double>(a, kp, lds, st);
+        e = mf ? launch_spmm_vt<double, true>(a, kp, lds, st) : launch_spmm_vt<double, false>(a, kp, lds, st);
     else if (prec == 1)
-        e = launch_spmm_vt<float>(a, kp, lds, st);
+        e = mf ? launch_spmm_vt<float, true>(a, kp, lds, st) : launch_spmm_vt<float, false>(a, kp, lds, st);
     else
-        e = launch_spmm_vt<unsigned short>(a, kp, lds, st);
+        e = mf ? launch_spmm_vt<unsigned short, true>(a, kp, lds, st)
+               : launch_spmm_vt<unsigned short, false>(a, kp, lds, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(spmm_fixup_kernel, dim3((a.nchunks + 255) / 256), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 template <typename VT, int KP>
-static const void *batch_kernel_ptr() {
-    return reinterpret_cast<const void *>(&spmm_batch_kernel<VT, KP>);
+static const void *batch_kernel_ptr(bool mf) {
+    return mf ? reinterpret_cast<const void *>(&spmm_batch_kernel<VT, KP, true>)
+              : reinterpret_cast<const void *>(&spmm_batch_kernel<VT, KP, false>);
+}
+template <typename VT>
+static const void *batch_kernel_for(int kp, bool mf) {
+    return kp == 8 ? batch_kernel_ptr<VT, 8>(mf) : kp == 16 ? batch_kernel_ptr<VT, 16>(mf) : batch_kernel_ptr<VT, 32>(mf);
 }
 
-hipError_t spmm_batch_grid(int prec, int k, size_t lds, int *grid) {
+hipError_t spmm_batch_grid(int prec, int k, size_t lds, int engine, int *grid) {
     const int kp = spmm_kp(k);
-    const void *f = nullptr;
-    if (prec == 1)
-        f = kp == 8 ? batch_kernel_ptr<float, 8>() : kp == 16 ? batch_kernel_ptr<float, 16>() : batch_kernel_ptr<float, 32>();
-    else if (prec == 0)
-        f = kp == 8 ? batch_kernel_ptr<double, 8>() : kp == 16 ? batch_kernel_ptr<double, 16>() : batch_kernel_ptr<double, 32>();
-    else
-        f = kp == 8 ? batch_kernel_ptr<unsigned short, 8>()
-                    : kp == 16 ? batch_kernel_ptr<unsigned short, 16>() : batch_kernel_ptr<unsigned short, 32>();
+    const bool mf = engine == kSpmmMfma;
+    const void *f = prec == 1 ? batch_kernel_for<float>(kp, mf)
+                    : prec == 0 ? batch_kernel_for<double>(kp, mf) : batch_kernel_for<unsigned short>(kp, mf);
     int per_cu = 0, dev = 0, ncu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, spmm_block(kp), lds);
     if (e != hipSuccess) return e;
@@ -551,25 +815,31 @@ hipError_t spmm_batch_grid(int prec, int k, size_t lds, int *grid) {
     return hipSuccess;
 }
 
+template <typename VT, bool MF>
+static void launch_batch_vt(int kp, dim3 g, dim3 b, size_t lds, hipStream_t st, const SpmmDesc *d_descs,
+                            const int2 *d_queue, int total, int *d_counter) {
+    switch (kp) {
+    case 8: hipLaunchKernelGGL((spmm_batch_kernel<VT, 8, MF>), g, b, lds, st, d_descs, d_queue, total, d_counter); break;
+    case 16: hipLaunchKernelGGL((spmm_batch_kernel<VT, 16, MF>), g, b, lds, st, d_descs, d_queue, total, d_counter); break;
+    default: hipLaunchKernelGGL((spmm_batch_kernel<VT, 32, MF>), g, b, lds, st, d_descs, d_queue, total, d_counter); break;
+    }
+}
+
 hipError_t launch_spmm_batch(const SpmmDesc *d_descs, const int2 *d_queue, int total, int *d_counter, int prec, int k,
-                             size_t lds, int grid, hipStream_t st) {
+                             size_t lds, int grid, int engine, hipStream_t st) {
     if (total == 0) return hipSuccess;
     const int kp = spmm_kp(k);
-    dim3 g(grid), b(spmm_block(kp));
-#define PAVA_BATCH(VT)                                                                                          \
-    switch (kp) {                                                                                              \
-    case 8: hipLaunchKernelGGL((spmm_batch_kernel<VT, 8>), g, b, lds, st, d_descs, d_queue, total, d_counter); break;  \
-    case 16: hipLaunchKernelGGL((spmm_batch_kernel<VT, 16>), g, b, lds, st, d_descs, d_queue, total, d_counter); break; \
-    default: hipLaunchKernelGGL((spmm_batch_kernel<VT, 32>), g, b, lds, st, d_descs, d_queue, total, d_counter); break; \
-    }
-    if (prec == 1) {
-        PAVA_BATCH(float)
-    } else if (prec == 0) {
-        PAVA_BATCH(double)
-    } else {
-        PAVA_BATCH(unsigned short)
-    }
-#undef PAVA_BATCH
+    const bool mf = engine == kSpmmMfma;
+    const dim3 g(grid), b(spmm_block(kp));
+    if (prec == 1)
+        mf ? launch_batch_vt<float, true>(kp, g, b, lds, st, d_descs, d_queue, total, d_counter)
+           : launch_batch_vt<float, false>(kp, g, b, lds, st, d_descs, d_queue, total, d_counter);
+    else if (prec == 0)
+        mf ? launch_batch_vt<double, true>(kp, g, b, lds, st, d_descs, d_queue, total, d_counter)
+           : launch_batch_vt<double, false>(kp, g, b, lds, st, d_descs, d_queue, total, d_counter);
+    else
+        mf ? launch_batch_vt<unsigned short, true>(kp, g, b, lds, st, d_descs, d_queue, total, d_counter)
+           : launch_batch_vt<unsigned short, false>(kp, g, b, lds, st, d_descs, d_queue, total, d_counter);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(spmm_batch_fixup_kernel, dim3((total + 255) / 256), dim3(256), 0, st, d_descs, d_queue, total,

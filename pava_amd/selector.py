@@ -23,7 +23,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (CSR, Csr5Hip, CvrHip, CsrHip, PREC_F32, PREC_F64, PavaError, _check, lib)
+from . import (CSR, Csr5Hip, CvrHip, CsrHip, PREC_F32, PREC_F64, SPMM_FMA, SPMM_MFMA, PavaError, _check, lib)
 
 N_FEATURES = 16
 
@@ -257,11 +257,20 @@ def gen_params(rng: np.random.Generator, max_rows: int = 1_000_000, min_rows: in
 # Candidate tiles of a batched multi-RHS SpMM, one choice per batch: CVR steps
 # per chunk S; the column width (0: the library's automatic width, 16-bit
 # chunk-relative columns wherever a chunk spans fewer than 65,536 columns; 32:
-# 32-bit columns); and the X staging: 0 reads X rows in place, 1 or 2 stage
-# the X rows around each group of 4 chunks in an LDS window sized for 1 or 2
-# workgroups per CU (k <= 8 only).
-BATCH_TILES = [(s, bits, win) for s in (16, 32, 64, 128, 256) for bits in (0, 32) for win in (0, 1, 2)]
-BATCH_FIXED = (64, 0, 0)  # the tile the batched path used before it was autotuned
+# 32-bit columns); the X staging: 0 reads X rows in place, 1 or 2 stage the X
+# rows around each group of 4 chunks in an LDS window sized for 1 or 2
+# workgroups per CU (k <= 8 only); and the engine: SPMM_FMA (per-lane FP64
+# FMA) or SPMM_MFMA (the matrix cores; X rows read in place).
+_STEPS = (16, 32, 64, 128, 256)
+BATCH_TILES = ([(s, bits, win, SPMM_FMA) for s in _STEPS for bits in (0, 32) for win in (0, 1, 2)] +
+               [(s, bits, 0, SPMM_MFMA) for s in _STEPS for bits in (0, 32)])
+BATCH_FIXED = (64, 0, 0, SPMM_FMA)  # the tile the batched path used before it was autotuned
+
+
+def tile_name(tile) -> str:
+    """S64-auto-w0-fma style name of a batch tile."""
+    s, bits, win, eng = tile
+    return f"S{s}-{'auto' if bits == 0 else bits}-w{win}-{'mfma' if eng == SPMM_MFMA else 'fma'}"
 
 
 def batch_family(nmat: int = 512, seed: int = 4, threads: int = 8):
@@ -299,10 +308,10 @@ def build_batch_handles(mats, S: int, bits: int):
     return hs
 
 
-def time_batch(hs, mats, k: int, iters: int = 10, seed: int = 0, win: int = 0) -> float:
+def time_batch(hs, mats, k: int, iters: int = 10, seed: int = 0, win: int = 0, engine: int = SPMM_FMA) -> float:
     """Milliseconds per replay of the captured persistent batched SpMM over
     handles hs (FP32 X blocks of k columns, uniform [-1, 1]), with X window
-    mode `win`."""
+    mode `win` and SpMM engine `engine`."""
     import torch
     from . import Batch
     g = torch.Generator(device="cuda")
@@ -311,6 +320,7 @@ def time_batch(hs, mats, k: int, iters: int = 10, seed: int = 0, win: int = 0) -
     Ys = [torch.empty((A.numRows, k), dtype=torch.float32, device="cuda") for A in mats]
     b = Batch(hs, k)
     try:
+        b.set_engine(engine)
         if win:
             b.set_x_window(win)
         b.set_operands([t.data_ptr() for t in Xs], [t.data_ptr() for t in Ys])
@@ -331,8 +341,8 @@ def time_batch(hs, mats, k: int, iters: int = 10, seed: int = 0, win: int = 0) -
 
 def autotune_batch(mats, k: int, tiles=BATCH_TILES, sample: int = 64, iters: int = 10, seed: int = 0):
     """Exhaustive tile search for a batched SpMM: every candidate (S, column
-    width, X staging) is built and timed as one persistent launch over a
-    seeded sample of the batch's matrices. Returns (best tile, {tile: ms per
+    width, X staging, engine) is built and timed as one persistent launch over
+    a seeded sample of the batch's matrices. Returns (best tile, {tile: ms per
     launch on the sample}); inf where a candidate does not apply (X windows
     beyond k = 8, a layout that cannot be built)."""
     rng = np.random.default_rng(seed)
@@ -340,18 +350,18 @@ def autotune_batch(mats, k: int, tiles=BATCH_TILES, sample: int = 64, iters: int
     sub = [mats[int(i)] for i in pick]
     times = {}
     for S, bits in dict.fromkeys((t[0], t[1]) for t in tiles):
-        wins = [t[2] for t in tiles if (t[0], t[1]) == (S, bits)]
+        rest = [t for t in tiles if (t[0], t[1]) == (S, bits)]
         try:
             hs = build_batch_handles(sub, S, bits)
         except PavaError:
-            times.update({(S, bits, w): float("inf") for w in wins})
+            times.update({t: float("inf") for t in rest})
             continue
         try:
-            for w in wins:
+            for t in rest:
                 try:
-                    times[(S, bits, w)] = time_batch(hs, sub, k, iters=iters, seed=seed, win=w)
+                    times[t] = time_batch(hs, sub, k, iters=iters, seed=seed, win=t[2], engine=t[3])
                 except PavaError:
-                    times[(S, bits, w)] = float("inf")
+                    times[t] = float("inf")
         finally:
             for h in hs:
                 h.close()

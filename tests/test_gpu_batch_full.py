@@ -47,6 +47,7 @@ def test_batch_family_full_size(P, O, family, k):
     hs = SEL.build_batch_handles(mats, tile[0], tile[1])
     b = P.Batch(hs, k)
     try:
+        b.set_engine(tile[3])
         if tile[2]:
             b.set_x_window(tile[2])
         g = torch.Generator(device="cuda")
