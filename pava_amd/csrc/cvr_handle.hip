@@ -65,6 +65,7 @@ struct cvr_hip_handle {
     void *d_xhot = nullptr;
     int n_hot = 0;
     int n_lds = 0, lds_blocks = 0;  // hot columns held in LDS by the SpMV, and its grid
+    int *d_wg_next = nullptr;       // LDS variant: per-workgroup pop counters + finished count
     std::vector<void *> allocs;
     // one device allocation holding the arrays whose sizes follow from the
     // layout's shape, carved in order by dev_alloc (fewer hipMalloc/hipFree
@@ -197,6 +198,7 @@ static int validate_opts(const cvr_hip_opts *o) {
 }
 
 static int plan_parity(cvr_hip_handle *h);
+static int upload_lds_lists(cvr_hip_handle *h, const std::vector<std::vector<int>> &lst);
 
 // Core: conversion from a CSR in device memory.
 static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d_ci, const double *d_va,
@@ -468,10 +470,60 @@ static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d
             const long long lds_bytes = o->x_lds_kb < 0 ? -1 : (long long)o->x_lds_kb * 1024;
             HIP_TRY(cvr::plan_spmv_lds(h->prec, h->d_rflag != nullptr || h->par, nh, lds_bytes, h->nchunks, &h->n_lds,
                                        &h->lds_blocks));
+            if (h->n_lds > 0) {
+                rc2 = dev_alloc(h, (void **)&h->d_wg_next, 2 * sizeof(int));
+                if (rc2) return rc2;
+                HIP_TRY(hipMemsetAsync(h->d_wg_next, 0, 2 * sizeof(int), h->stream));
+                HIP_TRY(hipStreamSynchronize(h->stream));
+            }
         }
         h->hot_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-    if (h->par && (rc = plan_parity(h))) return rc;
+    if (h->par) {
+        if ((rc = plan_parity(h))) return rc;
+    } else if (h->n_lds > 0) {
+        // contiguous ranges of nchunks / G chunks
+        const int G = h->lds_blocks, nc = h->nchunks;
+        std::vector<std::vector<int>> lst((size_t)G);
+        for (int g = 0; g < G; ++g)
+            for (int c = (int)((long long)nc * g / G); c < (int)((long long)nc * (g + 1) / G); ++c)
+                lst[(size_t)g].push_back(c);
+        if ((rc = upload_lds_lists(h, lst))) return rc;
+    }
+    return 0;
+}
+
+// The LDS variant's chunk lists: workgroup w keeps the first half of its list
+// lst[w] (wg_list[wg_start[w] .. wg_start[w + 1])); the second halves of all
+// lists, interleaved (the first tail chunk of every workgroup, then the
+// second, ...), form the queue wg_list[wg_start[G] .. nchunks) that every
+// workgroup pops from once its own part is done. Shared fraction measured on
+// R-MAT scale 22, FP64 with parity: 1/8 0.438 ms, 1/4 0.434, 1/2 0.431 (only
+// private ranges: 0.446); without parity 0.380-0.382 (0.392).
+constexpr int kShareDiv = 2;
+static int upload_lds_lists(cvr_hip_handle *h, const std::vector<std::vector<int>> &lst) {
+    const int G = (int)lst.size(), nc = h->nchunks;
+    std::vector<int> list, start((size_t)G + 1, 0);
+    std::vector<size_t> keep((size_t)G);
+    list.reserve((size_t)nc);
+    size_t max_tail = 0;
+    for (int g = 0; g < G; ++g) {
+        const size_t n = lst[(size_t)g].size();
+        keep[(size_t)g] = n - n / kShareDiv;
+        max_tail = std::max(max_tail, n - keep[(size_t)g]);
+        start[(size_t)g] = (int)list.size();
+        list.insert(list.end(), lst[(size_t)g].begin(), lst[(size_t)g].begin() + (long)keep[(size_t)g]);
+    }
+    start[(size_t)G] = (int)list.size();
+    for (size_t j = 0; j < max_tail; ++j)
+        for (int g = 0; g < G; ++g)
+            if (keep[(size_t)g] + j < lst[(size_t)g].size()) list.push_back(lst[(size_t)g][keep[(size_t)g] + j]);
+    if ((int)list.size() != nc) return set_error(PAVA_HIP_EDEVICE, "chunk lists hold %zu of %d chunks", list.size(), nc);
+    int rc;
+    if ((rc = dev_alloc(h, (void **)&h->d_wg_list, (size_t)nc * 4))) return rc;
+    if ((rc = dev_alloc(h, (void **)&h->d_wg_start, ((size_t)G + 1) * 4))) return rc;
+    HIP_TRY(hipMemcpy(h->d_wg_list, list.data(), (size_t)nc * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->d_wg_start, start.data(), ((size_t)G + 1) * 4, hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -479,13 +531,16 @@ static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d
 // chunks whether all its chunks run in one workgroup (its carries then pass
 // through ordinary memory with workgroup-scope ordering, else through
 // agent-scope atomics). The plain kernel runs chunks 4g..4g+3 in one
-// workgroup. The LDS variant's workgroup w runs a list: first the chunks of
-// the long rows (longer than kSpreadLen) assigned to it -- long rows are dealt
-// to the least loaded workgroups, longest first, so that their serial
-// recomputes, when the bound requires them, start early and on different
-// CUs -- then a contiguous run of the other chunks, filled up to its share
-// nchunks/G. A chunk holding the end of one long row and the start of
-// another goes with the first; the second then spans two workgroups.
+// workgroup. The LDS variant's workgroup w starts from a list: first the
+// chunks of the long rows (longer than kSpreadLen) assigned to it -- long rows
+// are dealt to the least loaded workgroups, longest first, so that their
+// serial recomputes, when the bound requires them, start early and on
+// different CUs -- then a contiguous run of the other chunks, filled up to its
+// share nchunks/G. A chunk holding the end of one long row and the start of
+// another goes with the first; the second then spans two workgroups. The last
+// half of each list goes to a queue shared by all workgroups
+// (upload_lds_lists), so a chunk may run anywhere: in the LDS variant every
+// row spanning chunks passes its carries through agent-scope atomics.
 constexpr int kSpreadLen = 2048;
 static int plan_parity(cvr_hip_handle *h) {
     const int nc = h->nchunks;
@@ -543,17 +598,7 @@ static int plan_parity(cvr_hip_handle *h) {
             wg_of[(size_t)c] = w;
             ++filled;
         }
-        std::vector<int> list, start((size_t)G + 1, 0);
-        list.reserve((size_t)nc);
-        for (int g = 0; g < G; ++g) {
-            start[(size_t)g] = (int)list.size();
-            list.insert(list.end(), lst[(size_t)g].begin(), lst[(size_t)g].end());
-        }
-        start[(size_t)G] = (int)list.size();
-        if ((rc = dev_alloc(h, (void **)&h->d_wg_list, (size_t)nc * 4))) return rc;
-        if ((rc = dev_alloc(h, (void **)&h->d_wg_start, ((size_t)G + 1) * 4))) return rc;
-        HIP_TRY(hipMemcpy(h->d_wg_list, list.data(), (size_t)nc * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(h->d_wg_start, start.data(), ((size_t)G + 1) * 4, hipMemcpyHostToDevice));
+        if ((rc = upload_lds_lists(h, lst))) return rc;
     } else {
         for (int c = 0; c < nc; ++c) wg_of[(size_t)c] = c / cvr::kWavesPerBlock;
     }
@@ -567,7 +612,7 @@ static int plan_parity(cvr_hip_handle *h) {
     for (int k = 0; k < nc; ++k) {
         const int row = fix[2 * (size_t)k], c0 = fix[2 * (size_t)k + 1];
         if (row < 0) continue;
-        bool same = true;
+        bool same = h->n_lds == 0;  // LDS variant: chunks move between workgroups
         for (int c = c0; c < k && same; ++c) same = wg_of[(size_t)c] == wg_of[(size_t)k];
         if (same) pinfo[2 * (size_t)k].w |= cvr::kPiHeadIntra;
         for (int c = c0; c < k; ++c) {
@@ -814,6 +859,7 @@ static cvr::SpmvArgs spmv_args(const cvr_hip_handle *h, const void *d_x, void *d
     a.recompute = h->d_rflag != nullptr;
     a.n_lds = h->n_lds;
     a.lds_blocks = h->lds_blocks;
+    a.wg_next = h->d_wg_next;
     if (h->par) {
         a.rp = h->d_rp;
         a.ci = h->d_ci;
@@ -823,10 +869,10 @@ static cvr::SpmvArgs spmv_args(const cvr_hip_handle *h, const void *d_x, void *d
         a.fix = h->d_fix;
         a.pinfo = h->d_pinfo;
         a.close_cnt = h->d_close_cnt;
-        a.wg_list = h->d_wg_list;
-        a.wg_start = h->d_wg_start;
         a.parity_tol = h->opts.parity_tol;
     }
+    a.wg_list = h->d_wg_list;
+    a.wg_start = h->d_wg_start;
     return a;
 }
 

@@ -96,6 +96,9 @@ struct SpmvArgs {
     int nchunks, S, M, has_empty, recompute;
     int n_lds;       // hot columns [0, n_lds) read from an LDS copy (0: no LDS table)
     int lds_blocks;  // grid of the LDS variant
+    // LDS variant: [0] = items of the shared chunk queue handed out, [1] =
+    // workgroups finished; both 0 between SpMVs (the last workgroup re-arms them)
+    int *wg_next;
     // FP64 serial parity (parity_tol > 0): the resident FP64 CSR, the bound
     // terms of each head / tail carry, and the rows recomputed per chunk
     const int *rp;
@@ -113,8 +116,9 @@ struct SpmvArgs {
     // by k runs in one workgroup
     const int4 *pinfo;
     int *close_cnt;      // per chunk: contributing chunks arrived (0 between SpMVs)
-    const int *wg_list;      // LDS variant: the chunks of each workgroup (or null: contiguous ranges)
-    const int *wg_start;     // LDS variant: workgroup w runs wg_list[wg_start[w] .. wg_start[w + 1])
+    const int *wg_list;      // LDS variant: the chunks of each workgroup, then the shared queue
+    const int *wg_start;     // LDS variant: workgroup w runs wg_list[wg_start[w] .. wg_start[w + 1]), then
+                             // takes items of the queue wg_list[wg_start[G] .. nchunks) (G workgroups)
     double parity_tol;
 };
 

@@ -586,18 +586,27 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
     };
 
     if constexpr (LDSX) {
-        // the workgroup's chunks, handed to its waves from an LDS counter (a
-        // wave busy with a serial recompute leaves the rest to the others):
-        // a contiguous range, or with FP64 parity the list wg_list[wg_start[w]
-        // .. wg_start[w + 1]), which begins with the chunks of the long rows
-        // assigned to this workgroup (long rows are spread over the
-        // workgroups, so that their serial recomputes, when needed, start
-        // early and in parallel) followed by a contiguous run of the others
+        // Workgroup w's chunks: first its list wg_list[wg_start[w] ..
+        // wg_start[w + 1]) -- a contiguous range, or with FP64 parity one that
+        // begins with the chunks of the long rows assigned to it (long rows
+        // are spread over the workgroups, so that their serial recomputes,
+        // when needed, start early and in parallel) -- popped by its waves
+        // from an LDS counter (a wave busy with a serial recompute leaves the
+        // rest to the others); then the shared queue wg_list[wg_start[G] ..
+        // nchunks) (the second half of every workgroup's range, interleaved),
+        // popped by every workgroup from one global counter. Recomputes
+        // cluster in some ranges (R-MAT scale 22 with parity: workgroups
+        // finished between 365 and 443 us with private ranges only); the queue
+        // lets the others take over the ends of those ranges. Each wave asks
+        // for its next queue item before running the current one, so the
+        // atomic's round trip overlaps the chunk. The last workgroup to finish
+        // zeroes the two counters (wg_next[0]: queue items handed out,
+        // wg_next[1]: workgroups finished) for the next SpMV.
         __shared__ int next_chunk;
-        const int lb = xcd_remap(blockIdx.x, gridDim.x);
-        const bool listed = PAR && a.wg_list;
-        const int c_lo = listed ? a.wg_start[lb] : (int)((long long)a.nchunks * lb / gridDim.x);
-        const int c_end = listed ? a.wg_start[lb + 1] : (int)((long long)a.nchunks * (lb + 1) / gridDim.x);
+        const int nwg = (int)gridDim.x;
+        const int lb = xcd_remap(blockIdx.x, nwg);
+        const int c_lo = a.wg_start[lb], c_end = a.wg_start[lb + 1];
+        const int q0 = a.wg_start[nwg], nq = a.nchunks - q0;
         XT *t = reinterpret_cast<XT *>(lds_tab);
         const XT *__restrict__ src = reinterpret_cast<const XT *>(a.xhot);
         for (int i = threadIdx.x; i < a.n_lds; i += blockDim.x) t[i] = src[i];
@@ -608,8 +617,21 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
             if (lane == 0) c = atomicAdd(&next_chunk, 1);
             c = __builtin_amdgcn_readfirstlane(c);
             if (c >= c_end) break;
-            if (listed) c = a.wg_list[c];
-            run_chunk(c);
+            run_chunk(a.wg_list[c]);
+        }
+        int q = 0;
+        if (lane == 0) q = atomicAdd(&a.wg_next[0], 1);
+        q = __builtin_amdgcn_readfirstlane(q);
+        while (q < nq) {
+            int qn = 0;
+            if (lane == 0) qn = atomicAdd(&a.wg_next[0], 1);
+            run_chunk(a.wg_list[q0 + q]);
+            q = __builtin_amdgcn_readfirstlane(qn);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && atomicAdd(&a.wg_next[1], 1) == nwg - 1) {
+            a.wg_next[0] = 0;
+            a.wg_next[1] = 0;
         }
     } else {
         const int g = xcd_remap(blockIdx.x, gridDim.x) * WPB;
@@ -934,7 +956,8 @@ hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st) {
         }
         return hipGetLastError();
     }
-    if (a.n_lds > 0 && (col16 || a.lds_blocks <= 0)) return hipErrorInvalidValue;
+    if (a.n_lds > 0 && (col16 || a.lds_blocks <= 0 || !a.wg_next || !a.wg_list || !a.wg_start))
+        return hipErrorInvalidValue;
     dim3 grid((a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock);
     if (prec == 0) {
         const bool par = a.parity_tol > 0.0;
