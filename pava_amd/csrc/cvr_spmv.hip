@@ -97,9 +97,9 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 // by lane: all lanes apply their K FMAs to the running sum s (wave-uniform,
 // an SGPR operand), and lane l's result becomes the next s, so the chain
 // costs K dependent FMAs and one readlane per lane -- no LDS. While round r
-// is chained, the x gathers of rounds up to r + D and the columns and values
-// of round r + D + 1 are in flight; nothing waits on a load before the
-// chain needs it. Elements past e are masked to (0, 0) when chained, which
+// is chained, the x gathers and values of rounds up to r + D and the columns
+// of round r + D + 1 are in flight; nothing waits on a load before the chain
+// needs it. Elements past e are masked to (0, 0) when chained, which
 // leaves s unchanged. All lanes must call it with the same b < e; the
 // result is wave-uniform.
 template <int K, int D>
@@ -107,29 +107,33 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
                                                  const double *__restrict__ x, int b, int e, int lane) {
     constexpr int R = 64 * K;
     double s = 0.0;
-    int cn[K];                             // columns of round r + D + 1
-    double vq[D + 2][K], xq[D + 1][K];     // values of rounds r .. r + D + 1, x of rounds r .. r + D
-    auto load_cv = [&](int j0, int (&c)[K], double (&v)[K]) {
+    int cn[K];                         // columns of round r + D + 1
+    double vq[D + 1][K], xq[D + 1][K];  // values and x of rounds r .. r + D
+    auto at = [&](int j0, int k) -> unsigned {
+        const int j = j0 + lane * K + k;
+        return (unsigned)(j < e ? j : e - 1);
+    };
+    auto load_c = [&](int j0, int (&c)[K]) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int j = j0 + lane * K + k;
-            const unsigned jj = (unsigned)(j < e ? j : e - 1);
-            c[k] = __builtin_nontemporal_load(&ci[jj]);
-            v[k] = __builtin_nontemporal_load(&va[jj]);
-        }
+        for (int k = 0; k < K; ++k) c[k] = __builtin_nontemporal_load(&ci[at(j0, k)]);
+    };
+    auto load_v = [&](int j0, double (&v)[K]) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = __builtin_nontemporal_load(&va[at(j0, k)]);
     };
     auto gather = [&](const int (&c)[K], double (&xo)[K]) {
 #pragma unroll
         for (int k = 0; k < K; ++k) xo[k] = x[(unsigned)c[k]];
     };
-    // prologue: rounds 0 .. D gathered, round D + 1 columns loading
+    // prologue: rounds 0 .. D loaded and gathered, the columns of round D + 1 loading
     static_assert(D >= 1 && D <= 3, "pipeline depth");
 #pragma unroll
     for (int d = 0; d <= D; ++d) {
-        load_cv(b + d * R, cn, vq[d]);
+        load_c(b + d * R, cn);
+        load_v(b + d * R, vq[d]);
         gather(cn, xq[d]);
     }
-    load_cv(b + (D + 1) * R, cn, vq[D + 1]);
+    load_c(b + (D + 1) * R, cn);
     for (int j0 = b; j0 < e; j0 += R) {
         const int nl = (e - j0 + K - 1) / K;  // lanes holding elements of this round
         const int nl64 = nl < 64 ? nl : 64;
@@ -154,18 +158,19 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
         } else {
             for (int l = 0; l < nl64; ++l) hop(l);
         }
-        // shift: gather round r + D + 1, load round r + D + 2
+        // shift: round r + D + 1 gathered and its values loaded, the columns
+        // of round r + D + 2 loading
         if (j0 + R < e) {
 #pragma unroll
             for (int d = 0; d < D; ++d)
 #pragma unroll
-                for (int k = 0; k < K; ++k) xq[d][k] = xq[d + 1][k];
-#pragma unroll
-            for (int d = 0; d <= D; ++d)
-#pragma unroll
-                for (int k = 0; k < K; ++k) vq[d][k] = vq[d + 1][k];
+                for (int k = 0; k < K; ++k) {
+                    xq[d][k] = xq[d + 1][k];
+                    vq[d][k] = vq[d + 1][k];
+                }
             gather(cn, xq[D]);
-            load_cv(j0 + (D + 2) * R, cn, vq[D + 1]);
+            load_v(j0 + (D + 1) * R, vq[D]);
+            load_c(j0 + (D + 2) * R, cn);
         }
     }
     return s;
