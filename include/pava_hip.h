@@ -232,7 +232,11 @@ int cvr_hip_destroy(cvr_hip_handle *h);
 int cvr_hip_get_info(const cvr_hip_handle *h, cvr_hip_info *info);
 
 /* y = A*x, FP64 handle, device pointers, enqueued on `stream`, asynchronous.
- * y is fully written (empty rows get 0); no pre-zeroing needed. */
+ * y is fully written (empty rows get 0); no pre-zeroing needed. A handle
+ * holds per-SpMV device state (the hot-x copy, carries of rows spanning
+ * chunks, their arrival counters and the shared chunk-queue counters, each
+ * left re-armed by the kernel for the next call): SpMVs and SpMMs on one
+ * handle must not overlap (one stream, or ordered streams). */
 int cvr_hip_spmv(cvr_hip_handle *h, const double *d_x, double *d_y, void *stream);
 /* Host-pointer variant (copies in and out, synchronous). */
 int cvr_hip_spmv_host(cvr_hip_handle *h, const double *x, double *y);
