@@ -176,6 +176,35 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
     return s;
 }
 
+// serial_fma_sum for the plain kernel's long rows, kept out of line: its
+// registers (8 elements per lane) are then allocated apart from the SpMV's
+// main loop, which would otherwise spill at the plain kernel's 128-VGPR
+// budget.
+template <int K, int D>
+__device__ __noinline__ double serial_fma_sum_ool(const int *__restrict__ ci, const double *__restrict__ va,
+                                                  const double *__restrict__ x, int b, int e, int lane) {
+    return serial_fma_sum<K, D>(ci, va, x, b, e, lane);
+}
+
+// The serial FMA sum of CSR elements [b, e) for the parity recompute. The LDS
+// variant (up to 256 VGPRs) chains 8 elements per lane inline. The plain
+// kernel chains rows of up to 2,048 elements with 4 per lane inline, longer
+// ones with 8 per lane out of line: fewer lane hops per element (a chain alone
+// in one wave: 13.5 cycles per element with 4, 10.1 with 8) where the chain
+// sets the kernel's end, with the call's register saves only where they pay
+// (selector-family matrix with a 20,000-element cancelling row spanning
+// chunks: 0.270 -> 0.229 ms; the 27-point stencil unchanged).
+template <bool LDSX, typename ARGS>
+__device__ __forceinline__ double serial_chain(const ARGS &a, int b, int e, int lane) {
+    const double *xs = reinterpret_cast<const double *>(a.x);
+    if constexpr (LDSX)
+        return serial_fma_sum<8, 1>(a.ci, a.va, xs, b, e, lane);
+    else if (e - b > 2048)
+        return serial_fma_sum_ool<8, 1>(a.ci, a.va, xs, b, e, lane);
+    else
+        return serial_fma_sum<4, 1>(a.ci, a.va, xs, b, e, lane);
+}
+
 // KAPPA also accumulates sum|v x| (the cancellation scale) for the FP64
 // recompute decision.
 template <typename VT, bool KAPPA>
@@ -247,6 +276,7 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
     SpmvArgs a) {
     constexpr int WPB = LDSX ? kWavesLds : kWavesPerBlock;
     const int lane = threadIdx.x & 63;
+
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     constexpr int G = kStepGroup;
     // FP64 with serial parity: split rows carry bound terms, and the rows that
@@ -501,7 +531,7 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
             for (int f = 0; f < nf; ++f) {
                 const int row = __builtin_amdgcn_readfirstlane(flist[f]);
                 const int b = a.rp[row], e = a.rp[row + 1];
-                const double v = serial_fma_sum<LDSX ? 8 : 4, 1>(a.ci, a.va, reinterpret_cast<const double *>(a.x), b, e, lane);
+                const double v = serial_chain<LDSX>(a, b, e, lane);
                 if (lane == 0) store_y<YT>(a.y, row, v);
             }
             if (lane == 0) a.rc_count[c] = nf;
@@ -561,8 +591,7 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                 }
                 const bool ok = par_ok(s, A, Q, N, a.parity_tol);
                 if (!ok)
-                    s = serial_fma_sum<LDSX ? 8 : 4, 1>(a.ci, a.va, reinterpret_cast<const double *>(a.x), a.rp[row],
-                                                     a.rp[row + 1], lane);
+                    s = serial_chain<LDSX>(a, a.rp[row], a.rp[row + 1], lane);
                 if (lane == 0) {
                     store_y<YT>(a.y, row, s);
                     a.rc_count[a.nchunks + k] = ok ? 0 : 1;
