@@ -211,6 +211,10 @@ typedef struct cvr_hip_info {
     int n_lds_cols;          /* of those, columns the SpMV reads from LDS */
     double hot_setup_ms;     /* wall time of the hot-column staging (counts, selection, remap) */
     double parity_tol;       /* in effect (0: off) */
+    int n_spec_rows;         /* FP64 parity without the LDS table: rows spanning chunks (at least
+                                4,096 elements, the 1,024 longest) whose serial sum is started at
+                                the beginning of each SpMV, before their bound is known; used when
+                                the bound fails, abandoned when it passes (results unchanged) */
 } cvr_hip_info;
 
 typedef struct cvr_hip_handle cvr_hip_handle;

@@ -59,7 +59,8 @@ class _Info(C.Structure):
                 ("layout_elems", C.c_longlong), ("seg_capacity", C.c_longlong), ("max_slots", C.c_int),
                 ("has_empty_rows", C.c_int), ("n_fixup_rows", C.c_int), ("device_bytes", C.c_longlong),
                 ("convert_ms", C.c_double), ("n_hot_cols", C.c_int), ("n_lds_cols", C.c_int),
-                ("hot_setup_ms", C.c_double), ("parity_tol", C.c_double)]
+                ("hot_setup_ms", C.c_double), ("parity_tol", C.c_double),
+                ("n_spec_rows", C.c_int)]
 
 
 class _GenParams(C.Structure):

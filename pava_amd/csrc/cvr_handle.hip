@@ -66,6 +66,10 @@ struct cvr_hip_handle {
     int n_hot = 0;
     int n_lds = 0, lds_blocks = 0;  // hot columns held in LDS by the SpMV, and its grid
     int *d_wg_next = nullptr;       // LDS variant: per-workgroup pop counters + finished count
+    // FP64 parity, plain kernel: speculative serial chains of the longest rows spanning chunks
+    int n_spec = 0, spec_blocks = 0;
+    int *d_spec_rows = nullptr, *d_spec_state = nullptr;
+    double *d_spec_val = nullptr;
     std::vector<void *> allocs;
     // one device allocation holding the arrays whose sizes follow from the
     // layout's shape, carved in order by dev_alloc (fewer hipMalloc/hipFree
@@ -622,6 +626,35 @@ static int plan_parity(cvr_hip_handle *h) {
             pinfo[2 * (size_t)c + 1] = make_int4(row, c0, 0, 0);
         }
     }
+    // Plain kernel: the kSpecMax longest rows spanning chunks with at least
+    // kSpecLen elements get a speculative serial chain, started at kernel
+    // start by a wave of their own instead of after the row's last chunk
+    // arrives (the LDS variant runs the chunks of long rows first instead).
+    // Such a chain stops once the row's bound has passed; otherwise its sum
+    // is the row's y, as the closer's own chain would have been.
+    if (h->n_lds == 0) {
+        std::vector<int> spec_k;
+        for (int k = 0; k < nc; ++k)
+            if (fix[2 * (size_t)k] >= 0 && len[(size_t)k] >= cvr::kSpecLen) spec_k.push_back(k);
+        std::stable_sort(spec_k.begin(), spec_k.end(), [&](int a, int b) { return len[(size_t)a] > len[(size_t)b]; });
+        if ((int)spec_k.size() > cvr::kSpecMax) spec_k.resize(cvr::kSpecMax);
+        std::vector<int> rows(spec_k.size());
+        for (size_t i = 0; i < spec_k.size(); ++i) {
+            const int k = spec_k[i], c0 = fix[2 * (size_t)k + 1];
+            rows[i] = fix[2 * (size_t)k];
+            pinfo[2 * (size_t)k + 1].w = (int)i + 1;  // the row chunk k closes
+            for (int c = c0; c < k; ++c) pinfo[2 * (size_t)c + 1].z = (int)i + 1;  // their tail carries' row
+        }
+        h->n_spec = (int)spec_k.size();
+        if (h->n_spec > 0) {
+            h->spec_blocks = (h->n_spec + 7) / 8 * 8;  // whole XCD rounds: the chunk blocks keep their XCDs
+            if ((rc = dev_alloc(h, (void **)&h->d_spec_rows, (size_t)h->n_spec * 4))) return rc;
+            if ((rc = dev_alloc(h, (void **)&h->d_spec_state, (size_t)h->n_spec * 4))) return rc;
+            if ((rc = dev_alloc(h, (void **)&h->d_spec_val, (size_t)h->n_spec * 8))) return rc;
+            HIP_TRY(hipMemcpy(h->d_spec_rows, rows.data(), (size_t)h->n_spec * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemset(h->d_spec_state, 0, (size_t)h->n_spec * 4));
+        }
+    }
     if ((rc = dev_alloc(h, (void **)&h->d_pinfo, (size_t)nc * 2 * sizeof(int4)))) return rc;
     HIP_TRY(hipMemcpy(h->d_pinfo, pinfo.data(), (size_t)nc * 2 * sizeof(int4), hipMemcpyHostToDevice));
     return 0;
@@ -830,6 +863,7 @@ extern "C" int cvr_hip_get_info(const cvr_hip_handle *h, cvr_hip_info *info) {
     info->n_lds_cols = h->n_lds;
     info->hot_setup_ms = h->hot_setup_ms;
     info->parity_tol = h->par ? h->opts.parity_tol : 0.0;
+    info->n_spec_rows = h->n_spec;
     return 0;
 }
 
@@ -870,6 +904,11 @@ static cvr::SpmvArgs spmv_args(const cvr_hip_handle *h, const void *d_x, void *d
         a.pinfo = h->d_pinfo;
         a.close_cnt = h->d_close_cnt;
         a.parity_tol = h->opts.parity_tol;
+        a.spec_rows = h->d_spec_rows;
+        a.spec_state = h->d_spec_state;
+        a.spec_val = h->d_spec_val;
+        a.n_spec = h->n_spec;
+        a.spec_blocks = h->spec_blocks;
     }
     a.wg_list = h->d_wg_list;
     a.wg_start = h->d_wg_start;

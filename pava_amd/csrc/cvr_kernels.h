@@ -111,11 +111,19 @@ struct SpmvArgs {
     // per chunk c, what its end needs, read when the chunk starts:
     //   pinfo[2c]     = (row closed by c or -1, that row's first chunk,
     //                    chunk k closing the row c's tail carry belongs to or -1, flags)
-    //   pinfo[2c + 1] = (row closed by k, that row's first chunk, 0, 0)
+    //   pinfo[2c + 1] = (row closed by k, that row's first chunk, speculative chain + 1 of
+    //                    the row closed by k, of the row closed by c; 0: none)
     // flags: kPiHeadIntra / kPiTailIntra -- every chunk of the row closed by c /
     // by k runs in one workgroup
     const int4 *pinfo;
     int *close_cnt;      // per chunk: contributing chunks arrived (0 between SpMVs)
+    // FP64 parity, plain kernel: serial chains of the longest rows spanning
+    // chunks, started at kernel start by the first spec_blocks workgroups
+    // (one row per workgroup) instead of by the row's last arriving chunk
+    const int *spec_rows;    // [i]: row of speculative chain i
+    int *spec_state;         // [i]: kSpecDone | kSpecOk | kSpecNeed; 0 between SpMVs
+    double *spec_val;        // [i]: chain i's serial sum
+    int n_spec, spec_blocks;
     const int *wg_list;      // LDS variant: the chunks of each workgroup, then the shared queue
     const int *wg_start;     // LDS variant: workgroup w runs wg_list[wg_start[w] .. wg_start[w + 1]), then
                              // takes items of the queue wg_list[wg_start[G] .. nchunks) (G workgroups)
@@ -144,6 +152,13 @@ struct SpmvArgs {
 // bit-identical to the reference. Underflow is covered by an absolute term of
 // 2^-1074 per element.
 constexpr int kPiHeadIntra = 1, kPiTailIntra = 2;  // pinfo flags
+// Speculative serial chains (plain kernel): rows spanning chunks with at least
+// kSpecLen elements, the kSpecMax longest. State bits: the chain is done (its
+// sum stored), the row's bound passed (the CVR-order sum is written), or the
+// row needs the serial sum. Each side sets its bit with one atomic OR; the
+// side that finds the other's bit writes y where needed and clears the word.
+constexpr int kSpecLen = 4096, kSpecMax = 1024;
+constexpr int kSpecDone = 1, kSpecOk = 2, kSpecNeed = 4;
 constexpr int kParRigorousLen = 4096;
 constexpr double kParLambda = 10.0;
 constexpr long long kParNMask = 0xFFFFF;  // low 20 bits of a packed Q hold N (N <= 64 * 4096)

@@ -102,9 +102,12 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 // needs it. Elements past e are masked to (0, 0) when chained, which
 // leaves s unchanged. All lanes must call it with the same b < e; the
 // result is wave-uniform.
-template <int K, int D>
+// POLL: `stop` is read once per round (the load overlaps the round's chain)
+// and the chain ends early, its result unused, once kSpecOk is set there.
+template <int K, int D, bool POLL = false>
 __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, const double *__restrict__ va,
-                                                 const double *__restrict__ x, int b, int e, int lane) {
+                                                 const double *__restrict__ x, int b, int e, int lane,
+                                                 const int *stop = nullptr) {
     constexpr int R = 64 * K;
     double s = 0.0;
     int cn[K];                         // columns of round r + D + 1
@@ -135,6 +138,8 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
     }
     load_c(b + (D + 1) * R, cn);
     for (int j0 = b; j0 < e; j0 += R) {
+        int flag = 0;
+        if constexpr (POLL) flag = __hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int nl = (e - j0 + K - 1) / K;  // lanes holding elements of this round
         const int nl64 = nl < 64 ? nl : 64;
         double vv[K], xx[K];
@@ -158,6 +163,9 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
         } else {
             for (int l = 0; l < nl64; ++l) hop(l);
         }
+        if constexpr (POLL) {
+            if (__builtin_amdgcn_readfirstlane(flag) & kSpecOk) break;
+        }
         // shift: round r + D + 1 gathered and its values loaded, the columns
         // of round r + D + 2 loading
         if (j0 + R < e) {
@@ -180,10 +188,11 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
 // registers (8 elements per lane) are then allocated apart from the SpMV's
 // main loop, which would otherwise spill at the plain kernel's 128-VGPR
 // budget.
-template <int K, int D>
+template <int K, int D, bool POLL = false>
 __device__ __noinline__ double serial_fma_sum_ool(const int *__restrict__ ci, const double *__restrict__ va,
-                                                  const double *__restrict__ x, int b, int e, int lane) {
-    return serial_fma_sum<K, D>(ci, va, x, b, e, lane);
+                                                  const double *__restrict__ x, int b, int e, int lane,
+                                                  const int *stop = nullptr) {
+    return serial_fma_sum<K, D, POLL>(ci, va, x, b, e, lane, stop);
 }
 
 // The serial FMA sum of CSR elements [b, e) for the parity recompute. The LDS
@@ -590,10 +599,29 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                     N += Nj;
                 }
                 const bool ok = par_ok(s, A, Q, N, a.parity_tol);
-                if (!ok)
-                    s = serial_chain<LDSX>(a, a.rp[row], a.rp[row + 1], lane);
+                const int sp = LDSX ? 0 : (t == 0 ? pi1.w : pi1.z);  // speculative chain + 1
+                if (sp > 0) {
+                    // the row's serial chain started at kernel start: tell it the
+                    // outcome; whichever side comes second writes y if the
+                    // serial sum is needed and clears the state word
+                    if (lane == 0) {
+                        int *stp = a.spec_state + (sp - 1);
+                        if (ok) store_y<YT>(a.y, row, s);
+                        const int old = __hip_atomic_fetch_or(stp, ok ? kSpecOk : kSpecNeed, __ATOMIC_ACQ_REL,
+                                                              __HIP_MEMORY_SCOPE_AGENT);
+                        if (old & kSpecDone) {
+                            if (!ok)
+                                store_y<YT>(a.y, row,
+                                            __hip_atomic_load(&a.spec_val[sp - 1], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT));
+                            __hip_atomic_store(stp, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                    }
+                } else {
+                    if (!ok) s = serial_chain<LDSX>(a, a.rp[row], a.rp[row + 1], lane);
+                    if (lane == 0) store_y<YT>(a.y, row, s);
+                }
                 if (lane == 0) {
-                    store_y<YT>(a.y, row, s);
                     a.rc_count[a.nchunks + k] = ok ? 0 : 1;
                     if (!local) __hip_atomic_store(&a.close_cnt[k], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
@@ -668,7 +696,33 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
             a.wg_next[1] = 0;
         }
     } else {
-        const int g = xcd_remap(blockIdx.x, gridDim.x) * WPB;
+        if constexpr (PAR) {
+            // the first spec_blocks workgroups (dispatched first) run the
+            // speculative serial chains, one row per workgroup: chains that
+            // shared a CU ran up to 3.5x slower than alone (on a matrix with
+            // 133 such rows of 20,000 elements: 0.193 ms with 4 per workgroup,
+            // 0.172 with 1); the other waves of such a workgroup exit
+            if ((int)blockIdx.x < a.spec_blocks) {
+                const int i = (int)blockIdx.x;
+                if (wv == 0 && i < a.n_spec) {
+                    const int row = a.spec_rows[i];
+                    int *stp = a.spec_state + i;
+                    const double v = serial_fma_sum_ool<8, 1, true>(
+                        a.ci, a.va, reinterpret_cast<const double *>(a.x), a.rp[row], a.rp[row + 1], lane, stp);
+                    if (lane == 0) {
+                        __hip_atomic_store(&a.spec_val[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const int old = __hip_atomic_fetch_or(stp, kSpecDone, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                        if (old & (kSpecNeed | kSpecOk)) {
+                            if (old & kSpecNeed) store_y<YT>(a.y, row, v);
+                            __hip_atomic_store(stp, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                    }
+                }
+                return;  // the whole workgroup: no barrier follows for it
+            }
+        }
+        const int sb = PAR ? a.spec_blocks : 0;  // a multiple of 8: the XCD of a block is unchanged
+        const int g = xcd_remap((int)blockIdx.x - sb, (int)gridDim.x - sb) * WPB;
         const int c = g + wv;
         if constexpr (PAR) {
             if (threadIdx.x < WPB) lcnt[threadIdx.x] = 0;
@@ -992,7 +1046,11 @@ hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st) {
     }
     if (a.n_lds > 0 && (col16 || a.lds_blocks <= 0 || !a.wg_next || !a.wg_list || !a.wg_start))
         return hipErrorInvalidValue;
-    dim3 grid((a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (a.n_spec > 0 && (a.n_lds > 0 || a.parity_tol <= 0.0 || prec != 0 || !a.spec_rows || !a.spec_state ||
+                         !a.spec_val || a.spec_blocks < a.n_spec || a.spec_blocks % 8))
+        return hipErrorInvalidValue;
+    const int sblocks = a.n_spec > 0 ? a.spec_blocks : 0;
+    dim3 grid((a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock + sblocks);
     if (prec == 0) {
         const bool par = a.parity_tol > 0.0;
         if (col16 && par)

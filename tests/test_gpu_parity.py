@@ -132,3 +132,68 @@ def test_parity_device_create_and_graph(P, O):
             assert np.array_equal(dy.cpu().numpy(), want), rep
     finally:
         h.close()
+
+
+def test_speculative_chains_both_outcomes_repeated_and_graphed(P, O):
+    """Rows of at least 4,096 elements spanning chunks have their serial sum
+    started at the beginning of the SpMV (plain kernel, no LDS table), before
+    their bound is known. Rows of positive terms pass the bound (the chain is
+    abandoned and the CVR-order sum stays); cancelling rows fail it (the
+    chain's sum is used, whichever of the chain and the row's last chunk
+    finishes first). Every call, with new x each time and under graph replay,
+    equals the oracle's emulation bit for bit and recomputes the same rows:
+    the per-row state words are left cleared for the next call."""
+    import torch
+    rng = np.random.default_rng(31)
+    n = 60000
+    lens = [5000, 9000, 20000, 4096, 4500, 12000, 30, 7, 6000, 4100]
+    rows = [list(np.sort(rng.choice(n, L, replace=False))) for L in lens]
+    A = csr_from_dense_pattern(P, len(rows), n, rows, seed=6)
+    rp = A.rowOffsets
+    # rows 0-2 and 8: positive terms (pass); the others: random signs, with
+    # the last value set so that the row cancels (fail)
+    pos = {0, 1, 2, 8}
+    x0 = rng.uniform(0.5, 1.0, n)
+    for r in range(len(rows)):
+        b, e = int(rp[r]), int(rp[r + 1])
+        if r in pos:
+            A.vals[b:e] = rng.uniform(0.1, 1.0, e - b)
+        else:
+            A.vals[b:e] = rng.uniform(-1.0, 1.0, e - b)
+            if e - b > 1:
+                part = float(np.dot(A.vals[b:e - 1], x0[A.cols[b:e - 1]]))
+                A.vals[e - 1] = -part / x0[A.cols[e - 1]] * (1 + 1e-9)
+    S = 8
+    with P.CvrHip(A, steps_per_chunk=S, x_hot_kb=0) as h:
+        inf = h.info()
+        assert inf["n_lds_cols"] == 0 and inf["parity_tol"] > 0
+        assert inf["n_spec_rows"] == sum(1 for L in lens if L >= 4096), inf
+        lay = O.cvr64_build(A, S, 0)
+        dx = torch.empty(n, dtype=torch.float64, device="cuda")
+        dy = torch.empty(A.numRows, dtype=torch.float64, device="cuda")
+        xs = [x0] + [x0 * rng.uniform(0.9, 1.1, n) for _ in range(3)]
+
+        def want(x):
+            ye, _, ne = O.cvr64_spmv(A, lay, x, parity_tol=inf["parity_tol"], details=True)
+            return ye, ne
+
+        for it, x in enumerate(xs + xs):
+            dx.copy_(torch.from_numpy(x))
+            h.spmv(dx.data_ptr(), dy.data_ptr())
+            torch.cuda.synchronize()
+            ye, ne = want(x)
+            _say(f"speculative chains, call {it}: {h.last_recompute_count()} rows recomputed")
+            assert np.array_equal(dy.cpu().numpy(), ye), it
+            assert h.last_recompute_count() == ne, it
+            assert np.all(_rel(dy.cpu().numpy(), O.spmv_fma(A, x)) <= TOL)
+        s = torch.cuda.Stream()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            h.spmv(dx.data_ptr(), dy.data_ptr(), s.cuda_stream)
+        for it, x in enumerate(xs):
+            dx.copy_(torch.from_numpy(x))
+            dy.fill_(float("nan"))
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert np.array_equal(dy.cpu().numpy(), want(x)[0]), ("replay", it)
