@@ -323,6 +323,21 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
             pi0 = a.pinfo[2 * (size_t)c];
             pi1 = a.pinfo[2 * (size_t)c + 1];
         }
+        // The chunk's end reads its split table and its words of the empty-row
+        // bitmap; their first 64 entries are loaded now, with the chunk's first
+        // loads, so that the end does not wait on two more dependent round
+        // trips (R-MAT scale 22: ~2-3 us each under the SpMV's load, against
+        // ~35 us of main loop). kSplitCap > 64, so lane + 1 stays inside the
+        // chunk's table.
+        const int ns = a.split_n[c];
+        const int2 *stab = a.split_tab + (size_t)c * kSplitCap;
+        const int2 tab_pre = stab[lane];
+        const int se_pre = stab[lane + 1].y;
+        const int nw_empty = (a.M + 31) >> 5;
+        const int w0 = (int)((long long)nw_empty * c / a.nchunks),
+                  w1 = (int)((long long)nw_empty * (c + 1) / a.nchunks);
+        unsigned ebits_pre = 0;
+        if (a.has_empty && w0 + lane < w1) ebits_pre = a.empty_bits[w0 + lane];
 
         // Pipeline, one 8-step group deep: while group g is processed, the x
         // gathers and the segment-list entries of group g+1 (exactly as many as
@@ -409,20 +424,8 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
         __builtin_amdgcn_s_waitcnt(kWaitVm0);
         int sbase = 0, segp = 0;
 
-        for (int g = 0; g < steps; g += G) {
-            const bool has1 = (g + G) < steps, has2 = (g + 2 * G) < steps;
-            const int sbase_next = sbase + ecur;
-            int enext = 0;
-            if (has1) {
-#pragma unroll
-                for (int i = 0; i < G; ++i) xn[i] = xval(cn[i]);
-                enext = group_ends(cn, en);
-                load_segs(sbase_next, enext, sn);
-#pragma unroll
-                for (int i = 0; i < G; ++i) vn[i] = __builtin_nontemporal_load(&vp[(size_t)(g + G + i) * 64]);
-                if (has2) load_cols(g + 2 * G, cnn, enn);
-            }
-            // this group's destinations -> LDS
+        // one group: its destinations to LDS, then its G steps
+        auto run_group = [&](const int g) {
 #pragma unroll
             for (int j = 0; j < G; ++j)
                 if (j * 64 < ecur) segbuf[wv][j * 64 + lane] = sc[j];
@@ -459,32 +462,49 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                     segp += __popcll(m);
                 }
             }
-            if (has1) {
+        };
+        // Every group but the last issues the next group's loads before its
+        // own steps and moves them into place after, unconditionally: with the
+        // loads and the moves under two tests of one condition, the compiler's
+        // wait counting saw a path on which a load stayed in flight into the
+        // next group, and drained every load in flight (the next group's
+        // gathers included) before reusing its register -- two memory round
+        // trips per group instead of one. The last group runs after the loop.
+        int g = 0;
+        for (; g + G < steps; g += G) {
+            const bool has2 = (g + 2 * G) < steps;
+            const int sbase_next = sbase + ecur;
 #pragma unroll
-                for (int i = 0; i < G; ++i) {
-                    vb[i] = vn[i];
-                    cb[i] = cn[i];
-                    xb[i] = xn[i];
-                    cn[i] = cnn[i];
-                    sc[i] = sn[i];
-                }
-                eb = en;
-                en = enn;
-                sbase = sbase_next;
-                ecur = enext;
+            for (int i = 0; i < G; ++i) xn[i] = xval(cn[i]);
+            const int enext = group_ends(cn, en);
+            load_segs(sbase_next, enext, sn);
+#pragma unroll
+            for (int i = 0; i < G; ++i) vn[i] = __builtin_nontemporal_load(&vp[(size_t)(g + G + i) * 64]);
+            if (has2) load_cols(g + 2 * G, cnn, enn);
+            run_group(g);
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+                vb[i] = vn[i];
+                cb[i] = cn[i];
+                xb[i] = xn[i];
+                cn[i] = cnn[i];
+                sc[i] = sn[i];
             }
+            eb = en;
+            en = enn;
+            sbase = sbase_next;
+            ecur = enext;
             wave_sync_lds();  // the next group overwrites segbuf
         }
+        if (g < steps) run_group(g);
+        wave_sync_lds();
 
         // split rows: sum slots in element order
         int *flist = &segbuf[wv][0];
         if (PAR && lane == 0) fcnt[wv] = 0;
         wave_sync_lds();
-        const int ns = a.split_n[c];
-        const int2 *tab = a.split_tab + (size_t)c * kSplitCap;
-        for (int k = lane; k < ns; k += 64) {
-            const int2 t = tab[k];
-            const int se = tab[k + 1].y;
+        // one split group per lane: its slots merged in element order
+        auto split_group = [&](const int2 t, const int se) {
             if constexpr (PAR) {
                 // merged sum and the bound terms of par_ok
                 double s = slots[wv][t.y], q0;
@@ -492,17 +512,28 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                 par_unpack(slots_abs[wv][t.y], &q0, &n0);
                 double A = 2.0 * __builtin_sqrt((double)n0 * q0), Q = 2.0 * q0;
                 int N = n0;
-                for (int j = t.y + 1; j < se; ++j) {
+                auto merge = [&](double sj, double pj) {
                     double qj;
                     int nj;
-                    par_unpack(slots_abs[wv][j], &qj, &nj);
+                    par_unpack(pj, &qj, &nj);
                     A += 2.0 * __builtin_sqrt((double)nj * qj) + (double)nj * __builtin_fabs(s);
                     Q += 2.0 * (double)nj * s * s + 3.0 * qj;
-                    s += slots[wv][j];
+                    s += sj;
                     A += __builtin_fabs(s);
                     Q += s * s;
                     N += nj;
+                };
+                // slots read four at a time, so that a long row's merge waits
+                // on LDS once per four segments (same order of operations)
+                int j = t.y + 1;
+                for (; j + 4 <= se; j += 4) {
+                    double sv[4], pv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) sv[u] = slots[wv][j + u], pv[u] = slots_abs[wv][j + u];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) merge(sv[u], pv[u]);
                 }
+                for (; j < se; ++j) merge(slots[wv][j], slots_abs[wv][j]);
                 if (t.x >= 0) {
                     if (par_ok(s, A, Q, N, a.parity_tol))
                         store_y<YT>(a.y, t.x, s);
@@ -514,10 +545,18 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                 } else if (t.x == kCodeTail) {
                     put_carry(a, 2 * c + 1, s, A, par_pack(Q, N), !(pi0.w & kPiTailIntra));
                 }
-                continue;
+                return;
             }
             double s = slots[wv][t.y];
-            for (int j = t.y + 1; j < se; ++j) s += slots[wv][j];
+            int j = t.y + 1;
+            for (; j + 4 <= se; j += 4) {
+                double sv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) sv[u] = slots[wv][j + u];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) s += sv[u];
+            }
+            for (; j < se; ++j) s += slots[wv][j];
             double sa = 0.0;
             if (KAPPA)
                 for (int j = t.y; j < se; ++j) sa += slots_abs[wv][j];
@@ -531,7 +570,13 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                 a.carry[2 * c + 1] = s;
                 if (KAPPA) a.carry_abs[2 * c + 1] = sa;
             }
-        }
+        };
+        // the first 64 groups from the registers loaded at the chunk's start;
+        // more (rare) are loaded here, in a loop of their own, so that the
+        // common path has no load to wait for (a wait on a load also waits
+        // for every store the main loop left in flight)
+        if (lane < ns) split_group(tab_pre, se_pre);
+        for (int k = lane + 64; k < ns; k += 64) split_group(stab[k], stab[k + 1].y);
         if constexpr (PAR) {
             // rows of this chunk whose CVR-order sum may differ from the
             // serial sum by more than parity_tol: one serial FMA chain each
@@ -563,40 +608,60 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                 const bool span_tail = pi0.z >= 0 && !(pi0.w & kPiTailIntra);
                 if (span_head || span_tail) __builtin_amdgcn_s_waitcnt(kWaitVm0);
             }
+            // the chunk's two count-ins (head row: counter c, tail row: counter
+            // pi0.z) are issued together, by lanes 0 and 1, so that they cost
+            // one round trip; rows inside one workgroup of the plain variant
+            // count in LDS (chunk k is then the workgroup's chunk k % WPB)
+            auto close_chunk = [&](int t) { return t == 0 ? (pi0.x >= 0 ? c : -1) : pi0.z; };
+            auto close_intra = [&](int t) { return (pi0.w & (t == 0 ? kPiHeadIntra : kPiTailIntra)) != 0; };
+            int mine = 0;
+            if (lane < 2) {
+                const int k = close_chunk(lane);
+                if (k >= 0)
+                    mine = (!LDSX && close_intra(lane))
+                               ? atomicAdd(&lcnt[k % WPB], 1)
+                               : __hip_atomic_fetch_add(&a.close_cnt[k], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            const int olds[2] = {__builtin_amdgcn_readlane(mine, 0), __builtin_amdgcn_readlane(mine, 1)};
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
-                const int k = t == 0 ? (pi0.x >= 0 ? c : -1) : pi0.z;
+                const int k = close_chunk(t);
                 if (k < 0) continue;
                 const int row = t == 0 ? pi0.x : pi1.x;
                 const int c0 = t == 0 ? pi0.y : pi1.y;
-                const bool intra = (pi0.w & (t == 0 ? kPiHeadIntra : kPiTailIntra)) != 0;
-                // rows inside one workgroup of the plain variant count in LDS
-                // (chunk k is then the workgroup's chunk k % WPB)
+                const bool intra = close_intra(t);
                 const bool local = !LDSX && intra;
-                int old = 0;
-                if (lane == 0)
-                    old = local ? atomicAdd(&lcnt[k % WPB], 1)
-                                : __hip_atomic_fetch_add(&a.close_cnt[k], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                old = __builtin_amdgcn_readfirstlane(old);
-                if (old != k - c0) continue;
+                if (olds[t] != k - c0) continue;
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                double s, A, Q, QN;
-                int N;
-                get_carry(a, 2 * c0 + 1, &s, &A, &QN, !intra);
-                par_unpack(QN, &Q, &N);
-                for (int j = c0 + 1; j <= k; ++j) {
-                    const int idx = j < k ? 2 * j + 1 : 2 * k;
-                    double sj, Aj, QNj;
-                    get_carry(a, idx, &sj, &Aj, &QNj, !intra);
-                    double Qj;
-                    int Nj;
-                    par_unpack(QNj, &Qj, &Nj);
-                    A += Aj + (double)Nj * __builtin_fabs(s);
-                    Q += 2.0 * Qj + 2.0 * (double)Nj * s * s;
-                    s += sj;
-                    A += __builtin_fabs(s);
-                    Q += s * s;
-                    N += Nj;
+                // the carries of chunks c0 .. k (tail carries, then chunk k's
+                // head carry), up to 64 at a time one per lane, so that they
+                // arrive in one round trip; then merged in chunk order
+                double s = 0.0, A = 0.0, Q = 0.0;
+                int N = 0;
+                const int ncar = k - c0 + 1;
+                for (int b0 = 0; b0 < ncar; b0 += 64) {
+                    double cs = 0.0, cA = 0.0, cQN = 0.0;
+                    if (b0 + lane < ncar) {
+                        const int j = c0 + b0 + lane;
+                        get_carry(a, j < k ? 2 * j + 1 : 2 * k, &cs, &cA, &cQN, !intra);
+                    }
+                    const int nb = ncar - b0 < 64 ? ncar - b0 : 64;
+                    for (int i = 0; i < nb; ++i) {
+                        const double sj = readlane_f64(cs, i), Aj = readlane_f64(cA, i);
+                        double Qj;
+                        int Nj;
+                        par_unpack(readlane_f64(cQN, i), &Qj, &Nj);
+                        if (b0 + i == 0) {
+                            s = sj, A = Aj, Q = Qj, N = Nj;
+                            continue;
+                        }
+                        A += Aj + (double)Nj * __builtin_fabs(s);
+                        Q += 2.0 * Qj + 2.0 * (double)Nj * s * s;
+                        s += sj;
+                        A += __builtin_fabs(s);
+                        Q += s * s;
+                        N += Nj;
+                    }
                 }
                 const bool ok = par_ok(s, A, Q, N, a.parity_tol);
                 const int sp = LDSX ? 0 : (t == 0 ? pi1.w : pi1.z);  // speculative chain + 1
@@ -632,17 +697,25 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
         // split evenly over the waves (not by the chunks' row ranges, which are
         // very uneven on power-law matrices)
         if (a.has_empty) {
-            const int nw = (a.M + 31) >> 5;
-            const int w0 = (int)((long long)nw * c / a.nchunks), w1 = (int)((long long)nw * (c + 1) / a.nchunks);
-            for (int w = w0 + lane; w < w1; w += 64) {
-                unsigned bits = a.empty_bits[w];
-                const int rb = w << 5;
-                while (bits) {
-                    const int j = __ffs(bits) - 1;
-                    store_y<YT>(a.y, rb + j, 0.0);
-                    bits &= bits - 1;
+            // one row per lane, so that each store instruction covers 64
+            // consecutive rows (lanes 0-31 the rows of word w, lanes 32-63 those
+            // of word w + 1) instead of one scattered row per lane: the
+            // memory pipeline is the SpMV's bottleneck, and every store
+            // instruction queues in it
+            auto zero_pair = [&](int w, unsigned lo, unsigned hi) {
+                if (lo | hi) {
+                    const int r = (w << 5) + lane;
+                    if ((((lane < 32 ? lo : hi) >> (lane & 31)) & 1u) && r < a.M) store_y<YT>(a.y, r, 0.0);
                 }
-            }
+            };
+            // words w0 .. w0 + 63 from the register loaded at the chunk's start
+            // (zero past w1); more (chunks over 2,048 rows) loaded in a loop of
+            // their own, so that the common path has no load to wait for
+            const int nwc = w1 - w0;
+            for (int i = 0; i < nwc && i < 64; i += 2)
+                zero_pair(w0 + i, (unsigned)__builtin_amdgcn_readlane((int)ebits_pre, i),
+                          (unsigned)__builtin_amdgcn_readlane((int)ebits_pre, i + 1));
+            for (int w = w0 + 64; w < w1; w += 2) zero_pair(w, a.empty_bits[w], w + 1 < w1 ? a.empty_bits[w + 1] : 0u);
         }
         wave_sync_lds();  // the next chunk reuses this wave's slots
     };
