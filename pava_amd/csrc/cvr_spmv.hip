@@ -501,6 +501,24 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
 
         // split rows: sum slots in element order
         int *flist = &segbuf[wv][0];
+        // PAR: 2 sqrt(n q) of every split slot, by all lanes at once, before the
+        // merge: the correctly rounded square root is a long dependent
+        // instruction sequence, and the merge is a serial chain per group (on
+        // R-MAT scale 22 the longest group of a chunk has 27 segments on
+        // average). Kept in segbuf after flist's 128 entries.
+        double *troot = reinterpret_cast<double *>(&segbuf[wv][128]);
+        static_assert(128 + 2 * kSlotCap <= 64 * kStepGroup, "segbuf holds flist and troot");
+        if constexpr (PAR) {
+            // slots in use: the split table's sentinel (all kSlotCap beyond 64
+            // groups, which are rare)
+            const int nsl = ns <= 0 ? 0 : ns <= 64 ? __builtin_amdgcn_readlane(se_pre, ns - 1) : kSlotCap;
+            for (int i = lane; i < nsl; i += 64) {
+                double qi;
+                int ni;
+                par_unpack(slots_abs[wv][i], &qi, &ni);
+                troot[i] = 2.0 * __builtin_sqrt((double)ni * qi);
+            }
+        }
         if (PAR && lane == 0) fcnt[wv] = 0;
         wave_sync_lds();
         // one split group per lane: its slots merged in element order
@@ -510,13 +528,13 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                 double s = slots[wv][t.y], q0;
                 int n0;
                 par_unpack(slots_abs[wv][t.y], &q0, &n0);
-                double A = 2.0 * __builtin_sqrt((double)n0 * q0), Q = 2.0 * q0;
+                double A = troot[t.y], Q = 2.0 * q0;
                 int N = n0;
-                auto merge = [&](double sj, double pj) {
+                auto merge = [&](double sj, double pj, double tj) {
                     double qj;
                     int nj;
                     par_unpack(pj, &qj, &nj);
-                    A += 2.0 * __builtin_sqrt((double)nj * qj) + (double)nj * __builtin_fabs(s);
+                    A += tj + (double)nj * __builtin_fabs(s);
                     Q += 2.0 * (double)nj * s * s + 3.0 * qj;
                     s += sj;
                     A += __builtin_fabs(s);
@@ -527,13 +545,14 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                 // on LDS once per four segments (same order of operations)
                 int j = t.y + 1;
                 for (; j + 4 <= se; j += 4) {
-                    double sv[4], pv[4];
+                    double sv[4], pv[4], tv[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) sv[u] = slots[wv][j + u], pv[u] = slots_abs[wv][j + u];
+                    for (int u = 0; u < 4; ++u)
+                        sv[u] = slots[wv][j + u], pv[u] = slots_abs[wv][j + u], tv[u] = troot[j + u];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) merge(sv[u], pv[u]);
+                    for (int u = 0; u < 4; ++u) merge(sv[u], pv[u], tv[u]);
                 }
-                for (; j < se; ++j) merge(slots[wv][j], slots_abs[wv][j]);
+                for (; j < se; ++j) merge(slots[wv][j], slots_abs[wv][j], troot[j]);
                 if (t.x >= 0) {
                     if (par_ok(s, A, Q, N, a.parity_tol))
                         store_y<YT>(a.y, t.x, s);
