@@ -134,7 +134,8 @@ def test_parity_device_create_and_graph(P, O):
         h.close()
 
 
-def test_speculative_chains_both_outcomes_repeated_and_graphed(P, O):
+@pytest.mark.parametrize("hot", [False, True])
+def test_speculative_chains_both_outcomes_repeated_and_graphed(P, O, hot):
     """Rows of at least 4,096 elements spanning chunks have their serial sum
     started at the beginning of the SpMV (plain kernel, no LDS table), before
     their bound is known. Rows of positive terms pass the bound (the chain is
@@ -142,7 +143,9 @@ def test_speculative_chains_both_outcomes_repeated_and_graphed(P, O):
     chain's sum is used, whichever of the chain and the row's last chunk
     finishes first). Every call, with new x each time and under graph replay,
     equals the oracle's emulation bit for bit and recomputes the same rows:
-    the per-row state words are left cleared for the next call."""
+    the per-row state words are left cleared for the next call. With hot-x
+    staged (hot), the CSR copy the chains read carries the hot renumbering,
+    and their hot columns are read from the hot-x copy."""
     import torch
     rng = np.random.default_rng(31)
     n = 60000
@@ -164,9 +167,11 @@ def test_speculative_chains_both_outcomes_repeated_and_graphed(P, O):
                 part = float(np.dot(A.vals[b:e - 1], x0[A.cols[b:e - 1]]))
                 A.vals[e - 1] = -part / x0[A.cols[e - 1]] * (1 + 1e-9)
     S = 8
-    with P.CvrHip(A, steps_per_chunk=S, x_hot_kb=0) as h:
+    opts = dict(x_hot_kb=64, x_lds_kb=0) if hot else dict(x_hot_kb=0)
+    with P.CvrHip(A, steps_per_chunk=S, **opts) as h:
         inf = h.info()
         assert inf["n_lds_cols"] == 0 and inf["parity_tol"] > 0
+        assert (inf["n_hot_cols"] > 0) == hot
         assert inf["n_spec_rows"] == sum(1 for L in lens if L >= 4096), inf
         lay = O.cvr64_build(A, S, 0)
         dx = torch.empty(n, dtype=torch.float64, device="cuda")
