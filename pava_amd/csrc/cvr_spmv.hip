@@ -17,8 +17,9 @@
 // a neighbouring chunk goes to a per-chunk carry, which the fixup kernel adds
 // in chunk order (pava uses omp atomic +=, cvr/cvr.h:1413-1421, 1717, 1855;
 // here the SpMV is deterministic: the same inputs give bit-identical y).
-// Finally the wave zeroes the empty rows of its row range (pava leaves them
-// unwritten and relies on the caller's zeroing, main.cpp:561).
+// Finally the wave zeroes the empty rows of its share of the empty-row bitmap
+// (pava leaves them unwritten and relies on the caller's zeroing,
+// main.cpp:561).
 
 #include <hip/hip_runtime.h>
 
@@ -728,8 +729,9 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                 }
             };
             // words w0 .. w0 + 63 from the register loaded at the chunk's start
-            // (zero past w1); more (chunks over 2,048 rows) loaded in a loop of
-            // their own, so that the common path has no load to wait for
+            // (zero past w1); more (over 2,048 rows per chunk on average)
+            // loaded in a loop of their own, so that the common path has no
+            // load to wait for
             const int nwc = w1 - w0;
             for (int i = 0; i < nwc && i < 64; i += 2)
                 zero_pair(w0 + i, (unsigned)__builtin_amdgcn_readlane((int)ebits_pre, i),
