@@ -652,7 +652,8 @@ def test_unsorted_and_repeated_columns(P, O):
 def test_extreme_shapes(P, O):
     """One row of 3,000,000 elements (hundreds of chunks, thousands of
     recompute pieces), a single column of 100,000 rows, a million rows with
-    only the last one non-empty, and 1 x 1: FP64 equals the emulation bit for
+    only the last one non-empty, 200,000 rows of which every 1,000th is
+    non-empty, and 1 x 1: FP64 equals the emulation bit for
     bit (both column widths). With FP32 storage and a cancellation threshold
     of 1, every row stays within FP32 input rounding of the oracle, and the long
     row (random signs, so certainly flagged) is recomputed in FP64: the serial
@@ -667,6 +668,13 @@ def test_extreme_shapes(P, O):
         "last_row_only": P.CSR(1_000_000, 5000, np.concatenate([np.zeros(1_000_000, np.int32), [100]]).astype(np.int32),
                                np.sort(rng.choice(5000, 100, replace=False)).astype(np.int32), rng.uniform(-1, 1, 100)),
         "one_by_one": P.CSR(1, 1, np.array([0, 1], np.int32), np.array([0], np.int32), np.array([-2.5])),
+        # every 1,000th of 200,000 rows holds 90 elements: 36 chunks at the
+        # automatic size, each zeroing ~174 words of the empty-row bitmap (more
+        # than the 64 a wave loads with its chunk's first loads)
+        "mostly_empty": P.CSR(200_000, 5000,
+                              np.concatenate([[0], np.cumsum(np.where(np.arange(200_000) % 1000 == 0, 90, 0))]).astype(np.int32),
+                              np.concatenate([np.sort(rng.choice(5000, 90, replace=False)) for _ in range(200)]).astype(np.int32),
+                              rng.uniform(-1, 1, 200 * 90)),
     }
     for name, A in shapes.items():
         x = P.fill_uniform(A.numCols, 3)
