@@ -639,6 +639,12 @@ __global__ void remap_hot_kernel(int *__restrict__ cols, long long L, const int 
     }
 }
 
+hipError_t launch_remap_hot(int *cols, long long n, const int *colmap, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(remap_hot_kernel, dim3(2048), dim3(256), 0, st, cols, n, colmap);
+    return hipGetLastError();
+}
+
 // Selection of the max_hot most used columns (each used at least twice),
 // ordered by count descending, then column ascending. From the number of
 // columns per count value: every column with count >= t is taken, and of the

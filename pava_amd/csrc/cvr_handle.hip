@@ -68,6 +68,7 @@ struct cvr_hip_handle {
     int *d_wg_next = nullptr;       // LDS variant: per-workgroup pop counters + finished count
     // FP64 parity, plain kernel: speculative serial chains of the longest rows spanning chunks
     int n_spec = 0, spec_blocks = 0;
+    int ci_hot = 0;  // d_ci carries the hot renumbering (FP64 parity with hot-x staging)
     int *d_spec_rows = nullptr, *d_spec_state = nullptr;
     double *d_spec_val = nullptr;
     std::vector<void *> allocs;
@@ -463,6 +464,15 @@ static int create_core_body(int M, int N, int nnz, const int *d_rp, const int *d
         } else {
             e = cvr::launch_hot_columns(d_ci, nnz, N, (int)std::min<long long>(max_hot, N), auto_hot ? kAutoHotSkew : 0.0,
                                         cnt, colmap, h->d_hot_cols, (int *)h->d_cols, h->layout_elems, &nh, h->stream);
+            // FP64 parity: the serial recompute reads the handle's own copy of
+            // the CSR; its columns get the same renumbering, so that its
+            // gathers of hot columns read the LDS table or the hot-x copy, as
+            // the SpMV's do (R-MAT scale 22: 1.9M recomputed elements, ~85% on
+            // hot columns)
+            if (e == hipSuccess && nh > 0 && h->par && h->d_ci) {
+                e = cvr::launch_remap_hot(h->d_ci, nnz, colmap, h->stream);
+                if (e == hipSuccess) h->ci_hot = 1;
+            }
         }
         (void)hipStreamSynchronize(h->stream);
         dfree(cnt);
@@ -898,6 +908,7 @@ static cvr::SpmvArgs spmv_args(const cvr_hip_handle *h, const void *d_x, void *d
         a.rp = h->d_rp;
         a.ci = h->d_ci;
         a.va = h->d_va;
+        a.ci_hot = h->ci_hot;
         a.carry_bnd = h->d_carry_bnd;
         a.rc_count = h->d_rc_count;
         a.fix = h->d_fix;

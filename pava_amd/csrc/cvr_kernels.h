@@ -102,8 +102,9 @@ struct SpmvArgs {
     // FP64 serial parity (parity_tol > 0): the resident FP64 CSR, the bound
     // terms of each head / tail carry, and the rows recomputed per chunk
     const int *rp;
-    const int *ci;
+    const int *ci;  // with ci_hot: hot columns as kHotFlag | hot index (the layout's renumbering)
     const double *va;
+    int ci_hot;
     double2 *carry_bnd;  // 2 per chunk: (A, packed Q|N) of the head / tail carry
     int *rc_count;       // [c]: rows of chunk c recomputed in serial order by the last SpMV;
                          // [nchunks + c]: 1 if the row closed by chunk c was
@@ -309,6 +310,9 @@ hipError_t launch_long_rows(const LongArgs &a, hipStream_t st);
 // used column.
 hipError_t launch_hot_columns(const int *ci, long long nnz, int N, int max_hot, double min_skew, int *cnt, int *colmap,
                               int *hot_cols, int *cols, long long L, int *n_hot, hipStream_t st);
+// The hot renumbering of launch_hot_columns (its colmap, still allocated)
+// applied to more column indices: the CSR copy kept for the parity recompute.
+hipError_t launch_remap_hot(int *cols, long long n, const int *colmap, hipStream_t st);
 hipError_t launch_hot_gather(const void *x, const int *hot_cols, void *xhot, int n_hot, int xsize, hipStream_t st);
 
 }  // namespace cvr

@@ -92,6 +92,53 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// x of a column of the FP64 CSR copy kept for the parity recompute. With hot
+// staging (hot = 1) the copy's columns carry the layout's renumbering
+// (kHotFlag | hot index, applied at handle creation), so that the recompute
+// reads a hot column where the SpMV's own gathers do -- the LDS table (TAB:
+// the LDS variant, hot numbers below n_lds) or the hot-x copy, whose lines
+// the SpMV keeps in L2 -- instead of lines of x that only the recompute
+// touches; the values are the same. A table column still issues its global
+// load, to the hot-x copy's first element, which all such lanes of an
+// instruction share: one line request instead of one per lane. Nothing is
+// loaded under a branch (the compiler then waits for every load in flight at
+// the join) and bases are chosen by a bit mask (a select of two pointers was
+// lowered to an indexed scratch load). TAB: get2 returns both values and the
+// choice, which the chain makes where it uses the value (a select right
+// after the loads waits for them there). R-MAT scale 22 FP64 with parity:
+// 0.4167 -> 0.4141 ms per step (five alternating timings).
+template <bool TAB>
+struct XSrcT {
+    static constexpr bool kTab = TAB;
+    uint64_t ax, ah;    // x, the hot-x copy
+    const double *tab;  // TAB: the LDS table
+    int n_lds, hot;
+    __device__ __forceinline__ void get2(int cv, double &g, double &t, bool &in_tab) const {
+        if (!hot) {
+            g = load_global<double>(ax + (uint64_t)(unsigned)cv * sizeof(double));
+            t = 0.0;
+            in_tab = false;
+            return;
+        }
+        static_assert(kHotFlag == (1 << 30), "bit 30 flags a hot column");
+        const uint32_t idx = (uint32_t)(cv & kColMask);
+        const uint32_t h = ((unsigned)cv >> 30) & 1u;
+        const bool in = h && idx < (uint32_t)n_lds;
+        t = tab[in ? idx : 0u];
+        const uint64_t m = 0ull - (uint64_t)h;  // all ones for a hot column
+        const uint64_t base = ax ^ ((ax ^ ah) & m);
+        g = load_global<double>(base + (in ? 0ull : (uint64_t)idx * sizeof(double)));
+        in_tab = in;
+    }
+    // !TAB: the value itself
+    __device__ __forceinline__ double operator()(int cv) const {
+        static_assert(!TAB, "the LDS variant uses get2");
+        if (!hot) return load_global<double>(ax + (uint64_t)(unsigned)cv * sizeof(double));
+        const uint64_t m = 0ull - (uint64_t)(((unsigned)cv >> 30) & 1u);
+        return load_global<double>((ax ^ ((ax ^ ah) & m)) + (uint64_t)(uint32_t)(cv & kColMask) * sizeof(double));
+    }
+};
+
 // The serial FMA sum of elements [b, e) of the FP64 CSR (the reference's
 // order), by one wave. A round covers 64 * K consecutive elements, K per lane
 // (lane l holds elements j0 + K l .. j0 + K l + K - 1). The chain runs lane
@@ -105,14 +152,17 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 // result is wave-uniform.
 // POLL: `stop` is read once per round (the load overlaps the round's chain)
 // and the chain ends early, its result unused, once kSpecOk is set there.
-template <int K, int D, bool POLL = false>
+template <int K, int D, bool POLL = false, typename XS>
 __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, const double *__restrict__ va,
-                                                 const double *__restrict__ x, int b, int e, int lane,
-                                                 const int *stop = nullptr) {
+                                                 const XS &xs, int b, int e, int lane, const int *stop = nullptr) {
     constexpr int R = 64 * K;
     double s = 0.0;
     int cn[K];                         // columns of round r + D + 1
     double vq[D + 1][K], xq[D + 1][K];  // values and x of rounds r .. r + D
+    // TAB: the LDS table's values of rounds r .. r + D and whether each
+    // element reads the table (XSrcT::get2)
+    double tq[XS::kTab ? D + 1 : 1][K];
+    bool iq[XS::kTab ? D + 1 : 1][K];
     auto at = [&](int j0, int k) -> unsigned {
         const int j = j0 + lane * K + k;
         return (unsigned)(j < e ? j : e - 1);
@@ -125,9 +175,14 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
 #pragma unroll
         for (int k = 0; k < K; ++k) v[k] = __builtin_nontemporal_load(&va[at(j0, k)]);
     };
-    auto gather = [&](const int (&c)[K], double (&xo)[K]) {
+    auto gather = [&](const int (&c)[K], int d) {
+        if constexpr (XS::kTab) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) xo[k] = x[(unsigned)c[k]];
+            for (int k = 0; k < K; ++k) xs.get2(c[k], xq[d][k], tq[d][k], iq[d][k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) xq[d][k] = xs(c[k]);
+        }
     };
     // prologue: rounds 0 .. D loaded and gathered, the columns of round D + 1 loading
     static_assert(D >= 1 && D <= 3, "pipeline depth");
@@ -135,7 +190,7 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
     for (int d = 0; d <= D; ++d) {
         load_c(b + d * R, cn);
         load_v(b + d * R, vq[d]);
-        gather(cn, xq[d]);
+        gather(cn, d);
     }
     load_c(b + (D + 1) * R, cn);
     for (int j0 = b; j0 < e; j0 += R) {
@@ -148,7 +203,10 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
         for (int k = 0; k < K; ++k) {
             const bool in = j0 + lane * K + k < e;
             vv[k] = in ? vq[0][k] : 0.0;
-            xx[k] = in ? xq[0][k] : 0.0;
+            if constexpr (XS::kTab)
+                xx[k] = in ? (iq[0][k] ? tq[0][k] : xq[0][k]) : 0.0;
+            else
+                xx[k] = in ? xq[0][k] : 0.0;
         }
         // a full round with the lane loop unrolled (no loop control between
         // the hops of the chain), a partial last round rolled
@@ -176,8 +234,12 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
                 for (int k = 0; k < K; ++k) {
                     xq[d][k] = xq[d + 1][k];
                     vq[d][k] = vq[d + 1][k];
+                    if constexpr (XS::kTab) {
+                        tq[d][k] = tq[d + 1][k];
+                        iq[d][k] = iq[d + 1][k];
+                    }
                 }
-            gather(cn, xq[D]);
+            gather(cn, D);
             load_v(j0 + (D + 1) * R, vq[D]);
             load_c(j0 + (D + 2) * R, cn);
         }
@@ -191,9 +253,8 @@ __device__ __forceinline__ double serial_fma_sum(const int *__restrict__ ci, con
 // budget.
 template <int K, int D, bool POLL = false>
 __device__ __noinline__ double serial_fma_sum_ool(const int *__restrict__ ci, const double *__restrict__ va,
-                                                  const double *__restrict__ x, int b, int e, int lane,
-                                                  const int *stop = nullptr) {
-    return serial_fma_sum<K, D, POLL>(ci, va, x, b, e, lane, stop);
+                                                  const XSrcT<false> xs, int b, int e, int lane, const int *stop = nullptr) {
+    return serial_fma_sum<K, D, POLL>(ci, va, xs, b, e, lane, stop);
 }
 
 // The serial FMA sum of CSR elements [b, e) for the parity recompute. The LDS
@@ -205,8 +266,7 @@ __device__ __noinline__ double serial_fma_sum_ool(const int *__restrict__ ci, co
 // (selector-family matrix with a 20,000-element cancelling row spanning
 // chunks: 0.270 -> 0.229 ms; the 27-point stencil unchanged).
 template <bool LDSX, typename ARGS>
-__device__ __forceinline__ double serial_chain(const ARGS &a, int b, int e, int lane) {
-    const double *xs = reinterpret_cast<const double *>(a.x);
+__device__ __forceinline__ double serial_chain(const ARGS &a, const XSrcT<LDSX> &xs, int b, int e, int lane) {
     if constexpr (LDSX)
         return serial_fma_sum<8, 1>(a.ci, a.va, xs, b, e, lane);
     else if (e - b > 2048)
@@ -305,6 +365,11 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
     __shared__ int lcnt[PAR && !LDSX ? WPB : 1];
     extern __shared__ __align__(16) unsigned char lds_tab[];
     const XT *__restrict__ tab = reinterpret_cast<const XT *>(lds_tab);
+    // PAR: where the serial recompute reads x
+    XSrcT<LDSX> xs{};
+    if constexpr (PAR)
+        xs = XSrcT<LDSX>{reinterpret_cast<uint64_t>(a.x), reinterpret_cast<uint64_t>(a.xhot),
+                  LDSX ? reinterpret_cast<const double *>(lds_tab) : nullptr, LDSX ? a.n_lds : 0, a.ci_hot};
 
     auto run_chunk = [&](const int c) {
         const long long C = 64LL * a.S;
@@ -605,7 +670,7 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
             for (int f = 0; f < nf; ++f) {
                 const int row = __builtin_amdgcn_readfirstlane(flist[f]);
                 const int b = a.rp[row], e = a.rp[row + 1];
-                const double v = serial_chain<LDSX>(a, b, e, lane);
+                const double v = serial_chain<LDSX>(a, xs, b, e, lane);
                 if (lane == 0) store_y<YT>(a.y, row, v);
             }
             if (lane == 0) a.rc_count[c] = nf;
@@ -703,7 +768,7 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                         }
                     }
                 } else {
-                    if (!ok) s = serial_chain<LDSX>(a, a.rp[row], a.rp[row + 1], lane);
+                    if (!ok) s = serial_chain<LDSX>(a, xs, a.rp[row], a.rp[row + 1], lane);
                     if (lane == 0) store_y<YT>(a.y, row, s);
                 }
                 if (lane == 0) {
@@ -801,8 +866,7 @@ __global__ __launch_bounds__(LDSX ? 64 * kWavesLds : 64 * kWavesPerBlock,
                 if (wv == 0 && i < a.n_spec) {
                     const int row = a.spec_rows[i];
                     int *stp = a.spec_state + i;
-                    const double v = serial_fma_sum_ool<8, 1, true>(
-                        a.ci, a.va, reinterpret_cast<const double *>(a.x), a.rp[row], a.rp[row + 1], lane, stp);
+                    const double v = serial_fma_sum_ool<8, 1, true>(a.ci, a.va, xs, a.rp[row], a.rp[row + 1], lane, stp);
                     if (lane == 0) {
                         __hip_atomic_store(&a.spec_val[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         const int old = __hip_atomic_fetch_or(stp, kSpecDone, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -1140,6 +1204,8 @@ hipError_t launch_spmv(const SpmvArgs &a, int prec, int col16, hipStream_t st) {
     }
     if (a.n_lds > 0 && (col16 || a.lds_blocks <= 0 || !a.wg_next || !a.wg_list || !a.wg_start))
         return hipErrorInvalidValue;
+    // a renumbered CSR copy needs the hot-x copy its hot columns point into
+    if (a.parity_tol > 0.0 && a.ci_hot && (!a.xhot || col16)) return hipErrorInvalidValue;
     if (a.n_spec > 0 && (a.n_lds > 0 || a.parity_tol <= 0.0 || prec != 0 || !a.spec_rows || !a.spec_state ||
                          !a.spec_val || a.spec_blocks < a.n_spec || a.spec_blocks % 8))
         return hipErrorInvalidValue;
